@@ -1,0 +1,167 @@
+"""Benchmark of the fused one-turn map on one MI355X.
+
+Workload (the reference's "full plain one-turn map on GPU, FP64"): protons at
+gamma = 27.7 on a ring of C = 6911.5 m, 1e6 macroparticles, intensity
+1.5e11; one smooth-optics transverse segment (Q = 20.13 / 20.18,
+beta = 54.6 m) with chromaticity Q' = 5 and amplitude detuning
+app_x = app_y = 1e-10, app_xy = -5e-11 per m (times p0); non-linear
+RFSystems h = 4620, V = 4.5 MV; UniformBinSlicer 500 slices over +-3 sigma_z;
+CircularResonator R_s = 1e6 Ohm/m, f_r = 1 GHz, Q = 1, one turn of wake
+memory.  A step is one turn of the whole bunch; value = macroparticle-turns/s
+with the bunch resident in device memory.
+
+The roofline entry is that of the dominant kernel (the fused particle pass):
+96 algorithmic bytes per macroparticle (read and write of six FP64
+coordinates) divided by its mean duration, measured with HIP events on its
+launch stream.  The CPU baseline is the oracle's NumPy restatement of the
+reference CPU path on one host core, timed on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 50]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_MP_PASS = 96
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--n", type=int, default=1000000)
+    ap.add_argument("--slices", type=int, default=500)
+    ap.add_argument("--cpu-turns", type=int, default=40)
+    ap.add_argument("--cpu-n", type=int, default=1000000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stage-turns", type=int, default=100)
+    ap.add_argument("--graph-turns", type=int, default=10)
+    return ap.parse_args()
+
+
+def cpu_baseline(n, turns):
+    """The oracle on one host core (runs before the GPU is touched)."""
+    import numpy as np
+    try:
+        old = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {sorted(old)[0]})
+    except (AttributeError, OSError):
+        old = None
+    try:
+        from oracle import workloads as W
+        ob = W.nonlinear_bunch(n, seed=12345)
+        m = W.nonlinear_map_oracle()
+        for el in m:          # one untimed turn (imports, caches)
+            el.track(ob)
+        t0 = time.perf_counter()
+        for _ in range(turns):
+            for el in m:
+                el.track(ob)
+        dt = time.perf_counter() - t0
+    finally:
+        if old is not None:
+            os.sched_setaffinity(0, old)
+    return dict(value=n * turns / dt, unit="macroparticle-turns/s", cores=1, kind="port",
+                sample="%d macroparticles x %d turns of the same one-turn map "
+                       "(NumPy restatement of the reference CPU path, 1 core, %.1f s)"
+                       % (n, turns, dt))
+
+
+def build_machine(p0_ref):
+    import numpy as np
+    from scipy.constants import e, m_p
+    from pyheadtail_amd.machines.synchrotron import Synchrotron
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.impedances.wakes import WakeField, CircularResonator
+    mach = Synchrotron(optics_mode='smooth', charge=e, mass=m_p, gamma=27.7,
+                       circumference=6911.5, n_segments=1, beta_x=54.6, beta_y=54.6,
+                       D_x=0., D_y=0., accQ_x=20.13, accQ_y=20.18, Qp_x=5., Qp_y=5.,
+                       app_x=1e-10 * p0_ref, app_y=1e-10 * p0_ref, app_xy=-5e-11 * p0_ref,
+                       longitudinal_mode='non-linear', alpha_mom_compaction=1.9e-3,
+                       h_RF=4620, V_RF=4.5e6, dphi_RF=0., p_increment=0.)
+    return mach
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(independent bunches would shard across GPUs without communication). "
+              "Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_turns)
+
+    import numpy as np
+    import torch
+    torch.cuda.set_device(0)
+    from scipy.constants import c
+    from pyheadtail_amd.engine import FusedOneTurnMap
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.impedances.wakes import WakeField, CircularResonator
+    from oracle import workloads as W
+    from tests.machines import to_product
+
+    p0 = W.p0_of(W.GAMMA)
+    mach = build_machine(p0)
+    wake = WakeField(UniformBinSlicer(args.slices, n_sigma_z=3), CircularResonator(1e6, 1e9, 1))
+    one_turn = list(mach.one_turn_map) + [wake]
+    bunch = to_product(W.nonlinear_bunch(args.n, seed=12345))
+    eng = FusedOneTurnMap(one_turn, graph_turns=args.graph_turns)
+
+    # warm-up (graph capture happens here)
+    eng.track_turns(bunch, max(args.warmup, args.graph_turns))
+    bunch.x  # applies the pending kick
+    torch.cuda.synchronize()
+
+    # per-stage kernel durations with HIP events on the launch stream
+    stage_ms = eng.timed_turns(bunch, args.stage_turns)
+    bunch.x
+    torch.cuda.synchronize()
+
+    # timed region: K turns, graph-launched, then the final kick
+    t0 = time.perf_counter()
+    eng.track_turns(bunch, args.steps)
+    bunch.x
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+
+    value = args.n * args.steps / dt
+    achieved = BYTES_PER_MP_PASS * args.n / (stage_ms[0] * 1e-3) / 1e9
+    out = {
+        "metric": "macroparticle-turns/s @1e6 MP+wake; %HBM roofline; batched-scan p99 latency",
+        "value": value,
+        "unit": "macroparticle-turns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded 6D Gaussian bunch, separatrix-cut)",
+        "config": {"workload": "full plain one-turn map, FP64: %d macroparticles, RFSystems "
+                               "h=4620 V=4.5MV, transverse Q'=5 + amplitude detuning, %d slices "
+                               "+-3 sigma_z, broadband resonator wake (1 turn)"
+                               % (args.n, args.slices),
+                   "macroparticles": args.n, "n_slices": args.slices, "turns": args.steps,
+                   "graph_turns": args.graph_turns},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_engine_pass (fused particle pass)",
+                     "algorithmic_bytes_per_launch": BYTES_PER_MP_PASS * args.n},
+        "stage_ms": {"pass": stage_ms[0], "hist": stage_ms[1], "conv": stage_ms[2]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,434 @@
+/*
+ * phx.h - C ABI of the MI355X (gfx950) macroparticle one-turn-map library.
+ *
+ * The library replaces the per-turn hot path of PyHEADTAIL 1.16.5: the
+ * transverse segment map with detuning, the longitudinal maps, uniform-bin
+ * slicing with per-slice moments, and the wake-field convolution and kick.
+ * It is the analogue of the reference's ctypes boundary to its Thrust
+ * library (PyHEADTAIL/gpu/thrust_interface.py:5-269 <-> gpu/thrust_code.cu),
+ * redesigned:
+ *   - every function returns int (PHX_OK = 0, a negative PHX_E_* code, or a
+ *     positive hipError_t value); phx_last_error() gives a message;
+ *   - every buffer is owned by the caller (device memory), every call is
+ *     asynchronous on the given stream (a hipStream_t passed as void*; NULL is
+ *     the default stream), and no call allocates device memory except
+ *     phx_graph_* (graph objects) and phx_init;
+ *   - counts are int64_t (no 2^31 limit on particle numbers);
+ *   - parameter records are plain C structs passed by host pointer and copied
+ *     into the launch, so the caller may reuse them immediately.
+ *
+ * Coordinates are the six FP64 structure-of-arrays columns of a Particles
+ * object (x, xp, y, yp, z, dp; particles/particles.py:16-229).
+ */
+#ifndef PHX_H
+#define PHX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PHX_ABI_VERSION 1
+
+enum {
+    PHX_OK = 0,
+    PHX_E_INVALID = -1,   /* bad argument (null pointer, bad size, bad enum) */
+    PHX_E_LIMIT = -2,     /* a compile-time limit below was exceeded */
+    PHX_E_NODEVICE = -3,  /* no HIP device or phx_init not called */
+    PHX_E_STATE = -4      /* inconsistent device state (e.g. empty wake history) */
+};
+
+#define PHX_MAX_DETUNERS 4       /* detuners per transverse segment */
+#define PHX_MAX_CHROMA 8         /* chromaticity polynomial order */
+#define PHX_MAX_LONG_OPS 10      /* drifts + kicks in one longitudinal sequence */
+#define PHX_MAX_KICKS 8          /* wake kicks in one wake field */
+#define PHX_MAX_TERMS 8          /* per-slice accumulated terms in one pass */
+#define PHX_MAX_PROGRAM 16       /* local elements fused into one particle pass */
+#define PHX_MAX_STATS 16         /* per-turn bunch statistics recorded by a pass */
+
+typedef void *phx_stream_t;
+
+/* ------------------------------------------------------------------------ */
+/* Library state                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* Message of the last failing call on this host thread. */
+const char *phx_last_error(void);
+/* ABI version (PHX_ABI_VERSION). */
+int phx_abi_version(void);
+/* Make `device` current for the calling thread and query its properties.  */
+int phx_init(int device);
+/* sizeof of a public record by name ("phx_tseg", "phx_turn_program", ...),
+   -1 if unknown: lets bindings check their struct layouts. */
+int64_t phx_sizeof(const char *type_name);
+/* Bytes of workspace that the reduction and slicing calls need for n
+   particles and up to n_slices slices.  The workspace must be zero-filled
+   once when allocated; every call leaves it zero-filled again. */
+int64_t phx_workspace_bytes(int64_t n, int32_t n_slices);
+
+/* ------------------------------------------------------------------------ */
+/* Bunch statistics  (Particles.mean_* / sigma_* / epsn_*:                  */
+/* particles/particles.py:311-366; general/pmath.py:183-185;                */
+/* cobra_functions/stats.pyx:17-122)                                        */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Reduces k <= 6 device arrays a_0..a_{k-1} of length n in one pass and
+ * writes to the device array `out` (length phx_bunch_sums_len(k)):
+ *   out[0      .. k)      sum a_i                       (for np.mean)
+ *   out[k      .. 2k)     sum (a_i - a_i[0])            (shifted sums of cov)
+ *   out[2k     .. 2k+P)   sum (a_i - a_i[0])(a_j - a_j[0]), i <= j, row-major
+ *   out[2k+P   .. 3k+P)   min a_i
+ *   out[3k+P   .. 4k+P)   max a_i
+ *   out[4k+P   .. 5k+P)   max |a_i|
+ * with P = k(k+1)/2.  The reduction order is fixed (chunks of PHX chunk size,
+ * then chunk order), so the result is bit-reproducible for a given n.
+ * Replaces np.mean, cobra_functions/stats.pyx:cov (shift = first element)
+ * and np.min/np.max used by Slicer.get_long_cuts (particles/slicing.py:395).
+ */
+int64_t phx_bunch_sums_len(int32_t k);
+int phx_bunch_sums(const double *const *arrays, int32_t k, int64_t n,
+                   double *out, void *ws, phx_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Uniform-bin slicing (particles/slicing.py:532-615, 58-317;              */
+/* cobra_functions/stats.pyx:192-425)                                      */
+/* ------------------------------------------------------------------------ */
+
+enum { PHX_CUTS_FIXED = 0, PHX_CUTS_NSIGMA = 1, PHX_CUTS_MINMAX = 2 };
+
+typedef struct {
+    int32_t n_slices;
+    int32_t cuts_mode;        /* PHX_CUTS_* */
+    double n_sigma_z;         /* PHX_CUTS_NSIGMA */
+    double z_cut_tail;        /* PHX_CUTS_FIXED */
+    double z_cut_head;        /* PHX_CUTS_FIXED */
+} phx_slicer;
+
+/* Device-resident result of the cut computation of one slicing. */
+typedef struct {
+    double tail;              /* z_cut_tail */
+    double head;              /* z_cut_head */
+    double width;             /* (head - tail) / n_slices */
+    double mean_z;            /* np.mean(z)   (PHX_CUTS_NSIGMA only) */
+    double sigma_z;           /* cov(z, z)^.5 (PHX_CUTS_NSIGMA only) */
+    double absmax[6];         /* max |coordinate| of x, xp, y, yp, z, dp */
+    int64_t n;                /* particles in the bunch */
+    int32_t n_slices;
+    int32_t turn;             /* turn index of the fused engine */
+    int32_t valid;
+    int32_t pad;
+} phx_slice_state;
+
+/*
+ * Computes the slicing window of UniformBinSlicer.compute_sliceset_kwargs
+ * (particles/slicing.py:599-615 with get_long_cuts :395-413) into the
+ * device record *state, and z_bins (n_slices + 1 FP64, device) restating
+ * np.arange(tail, head + 1e-7*width, width) (general/pmath.py:213-216).
+ * Also records max|u| of the six coordinates (null pointers give 0), which
+ * fixes the fixed-point scales of phx_slice_accumulate.
+ */
+int phx_slice_cuts(const double *const coords[6], int64_t n,
+                   const phx_slicer *cfg, phx_slice_state *state,
+                   double *z_bins, void *ws, phx_stream_t stream);
+
+/* One accumulated per-slice term: sum over particles of slice s of
+ *    (u_a - c_a[s]) * (u_b - c_b[s])      (b < 0: the single factor u_a - c_a[s])
+ * where c_a / c_b are per-slice centres (null: 0).  coord indices 0..5 are
+ * x, xp, y, yp, z, dp. */
+typedef struct {
+    int32_t a, b;
+    const double *center_a;
+    const double *center_b;
+} phx_term;
+
+/*
+ * One pass over the particles of the slicing described by *state:
+ *   idx = int32(floor((z - tail) / width))   (particles/slicing.py:609-611)
+ * or, when idx_in is not null, the stored slice_index_of_particle (as the
+ * reference's per-slice statistics use, stats.pyx:235-425); idx is
+ * written to idx_out when not null (x86 conversion semantics: NaN and
+ * out-of-range give INT32_MIN), counts[s] = number of particles with
+ * 0 <= idx < n_slices (stats.pyx:192-202), and for each term t the exact
+ * per-slice sum into sums[t*n_slices + s].  Sums use split 64-bit
+ * fixed-point accumulators scaled from max|u|, so they are independent of
+ * particle order and thread schedule (bit-reproducible).  When means is not
+ * null, means[t*n_slices+s] = sums/counts for counts > 0, else 0
+ * (mean_per_slice, stats.pyx:235-253).
+ */
+int phx_slice_accumulate(const double *const coords[6], int64_t n,
+                         const phx_slice_state *state, int32_t n_slices,
+                         const phx_term *terms, int32_t n_terms,
+                         const int32_t *idx_in, int32_t *idx_out,
+                         int32_t *counts,
+                         double *sums, double *means,
+                         void *ws, phx_stream_t stream);
+
+/* Per-slice statistics from centred second-moment sums (device arrays of
+ * length n_slices):  kind 0: std  = sqrt(c_aa/(n-1))        (stats.pyx:258-286)
+ *                    kind 1: cov  = c_ab/(n-1)              (stats.pyx:290-319)
+ *                    kind 2: emittance from 3 (dp==null) or 6 sums
+ *                            (stats.pyx:374-425)
+ * sums holds the needed centred sums in the order
+ * (uu, uup, upup[, udp, updp, dpdp]).  `factor` multiplies the result
+ * The result is ((f_pre * r) * f_mul) / f_div: (1, betagamma, 1) for
+ * epsn_x/y and (4*pi, p0, e) for epsn_z (particles/slicing.py:502-518). */
+int phx_slice_finalize(int32_t kind, int32_t n_slices, const int32_t *counts,
+                       const double *sums, int32_t n_sums, double f_pre,
+                       double f_mul, double f_div, double *out,
+                       phx_stream_t stream);
+
+/* z_centers and times of a slicing: z_centers = z_bins[:-1] + .5*diff(z_bins)
+ * (particles/slicing.py:124-126), times = z_centers / (beta*c) with
+ * beta_c = beta * c evaluated by the caller (:303-307). */
+int phx_slice_times(const double *z_bins, int32_t n_slices, double beta_c,
+                    double *z_centers, double *times, phx_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Transverse segment map (trackers/transverse_tracking.py:17-196,         */
+/* trackers/detuners.py:48-151)                                            */
+/* ------------------------------------------------------------------------ */
+
+enum { PHX_DET_CHROMA = 1, PHX_DET_AMPLITUDE = 2 };
+
+typedef struct {
+    int32_t type;             /* PHX_DET_* */
+    int32_t order;            /* chromaticity: number of coefficients */
+    /* chromaticity: c_i = Qp_i * dmu / (i+1)!  (detuners.py:83) */
+    double cx[PHX_MAX_CHROMA];
+    double cy[PHX_MAX_CHROMA];
+    /* amplitude detuning (detuners.py:139-151); host-evaluated factors
+       gx = (1 + alpha_x^2)/beta_x, ax2 = 2*alpha_x, bx = beta_x, same in y */
+    double gx, ax2, bx, gy, ay2, by;
+    double dapp_x, dapp_y, dapp_xy, dapp_yx;
+    double p0;                /* beam.p0 at track time */
+} phx_detuner;
+
+typedef struct {
+    /* I and J of _build_segment_map (transverse_tracking.py:71-103) */
+    double I00, I01, I10, I11, I22, I23, I32, I33;
+    double J00, J01, J10, J11, J22, J23, J32, J33;
+    /* M = I cos + J sin evaluated on the host when there is no detuner
+       (transverse_tracking.py:178-193) */
+    double M00, M01, M10, M11, M22, M23, M32, M33;
+    double dQ_x, dQ_y;
+    double D_x_s0, D_x_s1, D_y_s0, D_y_s1;
+    int32_t with_dispersion;  /* _track_with_dispersion (:105-125) */
+    int32_t n_detuners;
+    phx_detuner det[PHX_MAX_DETUNERS];
+} phx_tseg;
+
+/* TransverseSegmentMap.track (transverse_tracking.py:138-196). */
+int phx_transverse_segment(double *x, double *xp, double *y, double *yp,
+                           const double *dp, int64_t n, const phx_tseg *p,
+                           phx_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Longitudinal maps (trackers/longitudinal_tracking.py)                   */
+/* ------------------------------------------------------------------------ */
+
+enum { PHX_LONG_DRIFT = 1, PHX_LONG_KICK = 2 };
+
+typedef struct {
+    int32_t type;             /* PHX_LONG_* */
+    int32_t with_dispersion;  /* Kick.track_with_dispersion (:185-193) */
+    /* drift: z <- beta_ratio*z - (eta0*dp)*length       (:107-111, :62-81) */
+    double beta_ratio, eta0, length;
+    /* kick: phi = (h*((2pi*z)/C) + phi_offset) + phi_lock
+             dp <- (dp*p0_before + ((amplitude*sin(phi)) - p_increment))
+                   / p0_after                               (:195-203) */
+    double harmonic, two_pi, circumference, phi_offset, phi_lock;
+    double amplitude, p_increment, p0_before, p0_after;
+    double D_x, D_y;
+} phx_long_op;
+
+typedef struct {
+    int32_t n_ops;
+    int32_t shrink;           /* RFSystems._shrink_transverse_emittance */
+    double shrink_factor;     /* sqrt(betagamma_old/betagamma) (:556-565) */
+    phx_long_op op[PHX_MAX_LONG_OPS];
+} phx_long_seq;
+
+/* Drift.track, Kick.track and RFSystems.track (drift, kicks, drift;
+   longitudinal_tracking.py:84-203, 362-369, 556-574) as one sequence. */
+int phx_longitudinal(double *x, double *xp, double *y, double *yp,
+                     double *z, double *dp, int64_t n, const phx_long_seq *p,
+                     phx_stream_t stream);
+
+typedef struct {
+    double cosQ, sinQ, longfac;   /* host-evaluated as in :743-757 */
+    double D_x, D_y;
+} phx_linmap;
+
+/* LinearMap.track -> track_with_dispersion (longitudinal_tracking.py:
+   720-759). */
+int phx_linear_map(double *x, double *y, double *z, double *dp, int64_t n,
+                   const phx_linmap *p, phx_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Wake fields (impedances/wakes.py:77-145, 379-678;                        */
+/* impedances/wake_kicks.py:19-361)                                         */
+/* ------------------------------------------------------------------------ */
+
+enum {
+    PHX_WAKE_RES_TRANSVERSE = 1,  /* Resonator.function_transverse (:450-470) */
+    PHX_WAKE_RES_LONGITUDINAL = 2,/* Resonator.function_longitudinal (:472-493) */
+    PHX_WAKE_RESISTIVE_WALL = 3   /* ResistiveWall.function_transverse (:611-624) */
+};
+enum { PHX_MOM_N = 0, PHX_MOM_X = 1, PHX_MOM_Y = 2 };
+enum { PHX_PLANE_XP = 0, PHX_PLANE_YP = 1, PHX_PLANE_DP = 2 };
+
+typedef struct {
+    int32_t fn;               /* PHX_WAKE_* */
+    int32_t moment;           /* PHX_MOM_*: source moment n, n*mean_x, n*mean_y */
+    int32_t plane;            /* PHX_PLANE_*: momentum that receives the kick */
+    int32_t multiply;         /* -1: += k[idx];  0: += k[idx]*x;  2: += k[idx]*y */
+    int32_t n_turns_wake;
+    int32_t dot_product;      /* 1: _convolution_dot_product, 0: np.convolve */
+    /* resonator: branch by Q (>0.5, ==0.5, <0.5); prefix = host-evaluated
+       Yokoya*R*omega^2/(Q*omegabar) (transverse) or R (longitudinal) */
+    double Q, prefix, alpha, omegabar, alpha_over_omegabar;
+    /* resistive wall */
+    double yokoya, L, b3, dt_min_abs, k_sigma, sqrt_z0c;
+    double wake_factor;       /* WakeKick._wake_factor of the current bunch */
+} phx_wake_kick;
+
+/* Multi-turn history of slice data (impedances/wakes.py:116-145): a ring of
+   `capacity` slots; slot (head - i) mod capacity is the SliceSet i turns old.
+   All pointers are device memory.  The record itself lives in device memory
+   so that the fused engine can advance it without host round trips. */
+typedef struct {
+    int32_t n_slices;
+    int32_t capacity;
+    int32_t n_filled;
+    int32_t head;
+    double *times;            /* [capacity][n_slices] z_centers/(beta c) */
+    double *mom;              /* [capacity][3][n_slices] n, n*mean_x, n*mean_y */
+    double *age;              /* [capacity] */
+    double *beta;             /* [capacity] SliceSet.beta of each entry */
+} phx_wake_hist;
+
+/* Pushes the slice data of a new SliceSet into the device history *hist:
+   ages += age_increment (C/(beta c) of the current bunch, wakes.py:135-137),
+   then appendleft (times, counts, mean_x, mean_y) with age 0 (:141-142). */
+int phx_wake_hist_push(phx_wake_hist *hist, const double *times,
+                       const int32_t *counts, const double *mean_x,
+                       const double *mean_y, double age_increment,
+                       double beta, phx_stream_t stream);
+
+/* Wake signals of n_kicks kicks into kicks_out[k*n_slices + j]:
+   wake_factor * sum over the stored turns of the convolution of the source
+   moments with the wake function (wake_kicks.py:75-139), in FP64 with the
+   reference's arithmetic for the wake functions.  Kick k uses
+   min(n_filled, n_turns_wake, max_turns) stored turns; dot_product selects
+   _convolution_dot_product (dt[i,j] = t_tgt[j] - (t_src[i] + age)) instead
+   of the np.convolve 'valid' form.  ws: phx_wake_workspace_bytes bytes,
+   zero-filled once (calls leave it zero-filled). */
+int64_t phx_wake_workspace_bytes(int32_t n_kicks, int32_t max_turns,
+                                 int32_t n_slices);
+int phx_wake_convolve(const phx_wake_hist *hist, const phx_wake_kick *kicks,
+                      int32_t n_kicks, int32_t n_slices, int32_t max_turns,
+                      double *kicks_out, void *ws, phx_stream_t stream);
+
+/* Applies the kicks to the particles inside the cuts, in list order:
+   p[idx] += k[idx] (* x or * y)   (wake_kicks.py:160-361).  The slice index
+   comes from idx when not null, else it is recomputed from z and *state
+   (bit-identical). */
+int phx_wake_apply(double *const coords[6], int64_t n, const int32_t *idx,
+                   const phx_slice_state *state, const phx_wake_kick *kicks,
+                   int32_t n_kicks, int32_t n_slices, const double *kicks_in,
+                   phx_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Fused one-turn map                                                       */
+/* ------------------------------------------------------------------------ */
+
+enum { PHX_EL_TSEG = 1, PHX_EL_LONG = 2, PHX_EL_LINMAP = 3 };
+
+/* Parameters of the local (per-particle) elements of one turn, in order. */
+typedef struct {
+    int32_t n_el;
+    int32_t el_type[PHX_MAX_PROGRAM];
+    int32_t el_index[PHX_MAX_PROGRAM];
+    int32_t n_tseg, n_long, n_lin;
+    int32_t has_wake;
+    phx_tseg tseg[4];
+    phx_long_seq lseq[2];
+    phx_linmap lin[2];
+    /* wake field ending the turn */
+    phx_slicer slicer;
+    double beta;              /* beam.beta at slicing time (SliceSet.beta) */
+    double beta_c;            /* beam.beta * c at slicing time */
+    double age_increment;     /* C/(beta c) at WakeField.track time */
+    int32_t n_kicks;
+    int32_t pad;
+    phx_wake_kick kick[PHX_MAX_KICKS];
+} phx_turn_program;
+
+/* Per-turn bunch statistics recorded at the end of each turn. */
+enum {
+    PHX_STAT_MEAN_X = 0, PHX_STAT_MEAN_XP, PHX_STAT_MEAN_Y, PHX_STAT_MEAN_YP,
+    PHX_STAT_MEAN_Z, PHX_STAT_MEAN_DP, PHX_STAT_SIGMA_X, PHX_STAT_SIGMA_Y,
+    PHX_STAT_SIGMA_Z, PHX_STAT_SIGMA_DP, PHX_STAT_EPSN_X, PHX_STAT_EPSN_Y,
+    PHX_STAT_EPSN_Z, PHX_STAT_COUNT
+};
+
+/* Everything the fused engine needs; all pointers are device memory owned
+   by the caller.  The engine handles one-turn maps of the form
+   [local elements..., WakeField] (or local elements only: n_slices = 0). */
+typedef struct {
+    double *coords[6];                  /* x, xp, y, yp, z, dp */
+    int64_t n;
+    int32_t n_slices;                   /* 0: no wake field */
+    int32_t n_kicks;                    /* wake kicks of the wake field */
+    int32_t hist_capacity;              /* deepest n_turns_wake (ring slots) */
+    int32_t record_stats;               /* 1: fill stats[turn][33] */
+    const phx_turn_program *programs;   /* [n_programs] */
+    const int32_t *turn_program;        /* [n_turns_total] -> program index */
+    double *stats;                      /* raw moment sums per turn, see below */
+    int32_t *turn_counter;              /* device: index of the next turn */
+    int32_t *pending;                   /* device flag: kicks not yet applied */
+    phx_slice_state *state;             /* device: cuts of the last slicing */
+    phx_wake_hist *hist;                /* device: multi-turn history */
+    double *z_bins;                     /* [n_slices+1] */
+    double *kicks;                      /* [n_kicks][n_slices] pending kick */
+    int32_t *counts;                    /* [n_slices] of the last slicing */
+    double *slice_means;                /* [2][n_slices] mean_x, mean_y */
+    void *ws;                           /* phx_engine_ws_bytes, zeroed once */
+} phx_engine;
+
+/* stats row r (state after r turns, r = 0 .. turns run) holds 33 FP64 sums
+   over the bunch: [0,6) sum u for u = x, xp, y, yp, z, dp; [6,12)
+   sum (u - u0); [12,33) sum (u - u0)(v - v0) for u <= v in that order, with
+   u0 the coordinate of particle 0 (the shifts of stats.pyx:cov). */
+
+/* Workspace bytes of the fused engine. */
+int64_t phx_engine_ws_bytes(int64_t n, int32_t n_slices, int32_t n_kicks,
+                            int32_t max_turns);
+/* Device-side reset of the engine state (turn counter, pending flag,
+   workspace accumulators, wake history).  No memset nodes. */
+int phx_engine_reset(const phx_engine *e, phx_stream_t stream);
+/* Enqueue n_turns fused turns.  Each turn is three launches: the particle
+   pass (pending wake kick, local elements, z moments), the slice histogram
+   and the wake convolution.  The wake kick of the last turn stays pending
+   until phx_engine_flush. */
+int phx_engine_turns(const phx_engine *e, int32_t n_turns, phx_stream_t stream);
+/* Applies a pending wake kick (and records the final statistics row). */
+int phx_engine_flush(const phx_engine *e, phx_stream_t stream);
+/* Enqueue one stage of a turn: 0 particle pass, 1 slice histogram, 2 wake
+   convolution (stages 1-2 need a wake field).  Running 0, 1, 2 in order is
+   one turn; used to time the stages separately. */
+int phx_engine_stage(const phx_engine *e, int32_t stage, phx_stream_t stream);
+
+/* HIP graph of n_turns fused turns (captured on an internal stream);
+   replay with phx_graph_launch.  Returns an opaque handle. */
+int phx_graph_capture(const phx_engine *e, int32_t n_turns, void **graph);
+int phx_graph_launch(void *graph, phx_stream_t stream);
+int phx_graph_destroy(void *graph);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHX_H */

@@ -1,0 +1,40 @@
+// Host-side helpers of the C ABI: error reporting and launch checks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdarg.h>
+
+#include "../../include/phx.h"
+
+namespace phx {
+
+void set_error(const char *fmt, ...);
+int fail(int code, const char *fmt, ...);
+
+inline hipStream_t as_stream(phx_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int64_t n_chunks(int64_t n) { return n <= 0 ? 0 : (n + (256 * 4) - 1) / (256 * 4); }
+
+// Byte layout of the shared workspace of the reduction kernels.
+struct WsLayout {
+  static constexpr size_t kTicketBytes = 256;         // unsigned int tickets[64]
+  static constexpr size_t kPartialSlots = 64;         // doubles per chunk partial
+  static size_t partial_bytes(int64_t n) { return (size_t)n_chunks(n) * kPartialSlots * 8; }
+};
+
+}  // namespace phx
+
+#define PHX_CHECK_LAUNCH(name)                                                   \
+  do {                                                                           \
+    hipError_t _e = hipGetLastError();                                           \
+    if (_e != hipSuccess)                                                        \
+      return phx::fail((int)_e, "%s: launch failed: %s", name, hipGetErrorString(_e)); \
+  } while (0)
+
+#define PHX_CHECK_HIP(call, name)                                                \
+  do {                                                                           \
+    hipError_t _e = (call);                                                      \
+    if (_e != hipSuccess)                                                        \
+      return phx::fail((int)_e, "%s: %s", name, hipGetErrorString(_e));         \
+  } while (0)

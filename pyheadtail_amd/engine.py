@@ -1,0 +1,331 @@
+"""Fused one-turn map: many turns of [local elements..., WakeField] with three
+kernel launches per turn and no host synchronisation.
+
+FusedOneTurnMap(one_turn_map) accepts the element list that a user would
+otherwise track one by one (as Synchrotron.one_turn_map,
+machines/synchrotron.py:294-298, or the loop of
+tests/simulation_tests/test_bb_resonator.py:91-96): TransverseSegmentMap,
+LinearMap, RFSystems, Drift and Kick elements, optionally ended by one
+WakeField.  It is itself an Element, so it can replace that list.
+
+Host scalars (p0 updates of the RF kicks, wake ages, wake factors) are
+advanced on the host for every turn with the elements' own arithmetic
+(the same code the element-by-element path runs) and uploaded as per-turn
+programs; identical programs are stored once.  The wake kick of the last
+turn stays pending on the bunch until a coordinate is read (or flush()).
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+from scipy.constants import c, e
+
+from . import _phx
+from .general.element import Element
+from .impedances.wakes import WakeField
+from .particles.particles import ScalarState, COORDS
+from .trackers.longitudinal_tracking import Drift, Kick, LinearMap, RFSystems
+from .trackers.transverse_tracking import TransverseSegmentMap
+
+STAT_NAMES = ['mean_x', 'mean_xp', 'mean_y', 'mean_yp', 'mean_z', 'mean_dp',
+              'sigma_x', 'sigma_y', 'sigma_z', 'sigma_dp', 'epsn_x', 'epsn_y', 'epsn_z']
+
+
+def stats_from_sums(rows, n, betagamma, p0):
+    """Bunch statistics from the engine's raw per-turn sums, with the
+    formulas of Particles.mean_* / sigma_* / epsn_* (np.mean,
+    cobra_functions/stats.pyx:cov and emittance)."""
+    rows = np.asarray(rows, dtype=np.float64)
+    k = 6
+
+    def cov(i, j):
+        lo, hi = min(i, j), max(i, j)
+        p = lo * k - lo * (lo - 1) // 2 + (hi - lo)
+        if n < 2:
+            return np.zeros(len(rows))
+        return (rows[:, 12 + p] - rows[:, 6 + i] * rows[:, 6 + j] / n) / (n - 1)
+
+    def emit(u, up, dp):
+        s11, s12, s22 = cov(u, u), cov(up, u), cov(up, up)
+        if dp is not None:
+            cdp2 = cov(dp, dp)
+            nz = cdp2 != 0
+            cudp, cupdp = cov(u, dp), cov(up, dp)
+            safe = np.where(nz, cdp2, 1.0)
+            s11 = s11 - np.where(nz, cudp * cudp / safe, 0.)
+            s12 = s12 - np.where(nz, cudp * cupdp / safe, 0.)
+            s22 = s22 - np.where(nz, cupdp * cupdp / safe, 0.)
+        return np.sqrt(s11 * s22 - s12 * s12)
+
+    out = {}
+    for i, name in enumerate(COORDS):
+        out['mean_' + name] = rows[:, i] / n
+    for name, i in (('x', 0), ('y', 2), ('z', 4), ('dp', 5), ('xp', 1), ('yp', 3)):
+        out['sigma_' + name] = np.sqrt(cov(i, i))
+    out['epsn_x'] = emit(0, 1, 5) * betagamma
+    out['epsn_y'] = emit(2, 3, 5) * betagamma
+    out['epsn_z'] = 4 * np.pi * emit(4, 5, None) * p0 / e
+    return out
+
+
+class FusedOneTurnMap(Element):
+    """Runs turns of a one-turn map with the fused device engine."""
+
+    LOCAL = (TransverseSegmentMap, LinearMap, RFSystems, Drift, Kick)
+
+    def __init__(self, one_turn_map, record_stats=False, graph_turns=8, *args, **kwargs):
+        elements = list(one_turn_map)
+        self.wake = None
+        if elements and isinstance(elements[-1], WakeField):
+            self.wake = elements.pop()
+        for el in elements:
+            if not isinstance(el, self.LOCAL):
+                raise TypeError("FusedOneTurnMap cannot fuse %s: supported are "
+                                "TransverseSegmentMap, LinearMap, RFSystems, Drift, Kick "
+                                "and one final WakeField" % type(el).__name__)
+        n_t = sum(isinstance(el, TransverseSegmentMap) for el in elements)
+        n_l = sum(isinstance(el, (RFSystems, Drift, Kick)) for el in elements)
+        n_m = sum(isinstance(el, LinearMap) for el in elements)
+        if n_t > 4 or n_l > 2 or n_m > 2 or len(elements) > _phx.MAX_PROGRAM:
+            raise ValueError("too many elements for one fused pass")
+        if self.wake is not None and not self.wake.wake_kicks:
+            raise ValueError("WakeField without kicks")
+        self.local = elements
+        self.record_stats = bool(record_stats)
+        self.graph_turns = int(graph_turns)
+        self._bufs = None
+        self._graphs = {}
+        self._engine = None
+        self.last_stats = None
+
+    # -- per-turn programs -------------------------------------------------------
+    def _program(self, state):
+        prog = _phx.TurnProgram()
+        n_t = n_l = n_m = 0
+        for i, el in enumerate(self.local):
+            if isinstance(el, TransverseSegmentMap):
+                prog.el_type[i], prog.el_index[i] = _phx.EL_TSEG, n_t
+                prog.tseg[n_t] = el._advance(state)
+                n_t += 1
+            elif isinstance(el, LinearMap):
+                prog.el_type[i], prog.el_index[i] = _phx.EL_LINMAP, n_m
+                prog.lin[n_m] = el._advance(state)
+                n_m += 1
+            else:
+                prog.el_type[i], prog.el_index[i] = _phx.EL_LONG, n_l
+                if isinstance(el, RFSystems):
+                    prog.lseq[n_l] = el._advance(state)
+                else:
+                    seq = _phx.LongSeq()
+                    seq.n_ops = 1
+                    seq.op[0] = el._op(state)
+                    prog.lseq[n_l] = seq
+                n_l += 1
+        prog.n_el = len(self.local)
+        prog.n_tseg, prog.n_long, prog.n_lin = n_t, n_l, n_m
+        if self.wake is not None:
+            w = self.wake
+            prog.has_wake = 1
+            prog.age_increment = state.circumference / (state.beta * c)
+            prog.slicer = w.slicer.phx_config()
+            prog.beta = state.beta
+            prog.beta_c = state.beta * c
+            prog.n_kicks = len(w.wake_kicks)
+            for i, k in enumerate(w.wake_kicks):
+                prog.kick[i] = k.record(state)
+        return prog
+
+    def _schedule(self, beam, n_turns):
+        state = ScalarState(beam)
+        progs, table, index = [], np.empty(n_turns, dtype=np.int32), {}
+        for t in range(n_turns):
+            p = self._program(state)
+            key = bytes(memoryview(p).cast("B"))
+            if key not in index:
+                index[key] = len(progs)
+                progs.append(key)
+            table[t] = index[key]
+        state.write_back(beam)
+        return progs, table
+
+    # -- device buffers ----------------------------------------------------------
+    def _ensure_buffers(self, beam, n_turns, n_programs):
+        n = beam.macroparticlenumber
+        dev = _phx.device()
+        b = self._bufs
+        if b is None or b['n'] != n:
+            self._drop_graphs()
+            S = int(self.wake.slicer.n_slices) if self.wake is not None else 0
+            n_kicks = len(self.wake.wake_kicks) if self.wake is not None else 0
+            T = self.wake.n_turns_wake_max if self.wake is not None else 1
+            b = dict(n=n, S=S, n_kicks=n_kicks, T=T, turn_cap=0, prog_cap=0)
+            b['counter'] = torch.zeros(1, dtype=torch.int32, device=dev)
+            b['pending'] = torch.zeros(1, dtype=torch.int32, device=dev)
+            b['state'] = torch.zeros(C.sizeof(_phx.SliceState), dtype=torch.uint8, device=dev)
+            b['z_bins'] = torch.zeros(max(S + 1, 1), dtype=torch.float64, device=dev)
+            b['kicks'] = torch.zeros((max(n_kicks, 1), max(S, 1)), dtype=torch.float64,
+                                     device=dev)
+            b['counts'] = torch.zeros(max(S, 1), dtype=torch.int32, device=dev)
+            b['means'] = torch.zeros((2, max(S, 1)), dtype=torch.float64, device=dev)
+            nb = _phx.lib().phx_engine_ws_bytes(n, S, n_kicks, T)
+            b['ws'] = torch.zeros(nb, dtype=torch.uint8, device=dev)
+            if self.wake is not None:
+                from .impedances.wake_kicks import WakeHistory
+                b['history'] = WakeHistory(S, T)
+                b['hist'] = b['history'].record
+            else:
+                b['history'] = b['hist'] = None
+            b['needs_reset'] = True
+            self._bufs = b
+        if b['turn_cap'] < n_turns or b['prog_cap'] < n_programs:
+            self._drop_graphs()
+            b['turn_cap'] = max(n_turns, 2 * b['turn_cap'], 64)
+            b['prog_cap'] = max(n_programs, 2 * b['prog_cap'], 4)
+            psz = C.sizeof(_phx.TurnProgram)
+            b['programs'] = torch.zeros(b['prog_cap'] * psz, dtype=torch.uint8, device=dev)
+            b['table'] = torch.zeros(b['turn_cap'], dtype=torch.int32, device=dev)
+            b['stats'] = torch.zeros((b['turn_cap'] + 1, _phx.STAT_NV), dtype=torch.float64,
+                                     device=dev)
+        return b
+
+    def _engine_record(self, beam, b, n_turns):
+        eng = _phx.Engine()
+        for i, k in enumerate(COORDS):
+            eng.coords[i] = beam._raw(k).data_ptr()
+        eng.n = beam.macroparticlenumber
+        eng.n_slices = b['S']
+        eng.n_kicks = b['n_kicks']
+        eng.hist_capacity = b['T']
+        eng.record_stats = 1 if self.record_stats else 0
+        eng.programs = b['programs'].data_ptr()
+        eng.turn_program = b['table'].data_ptr()
+        eng.stats = b['stats'].data_ptr()
+        eng.turn_counter = b['counter'].data_ptr()
+        eng.pending = b['pending'].data_ptr()
+        eng.state = b['state'].data_ptr()
+        eng.hist = _phx.ptr(b['hist'])
+        eng.z_bins = b['z_bins'].data_ptr()
+        eng.kicks = b['kicks'].data_ptr()
+        eng.counts = b['counts'].data_ptr()
+        eng.slice_means = b['means'].data_ptr()
+        eng.ws = b['ws'].data_ptr()
+        return eng
+
+    def _drop_graphs(self):
+        L = _phx.load_library()
+        for h in self._graphs.values():
+            L.phx_graph_destroy(h)
+        self._graphs = {}
+
+    def __del__(self):
+        try:
+            self._drop_graphs()
+        except Exception:
+            pass
+
+    # -- tracking ---------------------------------------------------------------
+    def reset(self):
+        """Forget the wake history (the next turn starts without memory)."""
+        if self._bufs is not None:
+            self._bufs['needs_reset'] = True
+
+    def track(self, beam):
+        self.track_turns(beam, 1)
+
+    def _prepare(self, beam, n_turns):
+        """Schedule n_turns, upload the programs, reset the turn counter;
+        returns the engine record."""
+        if beam._pending is not None and beam._pending is not self:
+            beam._run_pending()
+        for k in COORDS:
+            beam._raw(k)  # all six coordinates must exist
+        L = _phx.lib()
+        progs, table = self._schedule(beam, n_turns)
+        b = self._ensure_buffers(beam, n_turns, len(progs))
+        psz = C.sizeof(_phx.TurnProgram)
+        host = torch.frombuffer(bytearray(b"".join(progs)), dtype=torch.uint8)
+        b['programs'][:len(progs) * psz].copy_(host)
+        b['table'][:n_turns].copy_(torch.from_numpy(table))
+        eng = self._engine_record(beam, b, n_turns)
+        self._engine = eng
+        if b.get('needs_reset'):
+            _phx.call(L.phx_engine_reset, C.byref(eng), _phx.stream())
+            b['needs_reset'] = False
+        b['counter'].zero_()
+        self._n_turns_last = n_turns
+        return eng
+
+    def _finish(self, beam):
+        beam.clean_slices()
+        if self.wake is not None:
+            beam._set_pending(self)
+        elif self.record_stats:
+            self._finish_stats(beam)
+
+    def track_turns(self, beam, n_turns, use_graph=True):
+        """Advance beam by n_turns turns."""
+        if n_turns <= 0:
+            return
+        L = _phx.lib()
+        eng = self._prepare(beam, n_turns)
+        s = _phx.stream()
+        G = self.graph_turns
+        done = 0
+        if use_graph and G > 0 and n_turns >= G:
+            key = (tuple(eng.coords), self.record_stats, G)
+            h = self._graphs.get(key)
+            if h is None:
+                h = C.c_void_p()
+                _phx.call(L.phx_graph_capture, C.byref(eng), G, C.byref(h))
+                self._graphs[key] = h
+            while n_turns - done >= G:
+                _phx.call(L.phx_graph_launch, h, s)
+                done += G
+        if n_turns > done:
+            _phx.call(L.phx_engine_turns, C.byref(eng), n_turns - done, s)
+        self._finish(beam)
+
+    def timed_turns(self, beam, n_turns):
+        """Run n_turns with one launch per stage, bracketing every stage with
+        HIP events on the launch stream; returns the mean duration of each
+        stage in ms (pass, histogram, convolution)."""
+        L = _phx.lib()
+        eng = self._prepare(beam, n_turns)
+        s = _phx.stream()
+        n_st = 3 if self.wake is not None else 1
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(n_st + 1)]
+              for _ in range(n_turns)]
+        for t in range(n_turns):
+            ev[t][0].record()
+            for st in range(n_st):
+                _phx.call(L.phx_engine_stage, C.byref(eng), st, s)
+                ev[t][st + 1].record()
+        torch.cuda.synchronize()
+        self._finish(beam)
+        return [float(np.mean([ev[t][st].elapsed_time(ev[t][st + 1]) for t in range(n_turns)]))
+                for st in range(n_st)]
+
+    def flush(self, beam):
+        """Apply the pending wake kick of the last turn (and record the
+        final statistics row)."""
+        L = _phx.lib()
+        _phx.call(L.phx_engine_flush, C.byref(self._engine), _phx.stream())
+        if self.record_stats:
+            self._finish_stats(beam)
+
+    def _finish_stats(self, beam):
+        n = self._n_turns_last
+        rows = self._bufs['stats'][:n + 1].cpu().numpy()
+        self.last_stats = stats_from_sums(rows, beam.macroparticlenumber, beam.betagamma,
+                                          beam.p0)
+        self.last_stats['raw'] = rows
+
+    @property
+    def slice_data(self):
+        """(z_bins, counts, mean_x, mean_y) of the last slicing (device)."""
+        b = self._bufs
+        return b['z_bins'], b['counts'], b['means'][0], b['means'][1]
+
+    @property
+    def pending_kicks(self):
+        return self._bufs['kicks']

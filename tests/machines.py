@@ -1,0 +1,68 @@
+"""Product-side one-turn maps of the benchmark workloads, built with the
+drop-in classes the way Synchrotron builds them (smooth optics, one segment,
+longitudinal map first), plus conversions between product and oracle bunches."""
+import numpy as np
+from scipy.constants import c, e, m_p
+
+from oracle import workloads as W
+from oracle import refpath as R
+
+COORDS = ('x', 'xp', 'y', 'yp', 'z', 'dp')
+
+
+def to_product(ob):
+    from pyheadtail_amd.particles.particles import Particles
+    return Particles(ob.macroparticlenumber, ob.particlenumber_per_mp, ob.charge, ob.mass,
+                     ob.circumference, ob.gamma, {k: getattr(ob, k) for k in COORDS})
+
+
+def to_numpy(pb):
+    return {k: getattr(pb, k).cpu().numpy() for k in COORDS}
+
+
+def smooth_transverse(Qp=0., app=0., app_xy=0.):
+    from pyheadtail_amd.trackers.transverse_tracking import TransverseMap
+    from pyheadtail_amd.trackers.detuners import Chromaticity, AmplitudeDetuning
+    C = W.CIRCUMFERENCE
+    s = np.arange(0, 2) * C / 1
+    dets = []
+    if Qp:
+        dets.append(Chromaticity(Qp, Qp))
+    if app or app_xy:
+        dets.append(AmplitudeDetuning(app, app, app_xy))
+    return TransverseMap(s, 0. * s, 0. * s + W.BETA_X, 0. * s, 0. * s, 0. * s + W.BETA_Y,
+                         0. * s, W.Q_X, W.Q_Y, detuners=dets)
+
+
+def linear_machine():
+    from pyheadtail_amd.trackers.longitudinal_tracking import LinearMap
+    lin = LinearMap(np.atleast_1d(W.ALPHA_C), W.CIRCUMFERENCE, 0.017)
+    return [lin, smooth_transverse()[0]]
+
+
+def nonlinear_machine(n_slices=500, n_turns_wake=1, p0=None, wake=True):
+    from pyheadtail_amd.trackers.longitudinal_tracking import RFSystems
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.impedances.wakes import WakeField, CircularResonator
+    p0 = W.p0_of(W.GAMMA) if p0 is None else p0
+    rf = RFSystems(W.CIRCUMFERENCE, np.atleast_1d(W.H_RF), np.atleast_1d(W.V_RF),
+                   np.atleast_1d(0.), np.atleast_1d(W.ALPHA_C), W.GAMMA, p_increment=0.,
+                   charge=e, mass=m_p)
+    tm = smooth_transverse(5., 1e-10 * p0, -5e-11 * p0)
+    els = [rf, tm[0]]
+    if wake:
+        els.append(WakeField(UniformBinSlicer(n_slices, n_sigma_z=3),
+                             CircularResonator(1e6, 1e9, 1, n_turns_wake=n_turns_wake)))
+    return els
+
+
+def coord_err(pb_np, ob):
+    """max over coordinates of max|a-b| / rms(b)."""
+    worst = 0.
+    for k in COORDS:
+        a, b = pb_np[k], getattr(ob, k)
+        rms = np.sqrt(np.mean(b * b))
+        if rms == 0:
+            continue
+        worst = max(worst, float(np.max(np.abs(a - b)) / rms))
+    return worst

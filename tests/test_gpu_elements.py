@@ -1,0 +1,184 @@
+"""Element-level parity of the HIP path against the oracle (same seeded
+bunch, one track() each).  Tolerances: slice indices and counts bit-exact;
+coordinates and slice moments within 1e-12 of the coordinate's RMS;
+arithmetic without transcendental functions (LinearMap, transverse map
+without detuning) is bit-identical to the NumPy restatement because the
+kernels evaluate the same IEEE operations in the same order."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import refpath as R
+from oracle import workloads as W
+from tests.machines import (COORDS, coord_err, linear_machine, nonlinear_machine,
+                            smooth_transverse, to_numpy, to_product)
+
+TOL = 1e-12
+
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(name):
+    print("\n[case] %s" % name, flush=True)
+
+
+def test_linear_map_bit_exact():
+    _print_case("LinearMap N=20000")
+    ob = W.linear_bunch(20000, seed=1)
+    pb = to_product(ob)
+    lin = linear_machine()[0]
+    lin.track(pb)
+    R.LinearMapOracle([W.ALPHA_C], W.CIRCUMFERENCE, 0.017).track(ob)
+    got = to_numpy(pb)
+    for k in COORDS:
+        np.testing.assert_array_equal(got[k], getattr(ob, k), err_msg=k)
+
+
+def test_transverse_without_detuning_bit_exact():
+    _print_case("TransverseSegmentMap no detuner")
+    ob = W.linear_bunch(20000, seed=2)
+    pb = to_product(ob)
+    smooth_transverse()[0].track(pb)
+    R.smooth_segment(W.Q_X, W.Q_Y, W.BETA_X, W.BETA_Y).track(ob)
+    got = to_numpy(pb)
+    for k in COORDS:
+        np.testing.assert_array_equal(got[k], getattr(ob, k), err_msg=k)
+
+
+def test_transverse_with_detuning():
+    _print_case("TransverseSegmentMap chroma+amplitude")
+    ob = W.nonlinear_bunch(20000, seed=3)
+    pb = to_product(ob)
+    p0 = W.p0_of(W.GAMMA)
+    smooth_transverse(5., 1e-10 * p0, -5e-11 * p0)[0].track(pb)
+    R.smooth_segment(W.Q_X, W.Q_Y, W.BETA_X, W.BETA_Y, Qp_x=5., Qp_y=5., app_x=1e-10 * p0,
+                     app_y=1e-10 * p0, app_xy=-5e-11 * p0).track(ob)
+    assert coord_err(to_numpy(pb), ob) <= TOL
+
+
+def test_rf_systems():
+    _print_case("RFSystems")
+    ob = W.nonlinear_bunch(20000, seed=4)
+    pb = to_product(ob)
+    rf = nonlinear_machine(wake=False)[0]
+    rf.track(pb)
+    R.RFSystemsOracle(W.CIRCUMFERENCE, [W.H_RF], [W.V_RF], [0.], [W.ALPHA_C], W.GAMMA,
+                      ob.charge).track(ob)
+    assert coord_err(to_numpy(pb), ob) <= TOL
+    assert pb.p0 == ob.p0 and pb.gamma == ob.gamma
+
+
+def test_bunch_statistics():
+    _print_case("bunch statistics")
+    ob = W.nonlinear_bunch(30001, seed=5)
+    pb = to_product(ob)
+    ref = R.bunch_statistics(ob)
+    for name, v in ref.items():
+        got = getattr(pb, name)()
+        if name.startswith('mean_'):
+            scale = ref['sigma_' + name[5:]]   # means can be ~0: judge against the RMS
+        else:
+            scale = abs(v)
+        assert abs(got - v) <= 1e-12 * scale, (name, got, v)
+
+
+@pytest.mark.parametrize("n_slices,n_sigma_z,z_cuts", [(50, 3, None), (500, 3, None),
+                                                         (64, None, (-0.5, 0.4)),
+                                                         (100, None, None)])
+def test_slicing_indices_counts_means(n_slices, n_sigma_z, z_cuts):
+    _print_case("slicing S=%d nsig=%s cuts=%s" % (n_slices, n_sigma_z, z_cuts))
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    ob = W.nonlinear_bunch(50000, seed=6)
+    pb = to_product(ob)
+    sl = UniformBinSlicer(n_slices, n_sigma_z=n_sigma_z, z_cuts=z_cuts)
+    ss = sl.slice(pb, statistics=['mean_x', 'mean_y', 'sigma_x', 'epsn_x', 'epsn_z'])
+    zb = ss.z_bins.cpu().numpy()
+    o_tail, o_head = R.long_cuts(ob, n_sigma_z, z_cuts)
+    assert abs(zb[0] - o_tail) <= 1e-12 * abs(o_tail)
+    assert abs(zb[-1] - o_head) <= 1e-11 * max(abs(o_head), 1e-3)
+    # bit-exact indices and counts given the device's cuts
+    st = ss._state.cpu().numpy().view(np.float64)
+    sd = R.slice_from_cuts(ob, n_slices, st[0], st[1])
+    np.testing.assert_array_equal(ss.z_bins.cpu().numpy(), sd.z_bins)
+    np.testing.assert_array_equal(ss.slice_index_of_particle.cpu().numpy(),
+                                  sd.slice_index_of_particle)
+    np.testing.assert_array_equal(ss.n_macroparticles_per_slice.cpu().numpy(),
+                                  sd.n_macroparticles_per_slice)
+    # free-running: the oracle's own cuts give the same indices
+    sd2 = R.slice_uniform(ob, n_slices, n_sigma_z, z_cuts)
+    np.testing.assert_array_equal(ss.slice_index_of_particle.cpu().numpy(),
+                                  sd2.slice_index_of_particle)
+    for name, u in (('mean_x', ob.x), ('mean_y', ob.y)):
+        ref = R.mean_per_slice(sd, u)
+        got = getattr(ss, name).cpu().numpy()
+        assert np.max(np.abs(got - ref)) <= TOL * np.std(u), name
+    ref = R.std_per_slice(sd, ob.x)
+    assert np.max(np.abs(ss.sigma_x.cpu().numpy() - ref)) <= 1e-11 * np.std(ob.x)
+    ref = R.emittance_per_slice(sd, ob.x, ob.xp, ob.dp) * ob.betagamma
+    got = ss.epsn_x.cpu().numpy()
+    assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(ref)
+    ref = R.emittance_per_slice(sd, ob.z, ob.dp) * 4 * np.pi * ob.p0 / 1.602176634e-19
+    got = ss.epsn_z.cpu().numpy()
+    assert np.max(np.abs(got - ref)) <= 1e-9 * np.max(ref)
+    assert int(ss.n_macroparticles_per_slice.sum()) + len(ss.particles_outside_cuts) == 50000
+
+
+def test_slicing_edges():
+    _print_case("slicing edge cases")
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    n = 8
+    z = np.array([-1., 1., 0.999999, np.nan, -1.0000001, 0.5, 0., 2.])
+    coords = {k: np.linspace(-1, 1, n) for k in COORDS}
+    coords['z'] = z
+    ob = R.Bunch(n, 1e9, 1.602176634e-19, 1.67262192369e-27, 100., 20., coords)
+    pb = to_product(ob)
+    ss = UniformBinSlicer(4, z_cuts=(-1., 1.)).slice(pb)
+    sd = R.slice_from_cuts(ob, 4, -1., 1.)
+    got = ss.slice_index_of_particle.cpu().numpy()
+    np.testing.assert_array_equal(got, sd.slice_index_of_particle)
+    assert got[0] == 0 and got[1] == 4 and got[3] == np.iinfo(np.int32).min
+    np.testing.assert_array_equal(ss.n_macroparticles_per_slice.cpu().numpy(),
+                                  sd.n_macroparticles_per_slice)
+
+
+def test_wakefield_element():
+    _print_case("WakeField element 500 slices")
+    ob = W.nonlinear_bunch(50000, seed=7)
+    ob.x = ob.x + 1e-3 * np.sin(ob.z * 10)
+    ob.y = ob.y - 5e-4 * np.cos(ob.z * 7)
+    pb = to_product(ob)
+    wf = nonlinear_machine()[2]
+    wf.track(pb)
+    owf = W.nonlinear_map_oracle()[2]
+    owf.track(ob)
+    got = to_numpy(pb)
+    for k in COORDS:
+        b = getattr(ob, k)
+        assert np.max(np.abs(got[k] - b)) <= TOL * np.sqrt(np.mean(b * b)), k
+
+
+def test_wakefield_multiturn_dot_product():
+    _print_case("WakeField 4-turn dot product, 50 slices")
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.impedances.wakes import WakeField, ParallelHorizontalPlatesResistiveWall
+    ob = W.nonlinear_bunch(20000, seed=8)
+    pb = to_product(ob)
+    rw = ParallelHorizontalPlatesResistiveWall(0.02, W.CIRCUMFERENCE, 5.9e7, 1e-12, n_turns_wake=4)
+    wf = WakeField(UniformBinSlicer(50, n_sigma_z=3), rw)
+    kinds = ['DipoleX', 'QuadX', 'DipoleY', 'QuadY']
+    yok = {'DipoleX': np.pi**2 / 24., 'QuadX': -np.pi**2 / 24., 'DipoleY': np.pi**2 / 12.,
+           'QuadY': np.pi**2 / 24.}
+    okicks = [R.Kick(k, R.resistive_wall_transverse(yok[k], 0.02, W.CIRCUMFERENCE, 5.9e7, 1e-12),
+                     4, uniform_bin=True, z_cuts=None) for k in kinds]
+    owf = R.WakeFieldOracle(50, okicks, n_sigma_z=3)
+    rf = nonlinear_machine(wake=False)
+    orf = W.nonlinear_map_oracle()[:2]
+    for turn in range(6):
+        for el in rf:
+            el.track(pb)
+        for el in orf:
+            el.track(ob)
+        wf.track(pb)
+        owf.track(ob)
+        err = coord_err(to_numpy(pb), ob)
+        assert err <= 1e-11, (turn, err)
