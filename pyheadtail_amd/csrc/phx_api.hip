@@ -52,7 +52,10 @@ extern "C" int phx_init(int device) {
 
 extern "C" int64_t phx_workspace_bytes(int64_t n, int32_t n_slices) {
   if (n < 0 || n_slices < 0) return -1;
-  return (int64_t)(256 + acc_partials_bytes(n) + acc_global_bytes(n_slices, PHX_MAX_TERMS));
+  const int64_t nc = (n + PHX_CHUNK - 1) / PHX_CHUNK;
+  const int H = (int)(nc < 256 ? nc : 256);
+  return (int64_t)(256 + acc_partials_bytes(n) +
+                   acc_parts_bytes(H > 0 ? H : 1, n_slices, PHX_MAX_TERMS));
 }
 
 extern "C" int64_t phx_sizeof(const char *name) {

@@ -263,25 +263,6 @@ __device__ __forceinline__ void reduce_partials(const double *part, int64_t n_ch
   __syncthreads();
 }
 
-// "Last block done" ticket.  Every block calls it once after writing its
-// partials; returns true in exactly one block, after all other blocks'
-// writes are visible.  The ticket counter is re-armed by that block.
-__device__ __forceinline__ bool last_block(unsigned int *ticket, unsigned int n_blocks) {
-  __shared__ bool is_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned int t = atomicAdd(ticket, 1u);
-    is_last = (t == n_blocks - 1);
-  }
-  __syncthreads();
-  if (is_last) {
-    __threadfence();
-    if (threadIdx.x == 0) atomicExch(ticket, 0u);
-  }
-  return is_last;
-}
-
 // ---------------------------------------------------------------------------
 // Split 64-bit fixed-point accumulation of per-slice sums.
 // A value u with |u| <= B is scaled by 2^e, e = 60 - ilogb(B), so

@@ -1,18 +1,24 @@
-// Fused one-turn map.  One turn of [local elements..., WakeField] is three
-// launches:
-//   1. particle pass: apply the wake kick left pending by the previous turn
-//      (it needs only z and the previous cuts, both unchanged since), record
-//      the bunch moments of that end-of-turn state, run the local elements
-//      (longitudinal maps, transverse segments), write the six coordinates
-//      once, and reduce the z moments of the new state; the last block turns
-//      them into the slicing cuts (96 B of HBM traffic per macroparticle);
-//   2. slice histogram: counts and fixed-point sums of x and y per slice
-//      (24 B per macroparticle); the last block forms the slice means and
-//      pushes them into the multi-turn wake history;
-//   3. wake convolution over the history (per-slice work only).
-// The kick of turn t is applied in pass 1 of turn t+1, which gives the same
-// arithmetic as WakeField.track at the end of turn t
-// (impedances/wakes.py:121-145).
+// Fused one-turn map.  One turn of [local elements..., WakeField]:
+//   1. particle pass (chunk grid): apply the wake kick left pending by the
+//      previous turn (it needs only z and the previous cuts, both unchanged
+//      since), record the bunch moments of that end-of-turn state, run the
+//      local elements (longitudinal maps, transverse segments), write the six
+//      coordinates once (96 B per macroparticle), and store per-chunk partial
+//      sums of the new z moments;
+//   2. finalize (one block): fold the partials in a fixed order into the
+//      slicing cuts and bin edges, the statistics row and the wake-history
+//      bookkeeping (ages, ring head);
+//   3. slice histogram: each block keeps an LDS histogram of counts and
+//      fixed-point sums of x and y (24 B per macroparticle) and stores it;
+//   4. merge (slice-parallel): exact totals, slice means and the new
+//      history entry (times, moments);
+//   5. wake convolution over the history (and, with several turns of
+//      memory, the in-order sum over turns).
+// No stage uses cross-block atomics or fences: kernel boundaries order the
+// stages, which keeps every reduction deterministic and avoids the
+// serialisation of device-scope atomics.  The kick of turn t is applied in
+// pass 1 of turn t+1, which gives the arithmetic of WakeField.track at the
+// end of turn t (impedances/wakes.py:121-145).
 #include "phx_device.h"
 #include "phx_host.h"
 #include "phx_slicing.h"
@@ -20,52 +26,44 @@
 
 #define EN_STAT_NV 33
 #define EN_NV (ZB_NV + EN_STAT_NV)
-#define EN_HIST_BLOCKS 1024
+#define EN_HIST_BLOCKS 256
+#define EN_HIST_THREADS 1024
 
 namespace {
 
-struct EngineWs {
-  unsigned int *tickets;       // [0] pass, [1] hist, [2] flush
-  unsigned int *conv_tickets;  // [PHX_MAX_KICKS * tiles]
-  double *partials;            // [chunks][EN_NV]
-  AccGlobal acc;               // T = 2 (x, y)
-  double *conv_part;           // [n_kicks][T][S]
-};
-
 inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
 
-__host__ __device__ inline EngineWs engine_ws(void *ws, int64_t n, int S, int n_kicks, int T) {
+struct EngineWs {
+  double *partials;   // [chunks][EN_NV]
+  AccParts acc;       // H partial histograms, T = 2 (x, y)
+  double *conv_part;  // [n_kicks][T][S]
+};
+
+__host__ __device__ inline int hist_blocks(int64_t n) {
+  const int64_t nc = (n + PHX_CHUNK - 1) / PHX_CHUNK;
+  return (int)(nc < EN_HIST_BLOCKS ? nc : EN_HIST_BLOCKS);
+}
+
+__host__ __device__ inline EngineWs engine_ws(void *ws, int64_t n, int S) {
   EngineWs w;
   char *p = reinterpret_cast<char *>(ws);
-  w.tickets = reinterpret_cast<unsigned int *>(p);
-  p += 256;
-  const int tiles = (S + WK_TILE - 1) / WK_TILE;
-  w.conv_tickets = reinterpret_cast<unsigned int *>(p);
-  p += ((size_t)PHX_MAX_KICKS * tiles * 4 + 255) / 256 * 256;
   const int64_t nc = (n + PHX_CHUNK - 1) / PHX_CHUNK;
+  const int Se = S > 0 ? S : 1;
   w.partials = reinterpret_cast<double *>(p);
   p += ((size_t)nc * EN_NV * 8 + 255) / 256 * 256;
-  w.acc.counts = reinterpret_cast<unsigned int *>(p);
-  p += ((size_t)S * 4 + 255) / 256 * 256;
-  w.acc.hi = reinterpret_cast<unsigned long long *>(p);
-  p += (size_t)2 * S * 8;
-  w.acc.lo = reinterpret_cast<unsigned long long *>(p);
-  p += (size_t)2 * S * 8;
-  w.acc.fallback = reinterpret_cast<double *>(p);
-  p += (size_t)2 * S * 8;
+  const int H = hist_blocks(n);
+  w.acc = acc_parts_layout(p, H, Se, 2);
+  p += ((size_t)H * Se * 4 + 255) / 256 * 256 + (size_t)H * 2 * Se * 16 + (size_t)2 * Se * 8;
   w.conv_part = reinterpret_cast<double *>(p);
-  (void)n_kicks;
-  (void)T;
   return w;
 }
 
 size_t engine_ws_bytes(int64_t n, int S, int n_kicks, int T) {
-  const int tiles = (S + WK_TILE - 1) / WK_TILE;
   const int64_t nc = (n + PHX_CHUNK - 1) / PHX_CHUNK;
-  size_t b = 256 + round256((size_t)PHX_MAX_KICKS * tiles * 4) +
-             round256((size_t)nc * EN_NV * 8) + round256((size_t)S * 4) + (size_t)6 * S * 8;
-  b += (size_t)n_kicks * T * S * 8;
-  return b;
+  const int Se = S > 0 ? S : 1;
+  size_t b = round256((size_t)nc * EN_NV * 8) + acc_parts_bytes(hist_blocks(n), Se, 2);
+  b += (size_t)(n_kicks > 0 ? n_kicks : 1) * T * Se * 8;
+  return round256(b);
 }
 
 struct EngineArgs {
@@ -121,18 +119,22 @@ __device__ __forceinline__ void stats_accumulate(double *v, const P6 &p, const d
     }
 }
 
+template <bool REC, bool WAKE>
+struct PassLayout {
+  static constexpr int NV = (WAKE ? ZB_NV : 0) + (REC ? EN_STAT_NV : 0) + ((!WAKE && !REC) ? 1 : 0);
+  static constexpr unsigned long long MINM = WAKE ? ZB_MINM : 0ULL;
+  static constexpr unsigned long long MAXM = WAKE ? ZB_MAXM : 0ULL;
+  static constexpr int SOFF = WAKE ? ZB_NV : 0;
+};
+
 // MODE 0: a turn; MODE 1: flush (pending kick + final statistics only).
 template <int MODE, bool REC, bool WAKE>
 __global__ void __launch_bounds__(PHX_BLOCK) k_engine_pass(EngineArgs a) {
+  using L = PassLayout<REC, WAKE>;
   const phx_engine &e = a.e;
-  constexpr int NV = (WAKE ? ZB_NV : 0) + (REC ? EN_STAT_NV : 0) + ((!WAKE && !REC) ? 1 : 0);
-  constexpr unsigned long long MINM = WAKE ? ZB_MINM : 0ULL;
-  constexpr unsigned long long MAXM = WAKE ? ZB_MAXM : 0ULL;
-  constexpr int SOFF = WAKE ? ZB_NV : 0;
-  __shared__ double lds[NV * (PHX_BLOCK / PHX_WAVE)];
-  __shared__ double res[NV];
+  __shared__ double lds[L::NV * (PHX_BLOCK / PHX_WAVE)];
   __shared__ double sh[8];
-  EngineWs w = engine_ws(e.ws, e.n, e.n_slices, a.n_kicks, a.T);
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
   const int turn = *e.turn_counter;
   const int prog_turn = MODE == 0 ? turn : (turn > 0 ? turn - 1 : 0);
   const phx_turn_program &g = e.programs[e.turn_program[prog_turn]];
@@ -140,6 +142,7 @@ __global__ void __launch_bounds__(PHX_BLOCK) k_engine_pass(EngineArgs a) {
   const double tail = e.state->tail, width = e.state->width;
   const int S = e.n_slices;
   if (threadIdx.x == 0) {
+    // particle 0 sets the shifts of the shifted sums (stats.pyx:cov)
     P6 p = load_p(e, 0);
     if (pend) kick_pending(p, g, tail, width, S, e.kicks);
     sh[0] = p.x; sh[1] = p.xp; sh[2] = p.y; sh[3] = p.yp; sh[4] = p.z; sh[5] = p.dp;
@@ -147,9 +150,9 @@ __global__ void __launch_bounds__(PHX_BLOCK) k_engine_pass(EngineArgs a) {
     sh[6] = p.z;
   }
   __syncthreads();
-  double v[NV];
+  double v[L::NV];
 #pragma unroll
-  for (int i = 0; i < NV; ++i) v[i] = red_ident<MINM, MAXM>(i);
+  for (int i = 0; i < L::NV; ++i) v[i] = red_ident<L::MINM, L::MAXM>(i);
   double shs[6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) shs[i] = sh[i];
@@ -166,7 +169,7 @@ __global__ void __launch_bounds__(PHX_BLOCK) k_engine_pass(EngineArgs a) {
     const int64_t j = base + (int64_t)k * PHX_BLOCK;
     if (j < e.n) {
       if (pend) kick_pending(p[k], g, tail, width, S, e.kicks);
-      if (REC) stats_accumulate(&v[SOFF], p[k], shs);
+      if (REC) stats_accumulate(&v[L::SOFF], p[k], shs);
       if (MODE == 0) apply_program(p[k], g);
       if (MODE == 0 || pend) store_p(e, j, p[k]);
       if (WAKE) {
@@ -179,56 +182,78 @@ __global__ void __launch_bounds__(PHX_BLOCK) k_engine_pass(EngineArgs a) {
       }
     }
   }
-  const double r = block_reduce_slots<NV, MINM, MAXM>(v, lds);
-  if (threadIdx.x < NV) w.partials[(int64_t)blockIdx.x * EN_NV + threadIdx.x] = r;
-  if (last_block(&w.tickets[MODE == 0 ? 0 : 2], gridDim.x)) {
-    // partials of this pass use stride EN_NV; fold them with the NV layout
-    double acc[NV];
+  const double r = block_reduce_slots<L::NV, L::MINM, L::MAXM>(v, lds);
+  if (threadIdx.x < L::NV) w.partials[(int64_t)blockIdx.x * EN_NV + threadIdx.x] = r;
+}
+
+// One block: fold the chunk partials, then the per-turn bookkeeping.
+template <int MODE, bool REC, bool WAKE>
+__global__ void __launch_bounds__(PHX_BLOCK) k_engine_fin(EngineArgs a) {
+  using L = PassLayout<REC, WAKE>;
+  const phx_engine &e = a.e;
+  __shared__ double lds[L::NV * (PHX_BLOCK / PHX_WAVE)];
+  __shared__ double res[L::NV];
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
+  const int turn = *e.turn_counter;
+  const int64_t nc = (e.n + PHX_CHUNK - 1) / PHX_CHUNK;
+  double acc[L::NV];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) acc[i] = red_ident<MINM, MAXM>(i);
-    for (int64_t c = threadIdx.x; c < (int64_t)gridDim.x; c += PHX_BLOCK) {
-      const double *pc = w.partials + c * EN_NV;
+  for (int i = 0; i < L::NV; ++i) acc[i] = red_ident<L::MINM, L::MAXM>(i);
+  for (int64_t c = threadIdx.x; c < nc; c += PHX_BLOCK) {
+    const double *pc = w.partials + c * EN_NV;
 #pragma unroll
-      for (int i = 0; i < NV; ++i) acc[i] = red_slot<MINM, MAXM>(i, acc[i], pc[i]);
+    for (int i = 0; i < L::NV; ++i) acc[i] = red_slot<L::MINM, L::MAXM>(i, acc[i], pc[i]);
+  }
+  const double rr = block_reduce_slots<L::NV, L::MINM, L::MAXM>(acc, lds);
+  if (threadIdx.x < L::NV) res[threadIdx.x] = rr;
+  __syncthreads();
+  if (REC && threadIdx.x < EN_STAT_NV)
+    e.stats[(size_t)turn * EN_STAT_NV + threadIdx.x] = res[L::SOFF + threadIdx.x];
+  if (MODE == 0) {
+    const phx_turn_program &g = e.programs[e.turn_program[turn]];
+    if (WAKE) {
+      zblock_finalize(res, g.slicer, e.n, e.state, e.z_bins, turn);
+      if (threadIdx.x == 0) {
+        // WakeField.track: ages += C/(beta c), then the new entry at age 0
+        phx_wake_hist *h = e.hist;
+        const int cap = h->capacity, nf = h->n_filled, head = h->head;
+        for (int i = 0; i < nf; ++i) {
+          const int sl = ring_slot(head, i, cap);
+          h->age[sl] = h->age[sl] + g.age_increment;
+        }
+        const int nh = (head + 1) % cap;
+        h->head = nh;
+        h->n_filled = nf + 1 < cap ? nf + 1 : cap;
+        h->age[nh] = 0.0;
+        h->beta[nh] = g.beta;
+      }
     }
-    const double rr = block_reduce_slots<NV, MINM, MAXM>(acc, lds);
-    if (threadIdx.x < NV) res[threadIdx.x] = rr;
-    __syncthreads();
-    if (WAKE) zblock_finalize(res, g.slicer, e.n, e.state, e.z_bins, turn);
-    if (REC && threadIdx.x < EN_STAT_NV)
-      e.stats[(size_t)turn * EN_STAT_NV + threadIdx.x] = res[SOFF + threadIdx.x];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      if (MODE == 0) *e.turn_counter = turn + 1;
-      else *e.pending = 0;
-    }
+    if (threadIdx.x == 0) *e.turn_counter = turn + 1;
+  } else {
+    if (threadIdx.x == 0) *e.pending = 0;
   }
 }
 
-// Slice histogram of the new state: counts, sum x, sum y per slice.  The
-// last block forms the means and pushes the SliceSet data into the history.
-__global__ void __launch_bounds__(PHX_BLOCK) k_engine_hist(EngineArgs a) {
+// Slice histogram of the new state: counts, sum x, sum y per slice, one
+// LDS histogram per block stored as a partial.
+__global__ void __launch_bounds__(EN_HIST_THREADS) k_engine_hist(EngineArgs a) {
   extern __shared__ unsigned long long acc_lds[];
   const phx_engine &e = a.e;
-  EngineWs w = engine_ws(e.ws, e.n, e.n_slices, a.n_kicks, a.T);
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
   const int S = e.n_slices;
   const double tail = e.state->tail, width = e.state->width;
-  AccTerms t{};
-  t.n = 2;
-  t.a[0] = 0; t.b[0] = -1;
-  t.a[1] = 2; t.b[1] = -1;
-  int ex[2];
-  ex[0] = fx_exponent(e.state->absmax[0]);
-  ex[1] = fx_exponent(e.state->absmax[2]);
+  const int ex = fx_exponent(e.state->absmax[0]);
+  const int ey = fx_exponent(e.state->absmax[2]);
   AccLds L = acc_lds_layout(acc_lds, S, 2);
   acc_lds_zero(L, S, 2);
-  const int64_t nc = (e.n + PHX_CHUNK - 1) / PHX_CHUNK;
+  constexpr int CH = EN_HIST_THREADS * PHX_PPT;
+  const int64_t nc = (e.n + CH - 1) / CH;
   for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
-    const int64_t base = c * PHX_CHUNK + threadIdx.x;
+    const int64_t base = c * CH + threadIdx.x;
     double zz[PHX_PPT], xx[PHX_PPT], yy[PHX_PPT];
 #pragma unroll
     for (int k = 0; k < PHX_PPT; ++k) {
-      const int64_t j = base + (int64_t)k * PHX_BLOCK;
+      const int64_t j = base + (int64_t)k * EN_HIST_THREADS;
       if (j < e.n) {
         zz[k] = e.coords[4][j];
         xx[k] = e.coords[0][j];
@@ -237,67 +262,74 @@ __global__ void __launch_bounds__(PHX_BLOCK) k_engine_hist(EngineArgs a) {
     }
 #pragma unroll
     for (int k = 0; k < PHX_PPT; ++k) {
-      const int64_t j = base + (int64_t)k * PHX_BLOCK;
+      const int64_t j = base + (int64_t)k * EN_HIST_THREADS;
       if (j < e.n) {
         const double f = slice_floor(zz[k], tail, width);
         if (f >= 0.0 && f < (double)S) {
-          P6 p;
-          p.x = xx[k]; p.y = yy[k]; p.z = zz[k];
-          p.xp = p.yp = p.dp = 0.0;
-          acc_add_lds(L, S, t, ex, (int)f, p, w.acc);
+          const int s = (int)f;
+          atomicAdd(&L.counts[s], 1u);
+          acc_add_value(L, S, 0, s, xx[k], ex, w.acc.fallback);
+          acc_add_value(L, S, 1, s, yy[k], ey, w.acc.fallback);
         }
       }
     }
   }
-  acc_lds_flush(L, S, 2, w.acc);
-  if (last_block(&w.tickets[1], gridDim.x)) {
-    const int turn = e.state->turn;
-    const phx_turn_program &g = e.programs[e.turn_program[turn]];
-    acc_finalize(w.acc, S, t, ex, e.counts, nullptr, e.slice_means);
-    __syncthreads();
-    // history entry: times = z_centers / (beta c) (particles/slicing.py:124-126, 303-307)
-    phx_wake_hist *h = e.hist;
-    __shared__ int sh_head;
-    if (threadIdx.x == 0) {
-      const int cap = h->capacity, nf = h->n_filled, head = h->head;
-      for (int i = 0; i < nf; ++i) {
-        const int sl = ring_slot(head, i, cap);
-        h->age[sl] = h->age[sl] + g.age_increment;
-      }
-      const int nh = (head + 1) % cap;
-      h->head = nh;
-      h->n_filled = nf + 1 < cap ? nf + 1 : cap;
-      h->age[nh] = 0.0;
-      h->beta[nh] = g.beta;
-      sh_head = nh;
-    }
-    __syncthreads();
-    const int nh = sh_head;
-    double *tt = h->times + (size_t)nh * S;
-    double *mm = h->mom + (size_t)nh * 3 * S;
-    for (int s = threadIdx.x; s < S; s += blockDim.x) {
-      const double zc = e.z_bins[s] + 0.5 * (e.z_bins[s + 1] - e.z_bins[s]);
-      tt[s] = zc / g.beta_c;
-      const double cnt = (double)e.counts[s];
-      mm[s] = cnt;
-      mm[S + s] = cnt * e.slice_means[s];
-      mm[2 * S + s] = cnt * e.slice_means[S + s];
-    }
-  }
+  acc_lds_store(L, S, 2, w.acc, blockIdx.x);
+}
+
+// Merge the partial histograms; means and the new history entry:
+// times = z_centers / (beta c) (particles/slicing.py:124-126, 303-307),
+// moments n, n*mean_x, n*mean_y (impedances/wake_kicks.py:217, 235).
+__global__ void __launch_bounds__(RD_SL * RD_G) k_engine_merge(EngineArgs a) {
+  const phx_engine &e = a.e;
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
+  const int S = e.n_slices;
+  SliceTotals tot;
+  if (!acc_merge(w.acc, S, 2, tot)) return;
+  const int s = blockIdx.x * RD_SL + threadIdx.x % RD_SL;
+  const int turn = e.state->turn;
+  const phx_turn_program &g = e.programs[e.turn_program[turn]];
+  const int ex = fx_exponent(e.state->absmax[0]);
+  const int ey = fx_exponent(e.state->absmax[2]);
+  const double cnt = (double)tot.count;
+  const double sx = fx_value(tot.hi[0], tot.lo[0], ex) + w.acc.fallback[s];
+  const double sy = fx_value(tot.hi[1], tot.lo[1], ey) + w.acc.fallback[S + s];
+  w.acc.fallback[s] = 0.0;
+  w.acc.fallback[S + s] = 0.0;
+  const double mx = tot.count ? sx / cnt : 0.0;
+  const double my = tot.count ? sy / cnt : 0.0;
+  e.counts[s] = (int32_t)tot.count;
+  e.slice_means[s] = mx;
+  e.slice_means[S + s] = my;
+  phx_wake_hist *h = e.hist;
+  const int nh = h->head;
+  const double zc = e.z_bins[s] + 0.5 * (e.z_bins[s + 1] - e.z_bins[s]);
+  h->times[(size_t)nh * S + s] = zc / g.beta_c;
+  double *mm = h->mom + (size_t)nh * 3 * S;
+  mm[s] = cnt;
+  mm[S + s] = cnt * mx;
+  mm[2 * S + s] = cnt * my;
 }
 
 __global__ void __launch_bounds__(PHX_BLOCK) k_engine_conv(EngineArgs a) {
   const phx_engine &e = a.e;
-  EngineWs w = engine_ws(e.ws, e.n, e.n_slices, a.n_kicks, a.T);
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
   const phx_turn_program &g = e.programs[e.turn_program[e.state->turn]];
   const WakeKicks &kk = *reinterpret_cast<const WakeKicks *>(g.kick);
-  wake_conv_block(e.hist, kk, e.n_slices, w.conv_part, a.T, w.conv_tickets, e.kicks);
   if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) *e.pending = 1;
+  wake_conv_block(e.hist, kk, e.n_slices, w.conv_part, a.T, e.kicks);
+}
+
+__global__ void k_engine_conv_reduce(EngineArgs a) {
+  const phx_engine &e = a.e;
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
+  const phx_turn_program &g = e.programs[e.turn_program[e.state->turn]];
+  const WakeKicks &kk = *reinterpret_cast<const WakeKicks *>(g.kick);
+  wake_conv_reduce(e.hist, kk, a.n_kicks, e.n_slices, w.conv_part, a.T, e.kicks);
 }
 
 __global__ void k_engine_reset(EngineArgs a, size_t clear_bytes) {
   const phx_engine &e = a.e;
-  // clear tickets, partials and accumulators of the workspace
   unsigned long long *p = reinterpret_cast<unsigned long long *>(e.ws);
   const size_t nw = clear_bytes / 8;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nw;
@@ -306,8 +338,10 @@ __global__ void k_engine_reset(EngineArgs a, size_t clear_bytes) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *e.turn_counter = 0;
     *e.pending = 0;
-    e.hist->n_filled = 0;
-    e.hist->head = e.hist->capacity - 1;
+    if (e.hist) {
+      e.hist->n_filled = 0;
+      e.hist->head = e.hist->capacity - 1;
+    }
     e.state->valid = 0;
   }
 }
@@ -332,6 +366,66 @@ int check_engine(const phx_engine *e, int *n_kicks, int *T) {
   return PHX_OK;
 }
 
+template <int MODE>
+int launch_pass(const EngineArgs &a, bool has_wake, hipStream_t s) {
+  const phx_engine &e = a.e;
+  const dim3 grid((unsigned)phx::n_chunks(e.n));
+  const bool rec = e.record_stats != 0;
+  if (MODE == 0 && has_wake) {
+    if (rec) k_engine_pass<MODE, true, true><<<grid, PHX_BLOCK, 0, s>>>(a);
+    else k_engine_pass<MODE, false, true><<<grid, PHX_BLOCK, 0, s>>>(a);
+  } else {
+    if (rec) k_engine_pass<MODE, true, false><<<grid, PHX_BLOCK, 0, s>>>(a);
+    else k_engine_pass<MODE, false, false><<<grid, PHX_BLOCK, 0, s>>>(a);
+  }
+  PHX_CHECK_LAUNCH("phx_engine: pass");
+  if (MODE == 0 && has_wake) {
+    if (rec) k_engine_fin<MODE, true, true><<<1, PHX_BLOCK, 0, s>>>(a);
+    else k_engine_fin<MODE, false, true><<<1, PHX_BLOCK, 0, s>>>(a);
+  } else {
+    if (rec) k_engine_fin<MODE, true, false><<<1, PHX_BLOCK, 0, s>>>(a);
+    else k_engine_fin<MODE, false, false><<<1, PHX_BLOCK, 0, s>>>(a);
+  }
+  PHX_CHECK_LAUNCH("phx_engine: finalize");
+  return PHX_OK;
+}
+
+int enqueue_stage(const EngineArgs &a, int stage, bool has_wake, hipStream_t s) {
+  const phx_engine &e = a.e;
+  if (stage == 0) return launch_pass<0>(a, has_wake, s);
+  if (stage == 1) {
+    k_engine_hist<<<dim3(hist_blocks(e.n)), EN_HIST_THREADS, acc_lds_bytes(e.n_slices, 2), s>>>(a);
+    PHX_CHECK_LAUNCH("phx_engine: hist");
+    k_engine_merge<<<dim3((e.n_slices + RD_SL - 1) / RD_SL), RD_SL * RD_G, 0, s>>>(a);
+    PHX_CHECK_LAUNCH("phx_engine: merge");
+    return PHX_OK;
+  }
+  const int tiles = (e.n_slices + WK_TILE - 1) / WK_TILE;
+  k_engine_conv<<<dim3(tiles, a.n_kicks, a.T), PHX_BLOCK, wake_conv_lds_bytes(e.n_slices), s>>>(a);
+  PHX_CHECK_LAUNCH("phx_engine: conv");
+  if (a.T > 1) {
+    const int tot = a.n_kicks * e.n_slices;
+    k_engine_conv_reduce<<<dim3((tot + 255) / 256), 256, 0, s>>>(a);
+    PHX_CHECK_LAUNCH("phx_engine: conv reduce");
+  }
+  return PHX_OK;
+}
+
+int enqueue_turn(const EngineArgs &a, bool has_wake, hipStream_t s) {
+  int rc = enqueue_stage(a, 0, has_wake, s);
+  if (rc || !has_wake) return rc;
+  rc = enqueue_stage(a, 1, has_wake, s);
+  if (rc) return rc;
+  return enqueue_stage(a, 2, has_wake, s);
+}
+
+int check_lds(const phx_engine *e) {
+  if (e->n_slices > 0 && (acc_lds_bytes(e->n_slices, 2) > phx_max_lds_bytes() ||
+                          wake_conv_lds_bytes(e->n_slices) > phx_max_lds_bytes()))
+    return phx::fail(PHX_E_LIMIT, "phx_engine: n_slices=%d exceeds LDS", e->n_slices);
+  return PHX_OK;
+}
+
 }  // namespace
 
 extern "C" int64_t phx_engine_ws_bytes(int64_t n, int32_t n_slices, int32_t n_kicks,
@@ -351,51 +445,14 @@ extern "C" int phx_engine_reset(const phx_engine *e, phx_stream_t stream) {
   return PHX_OK;
 }
 
-static int enqueue_stage(const EngineArgs &a, int stage, bool has_wake, hipStream_t s) {
-  const phx_engine &e = a.e;
-  if (stage == 0) {
-    const dim3 grid((unsigned)phx::n_chunks(e.n));
-    if (has_wake) {
-      if (e.record_stats) k_engine_pass<0, true, true><<<grid, PHX_BLOCK, 0, s>>>(a);
-      else k_engine_pass<0, false, true><<<grid, PHX_BLOCK, 0, s>>>(a);
-    } else {
-      if (e.record_stats) k_engine_pass<0, true, false><<<grid, PHX_BLOCK, 0, s>>>(a);
-      else k_engine_pass<0, false, false><<<grid, PHX_BLOCK, 0, s>>>(a);
-    }
-    PHX_CHECK_LAUNCH("phx_engine: pass");
-  } else if (stage == 1) {
-    const int64_t nc = phx::n_chunks(e.n);
-    const unsigned hb = (unsigned)(nc < EN_HIST_BLOCKS ? nc : EN_HIST_BLOCKS);
-    k_engine_hist<<<dim3(hb), PHX_BLOCK, acc_lds_bytes(e.n_slices, 2), s>>>(a);
-    PHX_CHECK_LAUNCH("phx_engine: hist");
-  } else {
-    const int tiles = (e.n_slices + WK_TILE - 1) / WK_TILE;
-    k_engine_conv<<<dim3(tiles, a.n_kicks, a.T), PHX_BLOCK, wake_conv_lds_bytes(e.n_slices),
-                    s>>>(a);
-    PHX_CHECK_LAUNCH("phx_engine: conv");
-  }
-  return PHX_OK;
-}
-
-static int enqueue_turn(const EngineArgs &a, bool has_wake, hipStream_t s) {
-  int rc = enqueue_stage(a, 0, has_wake, s);
-  if (rc || !has_wake) return rc;
-  rc = enqueue_stage(a, 1, has_wake, s);
-  if (rc) return rc;
-  return enqueue_stage(a, 2, has_wake, s);
-}
-
 extern "C" int phx_engine_turns(const phx_engine *e, int32_t n_turns, phx_stream_t stream) {
   int nk, T;
   int rc = check_engine(e, &nk, &T);
   if (rc) return rc;
   if (n_turns < 0) return phx::fail(PHX_E_INVALID, "phx_engine_turns: n_turns < 0");
+  rc = check_lds(e);
+  if (rc) return rc;
   const bool has_wake = e->n_slices > 0 && nk > 0;
-  if (has_wake) {
-    if (acc_lds_bytes(e->n_slices, 2) > phx_max_lds_bytes() ||
-        wake_conv_lds_bytes(e->n_slices) > phx_max_lds_bytes())
-      return phx::fail(PHX_E_LIMIT, "phx_engine_turns: n_slices=%d exceeds LDS", e->n_slices);
-  }
   EngineArgs a{*e, nk, T};
   for (int t = 0; t < n_turns; ++t) {
     rc = enqueue_turn(a, has_wake, phx::as_stream(stream));
@@ -407,6 +464,8 @@ extern "C" int phx_engine_turns(const phx_engine *e, int32_t n_turns, phx_stream
 extern "C" int phx_engine_stage(const phx_engine *e, int32_t stage, phx_stream_t stream) {
   int nk, T;
   int rc = check_engine(e, &nk, &T);
+  if (rc) return rc;
+  rc = check_lds(e);
   if (rc) return rc;
   const bool has_wake = e->n_slices > 0 && nk > 0;
   if (stage < 0 || stage > 2 || (stage > 0 && !has_wake))
@@ -420,11 +479,7 @@ extern "C" int phx_engine_flush(const phx_engine *e, phx_stream_t stream) {
   int rc = check_engine(e, &nk, &T);
   if (rc) return rc;
   EngineArgs a{*e, nk, T};
-  const dim3 grid((unsigned)phx::n_chunks(e->n));
-  if (e->record_stats) k_engine_pass<1, true, false><<<grid, PHX_BLOCK, 0, phx::as_stream(stream)>>>(a);
-  else k_engine_pass<1, false, false><<<grid, PHX_BLOCK, 0, phx::as_stream(stream)>>>(a);
-  PHX_CHECK_LAUNCH("phx_engine_flush");
-  return PHX_OK;
+  return launch_pass<1>(a, false, phx::as_stream(stream));
 }
 
 struct GraphHandle {

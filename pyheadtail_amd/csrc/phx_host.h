@@ -18,7 +18,7 @@ inline int64_t n_chunks(int64_t n) { return n <= 0 ? 0 : (n + (256 * 4) - 1) / (
 
 // Byte layout of the shared workspace of the reduction kernels.
 struct WsLayout {
-  static constexpr size_t kTicketBytes = 256;         // unsigned int tickets[64]
+  static constexpr size_t kTicketBytes = 256;         // reserved head of the workspace
   static constexpr size_t kPartialSlots = 64;         // doubles per chunk partial
   static size_t partial_bytes(int64_t n) { return (size_t)n_chunks(n) * kPartialSlots * 8; }
 };

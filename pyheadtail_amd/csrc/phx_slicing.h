@@ -65,11 +65,17 @@ struct AccTerms {
   const double *cb[PHX_MAX_TERMS];
 };
 
-struct AccGlobal {
-  unsigned int *counts;       // [S]
-  unsigned long long *hi;     // [T][S]
-  unsigned long long *lo;     // [T][S]
-  double *fallback;           // [T][S] non-finite or out-of-scale values
+// Per-block partial histograms (no cross-block atomics: device-scope atomics
+// to one address from many blocks serialise, so each block stores its own
+// histogram and a slice-parallel kernel merges them).
+//   cnt[h][s] (u32), hi[h][t][s], lo[h][t][s] (u64), fallback[t][s] (f64,
+//   only for non-finite or out-of-scale values, merged with FP64 atomics).
+struct AccParts {
+  unsigned int *cnt;
+  unsigned long long *hi;
+  unsigned long long *lo;
+  double *fallback;
+  int H;  // number of partial histograms (blocks)
 };
 
 struct AccLds {
@@ -78,28 +84,28 @@ struct AccLds {
   unsigned int *counts;
 };
 
-// Workspace: [tickets 256 B][chunk partials][counts S u32, padded][hi][lo][fallback]
+// Standalone workspace: [256 B reserved][chunk partials][partial histograms]
 inline size_t acc_partials_bytes(int64_t n) {
   const int64_t nc = n <= 0 ? 0 : (n + PHX_CHUNK - 1) / PHX_CHUNK;
-  return (size_t)nc * 64 * sizeof(double);
+  return ((size_t)nc * 64 * sizeof(double) + 255) / 256 * 256;
 }
 
-inline size_t acc_global_bytes(int S, int T) {
-  const size_t cb = ((size_t)S * 4 + 255) / 256 * 256;
-  return cb + (size_t)T * S * 24;   // multiple of 8 bytes
+inline size_t acc_parts_bytes(int H, int S, int T) {
+  return ((size_t)H * S * 4 + 255) / 256 * 256 + (size_t)H * T * S * 16 + (size_t)T * S * 8;
 }
 
-inline AccGlobal acc_global_layout(void *ws, int64_t n, int S, int T) {
-  char *p = reinterpret_cast<char *>(ws) + 256 + acc_partials_bytes(n);
-  AccGlobal g;
-  g.counts = reinterpret_cast<unsigned int *>(p);
-  p += ((size_t)S * 4 + 255) / 256 * 256;
-  g.hi = reinterpret_cast<unsigned long long *>(p);
-  p += (size_t)T * S * 8;
-  g.lo = reinterpret_cast<unsigned long long *>(p);
-  p += (size_t)T * S * 8;
-  g.fallback = reinterpret_cast<double *>(p);
-  return g;
+__host__ __device__ inline AccParts acc_parts_layout(void *base, int H, int S, int T) {
+  char *p = reinterpret_cast<char *>(base);
+  AccParts a;
+  a.H = H;
+  a.cnt = reinterpret_cast<unsigned int *>(p);
+  p += ((size_t)H * S * 4 + 255) / 256 * 256;
+  a.hi = reinterpret_cast<unsigned long long *>(p);
+  p += (size_t)H * T * S * 8;
+  a.lo = reinterpret_cast<unsigned long long *>(p);
+  p += (size_t)H * T * S * 8;
+  a.fallback = reinterpret_cast<double *>(p);
+  return a;
 }
 
 inline size_t acc_lds_bytes(int S, int T) { return (size_t)T * S * 16 + (size_t)S * 4; }
@@ -147,70 +153,83 @@ __device__ __forceinline__ bool fx_fits(double u, int e) {
   return fabs(q) < 4611686018427387904.0;  // 2^62
 }
 
+__device__ __forceinline__ void acc_add_value(const AccLds &L, int S, int i, int s, double u,
+                                              int e, double *fallback) {
+  if (fx_fits(u, e)) {
+    unsigned long long hi, lo;
+    fx_split(u, e, hi, lo);
+    atomicAdd(&L.hi[(size_t)i * S + s], hi);
+    atomicAdd(&L.lo[(size_t)i * S + s], lo);
+  } else {
+    atomicAdd(&fallback[(size_t)i * S + s], u);
+  }
+}
+
 __device__ __forceinline__ void acc_add_lds(const AccLds &L, int S, const AccTerms &t,
                                             const int *e, int s, const P6 &p,
-                                            const AccGlobal &g) {
+                                            double *fallback) {
   atomicAdd(&L.counts[s], 1u);
-  for (int i = 0; i < t.n; ++i) {
-    const double u = term_value(t, i, s, p);
-    if (fx_fits(u, e[i])) {
-      unsigned long long hi, lo;
-      fx_split(u, e[i], hi, lo);
-      atomicAdd(&L.hi[(size_t)i * S + s], hi);
-      atomicAdd(&L.lo[(size_t)i * S + s], lo);
-    } else {
-      atomicAdd(&g.fallback[(size_t)i * S + s], u);
-    }
-  }
+  for (int i = 0; i < t.n; ++i) acc_add_value(L, S, i, s, term_value(t, i, s, p), e[i], fallback);
 }
 
-__device__ __forceinline__ void acc_add_global(const AccGlobal &g, int S, const AccTerms &t,
-                                               const int *e, int s, const P6 &p) {
-  atomicAdd(&g.counts[s], 1u);
-  for (int i = 0; i < t.n; ++i) {
-    const double u = term_value(t, i, s, p);
-    if (fx_fits(u, e[i])) {
-      unsigned long long hi, lo;
-      fx_split(u, e[i], hi, lo);
-      atomicAdd(&g.hi[(size_t)i * S + s], hi);
-      atomicAdd(&g.lo[(size_t)i * S + s], lo);
-    } else {
-      atomicAdd(&g.fallback[(size_t)i * S + s], u);
-    }
-  }
-}
-
-__device__ __forceinline__ void acc_lds_flush(const AccLds &L, int S, int T,
-                                              const AccGlobal &g) {
+// Store this block's LDS histogram as partial number h (every entry, zeros
+// included, so the merge needs no prior clearing).
+__device__ __forceinline__ void acc_lds_store(const AccLds &L, int S, int T, const AccParts &P,
+                                              int h) {
   __syncthreads();
   for (int s = threadIdx.x; s < S; s += blockDim.x) {
-    const unsigned int c = L.counts[s];
-    if (c == 0u) continue;
-    atomicAdd(&g.counts[s], c);
+    P.cnt[(size_t)h * S + s] = L.counts[s];
     for (int i = 0; i < T; ++i) {
-      const unsigned long long h = L.hi[(size_t)i * S + s], l = L.lo[(size_t)i * S + s];
-      if (h) atomicAdd(&g.hi[(size_t)i * S + s], h);
-      if (l) atomicAdd(&g.lo[(size_t)i * S + s], l);
+      P.hi[((size_t)h * T + i) * S + s] = L.hi[(size_t)i * S + s];
+      P.lo[((size_t)h * T + i) * S + s] = L.lo[(size_t)i * S + s];
     }
   }
 }
 
-// Last block: exact sums -> outputs, then re-zero the global accumulators.
-__device__ __forceinline__ void acc_finalize(const AccGlobal &g, int S, const AccTerms &t,
-                                             const int *e, int32_t *counts, double *sums,
-                                             double *means) {
-  for (int s = threadIdx.x; s < S; s += blockDim.x) {
-    const unsigned int c = g.counts[s];
-    if (counts) counts[s] = (int32_t)c;
-    for (int i = 0; i < t.n; ++i) {
-      const size_t k = (size_t)i * S + s;
-      const double v = fx_value(g.hi[k], g.lo[k], e[i]) + g.fallback[k];
-      if (sums) sums[k] = v;
-      if (means) means[k] = c ? v / (double)c : 0.0;
-      g.hi[k] = 0ULL;
-      g.lo[k] = 0ULL;
-      g.fallback[k] = 0.0;
+// Merge of the H partial histograms for a tile of RD_SL slices: RD_G thread
+// groups stride over the partials, then the groups are combined (integer
+// sums: order-free).  Thread g == 0 of slice s gets the totals.
+#define RD_SL 32
+#define RD_G 8
+
+struct SliceTotals {
+  unsigned int count;
+  unsigned long long hi[PHX_MAX_TERMS], lo[PHX_MAX_TERMS];
+};
+
+__device__ __forceinline__ bool acc_merge(const AccParts &P, int S, int T, SliceTotals &out) {
+  __shared__ unsigned long long red[RD_G][RD_SL][2 * PHX_MAX_TERMS + 1];
+  const int sl = threadIdx.x % RD_SL, g = threadIdx.x / RD_SL;
+  const int s = blockIdx.x * RD_SL + sl;
+  unsigned long long c = 0, hs[PHX_MAX_TERMS], ls[PHX_MAX_TERMS];
+  for (int i = 0; i < T; ++i) hs[i] = ls[i] = 0ULL;
+  if (s < S) {
+    for (int h = g; h < P.H; h += RD_G) {
+      c += P.cnt[(size_t)h * S + s];
+      for (int i = 0; i < T; ++i) {
+        hs[i] += P.hi[((size_t)h * T + i) * S + s];
+        ls[i] += P.lo[((size_t)h * T + i) * S + s];
+      }
     }
-    g.counts[s] = 0u;
   }
+  red[g][sl][0] = c;
+  for (int i = 0; i < T; ++i) {
+    red[g][sl][1 + i] = hs[i];
+    red[g][sl][1 + PHX_MAX_TERMS + i] = ls[i];
+  }
+  __syncthreads();
+  if (g != 0 || s >= S) return false;
+  for (int q = 1; q < RD_G; ++q) {
+    c += red[q][sl][0];
+    for (int i = 0; i < T; ++i) {
+      hs[i] += red[q][sl][1 + i];
+      ls[i] += red[q][sl][1 + PHX_MAX_TERMS + i];
+    }
+  }
+  out.count = (unsigned int)c;
+  for (int i = 0; i < T; ++i) {
+    out.hi[i] = hs[i];
+    out.lo[i] = ls[i];
+  }
+  return true;
 }

@@ -29,10 +29,9 @@ struct BunchLayout {
 
 template <int K>
 __global__ void __launch_bounds__(PHX_BLOCK)
-    k_bunch_sums(Arrays6 arr, int64_t n, double *out, double *partials, unsigned int *ticket) {
+    k_bunch_sums(Arrays6 arr, int64_t n, double *partials) {
   using L = BunchLayout<K>;
   __shared__ double lds[L::NV * (PHX_BLOCK / PHX_WAVE)];
-  __shared__ double res[L::NV];
   double shift[K];
 #pragma unroll
   for (int i = 0; i < K; ++i) shift[i] = arr.a[i][0];
@@ -68,10 +67,17 @@ __global__ void __launch_bounds__(PHX_BLOCK)
   }
   const double r = block_reduce_slots<L::NV, L::MINM, L::MAXM>(v, lds);
   if (threadIdx.x < L::NV) partials[c * L::NV + threadIdx.x] = r;
-  if (last_block(ticket, gridDim.x)) {
-    reduce_partials<L::NV, L::MINM, L::MAXM>(partials, gridDim.x, res, lds);
-    if (threadIdx.x < L::NV) out[threadIdx.x] = res[threadIdx.x];
-  }
+}
+
+// One block folds the chunk partials in a fixed order.
+template <int K>
+__global__ void __launch_bounds__(PHX_BLOCK)
+    k_bunch_sums_fin(const double *partials, int64_t n_chunks, double *out) {
+  using L = BunchLayout<K>;
+  __shared__ double lds[L::NV * (PHX_BLOCK / PHX_WAVE)];
+  __shared__ double res[L::NV];
+  reduce_partials<L::NV, L::MINM, L::MAXM>(partials, n_chunks, res, lds);
+  if (threadIdx.x < L::NV) out[threadIdx.x] = res[threadIdx.x];
 }
 
 extern "C" int64_t phx_bunch_sums_len(int32_t k) {
@@ -89,18 +95,27 @@ extern "C" int phx_bunch_sums(const double *const *arrays, int32_t k, int64_t n,
     if (!arrays[i]) return phx::fail(PHX_E_INVALID, "phx_bunch_sums: array %d is null", i);
     a.a[i] = arrays[i];
   }
-  unsigned int *ticket = reinterpret_cast<unsigned int *>(ws);
   double *partials = reinterpret_cast<double *>(reinterpret_cast<char *>(ws) + phx::WsLayout::kTicketBytes);
-  const dim3 grid((unsigned)phx::n_chunks(n));
+  const int64_t nc = phx::n_chunks(n);
+  const dim3 grid((unsigned)nc);
   hipStream_t s = phx::as_stream(stream);
+#define PHX_BS(KK)                                                              \
+  case KK:                                                                      \
+    k_bunch_sums<KK><<<grid, PHX_BLOCK, 0, s>>>(a, n, partials);                \
+    k_bunch_sums_fin<KK><<<1, PHX_BLOCK, 0, s>>>(partials, nc, out);            \
+    break;
   switch (k) {
-    case 1: k_bunch_sums<1><<<grid, PHX_BLOCK, 0, s>>>(a, n, out, partials, ticket); break;
-    case 2: k_bunch_sums<2><<<grid, PHX_BLOCK, 0, s>>>(a, n, out, partials, ticket); break;
-    case 3: k_bunch_sums<3><<<grid, PHX_BLOCK, 0, s>>>(a, n, out, partials, ticket); break;
-    case 4: k_bunch_sums<4><<<grid, PHX_BLOCK, 0, s>>>(a, n, out, partials, ticket); break;
-    case 5: k_bunch_sums<5><<<grid, PHX_BLOCK, 0, s>>>(a, n, out, partials, ticket); break;
-    default: k_bunch_sums<6><<<grid, PHX_BLOCK, 0, s>>>(a, n, out, partials, ticket); break;
+    PHX_BS(1)
+    PHX_BS(2)
+    PHX_BS(3)
+    PHX_BS(4)
+    PHX_BS(5)
+    default:
+      k_bunch_sums<6><<<grid, PHX_BLOCK, 0, s>>>(a, n, partials);
+      k_bunch_sums_fin<6><<<1, PHX_BLOCK, 0, s>>>(partials, nc, out);
+      break;
   }
+#undef PHX_BS
   PHX_CHECK_LAUNCH("phx_bunch_sums");
   return PHX_OK;
 }
@@ -109,10 +124,8 @@ extern "C" int phx_bunch_sums(const double *const *arrays, int32_t k, int64_t n,
 // Slicing cuts: z sums of the current coordinates and max|u| of all six.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(PHX_BLOCK)
-    k_slice_cuts(Coords6 co, int64_t n, phx_slicer cfg, phx_slice_state *state, double *z_bins,
-                 double *partials, unsigned int *ticket) {
+    k_slice_cuts(Coords6 co, int64_t n, double *partials) {
   __shared__ double lds[ZB_NV * (PHX_BLOCK / PHX_WAVE)];
-  __shared__ double res[ZB_NV];
   const double shift = co.c[4][0];
   double v[ZB_NV];
 #pragma unroll
@@ -135,10 +148,15 @@ __global__ void __launch_bounds__(PHX_BLOCK)
   }
   const double r = block_reduce_slots<ZB_NV, ZB_MINM, ZB_MAXM>(v, lds);
   if (threadIdx.x < ZB_NV) partials[c * ZB_NV + threadIdx.x] = r;
-  if (last_block(ticket, gridDim.x)) {
-    reduce_partials<ZB_NV, ZB_MINM, ZB_MAXM>(partials, gridDim.x, res, lds);
-    zblock_finalize(res, cfg, n, state, z_bins, -1);
-  }
+}
+
+__global__ void __launch_bounds__(PHX_BLOCK)
+    k_slice_cuts_fin(const double *partials, int64_t n_chunks, phx_slicer cfg, int64_t n,
+                     phx_slice_state *state, double *z_bins) {
+  __shared__ double lds[ZB_NV * (PHX_BLOCK / PHX_WAVE)];
+  __shared__ double res[ZB_NV];
+  reduce_partials<ZB_NV, ZB_MINM, ZB_MAXM>(partials, n_chunks, res, lds);
+  zblock_finalize(res, cfg, n, state, z_bins, -1);
 }
 
 extern "C" int phx_slice_cuts(const double *const coords[6], int64_t n, const phx_slicer *cfg,
@@ -153,10 +171,11 @@ extern "C" int phx_slice_cuts(const double *const coords[6], int64_t n, const ph
     return phx::fail(PHX_E_INVALID, "phx_slice_cuts: bad cuts_mode %d", cfg->cuts_mode);
   Coords6 co{};
   for (int i = 0; i < 6; ++i) co.c[i] = coords[i];
-  unsigned int *ticket = reinterpret_cast<unsigned int *>(ws);
   double *partials = reinterpret_cast<double *>(reinterpret_cast<char *>(ws) + phx::WsLayout::kTicketBytes);
-  k_slice_cuts<<<dim3((unsigned)phx::n_chunks(n)), PHX_BLOCK, 0, phx::as_stream(stream)>>>(
-      co, n, *cfg, state, z_bins, partials, ticket);
+  const int64_t nc = phx::n_chunks(n);
+  hipStream_t s = phx::as_stream(stream);
+  k_slice_cuts<<<dim3((unsigned)nc), PHX_BLOCK, 0, s>>>(co, n, partials);
+  k_slice_cuts_fin<<<1, PHX_BLOCK, 0, s>>>(partials, nc, *cfg, n, state, z_bins);
   PHX_CHECK_LAUNCH("phx_slice_cuts");
   return PHX_OK;
 }
@@ -170,18 +189,17 @@ __global__ void k_fill_zero(unsigned long long *p, size_t words) {
     p[i] = 0ULL;
 }
 
+#define ACC_BLOCKS 256
+
 __global__ void __launch_bounds__(PHX_BLOCK)
-    k_slice_accumulate(Coords6 co, int64_t n, const phx_slice_state *state, AccTerms terms,
-                       const int32_t *idx_in, int32_t *idx_out, int32_t *counts,
-                       double *sums, double *means, AccGlobal g, unsigned int *ticket,
-                       int use_lds) {
+    k_slice_accumulate(Coords6 co, int64_t n, const phx_slice_state *state, int S, AccTerms terms,
+                       const int32_t *idx_in, int32_t *idx_out, AccParts P) {
   extern __shared__ unsigned long long acc_lds[];
   const double tail = state->tail, width = state->width;
-  const int S = state->n_slices;
   int e[PHX_MAX_TERMS];
   for (int t = 0; t < terms.n; ++t) e[t] = term_exponent(terms, t, state->absmax);
   AccLds L = acc_lds_layout(acc_lds, S, terms.n);
-  if (use_lds) acc_lds_zero(L, S, terms.n);
+  acc_lds_zero(L, S, terms.n);
   const int64_t n_chunks = (n + PHX_CHUNK - 1) / PHX_CHUNK;
   for (int64_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
     const int64_t base = c * PHX_CHUNK;
@@ -194,8 +212,7 @@ __global__ void __launch_bounds__(PHX_BLOCK)
         if (idx_in) {
           s = idx_in[j];
         } else {
-          const double f = slice_floor(z, tail, width);
-          s = x86_cvt_i32(f);
+          s = x86_cvt_i32(slice_floor(z, tail, width));
         }
         if (idx_out) idx_out[j] = s;
         if (s >= 0 && s < S) {
@@ -206,14 +223,31 @@ __global__ void __launch_bounds__(PHX_BLOCK)
           p.yp = co.c[3] ? co.c[3][j] : 0.0;
           p.z = z;
           p.dp = co.c[5] ? co.c[5][j] : 0.0;
-          if (use_lds) acc_add_lds(L, S, terms, e, s, p, g);
-          else acc_add_global(g, S, terms, e, s, p);
+          acc_add_lds(L, S, terms, e, s, p, P.fallback);
         }
       }
     }
   }
-  if (use_lds) acc_lds_flush(L, S, terms.n, g);
-  if (last_block(ticket, gridDim.x)) acc_finalize(g, S, terms, e, counts, sums, means);
+  acc_lds_store(L, S, terms.n, P, blockIdx.x);
+}
+
+// Merge the partial histograms: counts, exact sums and means per slice;
+// re-zero the FP64 fallback entries.
+__global__ void __launch_bounds__(PHX_BLOCK)
+    k_slice_acc_merge(AccParts P, int S, AccTerms terms, const phx_slice_state *state,
+                      int32_t *counts, double *sums, double *means) {
+  SliceTotals tot;
+  if (!acc_merge(P, S, terms.n, tot)) return;
+  const int s = blockIdx.x * RD_SL + threadIdx.x % RD_SL;
+  if (counts) counts[s] = (int32_t)tot.count;
+  for (int i = 0; i < terms.n; ++i) {
+    const int e = term_exponent(terms, i, state->absmax);
+    const size_t k = (size_t)i * S + s;
+    const double v = fx_value(tot.hi[i], tot.lo[i], e) + P.fallback[k];
+    P.fallback[k] = 0.0;
+    if (sums) sums[k] = v;
+    if (means) means[k] = tot.count ? v / (double)tot.count : 0.0;
+  }
 }
 
 extern "C" int phx_slice_accumulate(const double *const coords[6], int64_t n,
@@ -231,6 +265,11 @@ extern "C" int phx_slice_accumulate(const double *const coords[6], int64_t n,
                      PHX_MAX_TERMS);
   if (n_terms > 0 && (!terms || !sums))
     return phx::fail(PHX_E_INVALID, "phx_slice_accumulate: terms/sums null");
+  const size_t lds_bytes = acc_lds_bytes(n_slices, n_terms);
+  if (lds_bytes > phx_max_lds_bytes())
+    return phx::fail(PHX_E_LIMIT,
+                     "phx_slice_accumulate: %d slices x %d terms need %zu B of LDS (max %zu)",
+                     n_slices, n_terms, lds_bytes, phx_max_lds_bytes());
   AccTerms t{};
   t.n = n_terms;
   for (int i = 0; i < n_terms; ++i) {
@@ -246,25 +285,25 @@ extern "C" int phx_slice_accumulate(const double *const coords[6], int64_t n,
   Coords6 co{};
   for (int i = 0; i < 6; ++i) co.c[i] = coords[i];
   const int32_t S = n_slices;
-  AccGlobal g = acc_global_layout(ws, n, S, n_terms);
-  // the accumulators' offset depends on n, so a region used by earlier
-  // calls as chunk partials may hold stale data: clear it first
-  {
-    const size_t words = acc_global_bytes(S, n_terms) / 8;
-    k_fill_zero<<<dim3((unsigned)((words + 255) / 256 < 1024 ? (words + 255) / 256 : 1024)), 256, 0,
-                  phx::as_stream(stream)>>>(reinterpret_cast<unsigned long long *>(g.counts),
-                                            words);
+  const int64_t nc = phx::n_chunks(n);
+  const int H = (int)(nc < ACC_BLOCKS ? nc : ACC_BLOCKS);
+  char *base = reinterpret_cast<char *>(ws) + phx::WsLayout::kTicketBytes + acc_partials_bytes(n);
+  AccParts P = acc_parts_layout(base, H, S, n_terms);
+  hipStream_t s = phx::as_stream(stream);
+  // the FP64 fallback region may hold data of earlier calls with other
+  // sizes: clear it (the merge leaves it zero again)
+  const size_t fw = (size_t)n_terms * S;
+  if (fw) {
+    k_fill_zero<<<dim3((unsigned)((fw + 255) / 256 < 1024 ? (fw + 255) / 256 : 1024)), 256, 0,
+                  s>>>(reinterpret_cast<unsigned long long *>(P.fallback), fw);
     PHX_CHECK_LAUNCH("phx_slice_accumulate: clear");
   }
-  unsigned int *ticket = reinterpret_cast<unsigned int *>(ws) + 1;
-  const size_t lds_bytes = acc_lds_bytes(S, n_terms);
-  const int use_lds = lds_bytes <= phx_max_lds_bytes() ? 1 : 0;
-  const int64_t nc = phx::n_chunks(n);
-  const unsigned grid = (unsigned)(nc < 1024 ? nc : 1024);
-  k_slice_accumulate<<<dim3(grid), PHX_BLOCK, use_lds ? lds_bytes : 0,
-                       phx::as_stream(stream)>>>(co, n, state, t, idx_in, idx_out, counts, sums,
-                                                 means, g, ticket, use_lds);
+  k_slice_accumulate<<<dim3(H), PHX_BLOCK, lds_bytes, s>>>(co, n, state, S, t, idx_in, idx_out,
+                                                           P);
   PHX_CHECK_LAUNCH("phx_slice_accumulate");
+  k_slice_acc_merge<<<dim3((S + RD_SL - 1) / RD_SL), RD_SL * RD_G, 0, s>>>(P, S, t, state, counts,
+                                                                          sums, means);
+  PHX_CHECK_LAUNCH("phx_slice_accumulate: merge");
   return PHX_OK;
 }
 

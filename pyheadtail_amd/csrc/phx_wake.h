@@ -96,77 +96,83 @@ __device__ __forceinline__ void hist_push_device(phx_wake_hist *h, const double 
   __syncthreads();
 }
 
-// One block = (target tile, kick, turn).  Writes the convolution of that
-// turn for the tile into part[(k*T + t)*S + j]; the last block of the
-// (tile, kick) column sums the turns in order and scales by wake_factor
-// (WakeKick._accumulate_source_signal, wake_kicks.py:118-139).
+// One block = (target tile, kick, turn).  With one history slot the block
+// writes the kick directly, wake_factor * (0 + conv); otherwise it writes the
+// convolution of its turn into part[(k*T + t)*S + j] and wake_conv_reduce
+// sums the turns in order (WakeKick._accumulate_source_signal,
+// wake_kicks.py:118-139).
 struct WakeKicks {
   phx_wake_kick k[PHX_MAX_KICKS];
 };
 
 __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const WakeKicks &kk,
-                                                int S, double *part, int T,
-                                                unsigned int *tickets, double *out) {
+                                                int S, double *part, int T, double *out) {
   extern __shared__ double vbuf[];
   __shared__ double red[PHX_BLOCK];
   __shared__ double rwp[3];
   const int tile = blockIdx.x, ki = blockIdx.y, t = blockIdx.z;
   const phx_wake_kick &w = kk.k[ki];
   const int nf = h->n_filled, head = h->head, cap = h->capacity;
-  int n_turns = nf < w.n_turns_wake ? nf : w.n_turns_wake;
+  const int n_turns = nf < w.n_turns_wake ? nf : w.n_turns_wake;
+  if (t >= n_turns) return;
   const int j0 = tile * WK_TILE;
   const int jj = threadIdx.x % WK_TILE, g = threadIdx.x / WK_TILE;
   const int j = j0 + jj;
-  if (t < n_turns) {
-    const int slot = ring_slot(head, t, cap);
-    const double age = h->age[slot];
-    const double *tsrc = h->times + (size_t)slot * S;
-    const double *ttgt = h->times + (size_t)head * S;
-    const double *a = h->mom + ((size_t)slot * 3 + w.moment) * S;
-    if (threadIdx.x == 0 && w.fn == PHX_WAKE_RESISTIVE_WALL) rw_prefixes(w, h->beta[slot], rwp);
-    __syncthreads();
-    double acc = 0.0;
-    if (w.dot_product) {
-      // dt[i, j] = t_target[j] - (t_source[i] + age)     (wake_kicks.py:81-85)
-      if (j < S) {
-        const double tj = ttgt[j];
-        for (int i = g; i < S; i += WK_GROUPS) {
-          const double si = tsrc[i] + age;
-          acc = acc + a[i] * wake_eval(w, tj - si, rwp);
-        }
-      }
-    } else {
-      // dt = concat(t_tgt - s_src[-1], (t_tgt - s_src[0])[1:]) and
-      // np.convolve(moments, W, 'valid')[j] = sum_m a[m] W[j + S - 1 - m]
-      // (wake_kicks.py:109-116)
-      const double s_last = tsrc[S - 1] + age, s_first = tsrc[0] + age;
-      const int p0 = j0, p1 = min(j0 + WK_TILE - 1 + S - 1, 2 * S - 2);
-      for (int p = p0 + threadIdx.x; p <= p1; p += blockDim.x) {
-        const double dt = p <= S - 1 ? ttgt[p] - s_last : ttgt[p - S + 1] - s_first;
-        vbuf[p - p0] = wake_eval(w, dt, rwp);
-      }
-      __syncthreads();
-      if (j < S) {
-        for (int m = g; m < S; m += WK_GROUPS) acc = acc + a[m] * vbuf[j + S - 1 - m - p0];
+  const int slot = ring_slot(head, t, cap);
+  const double age = h->age[slot];
+  const double *tsrc = h->times + (size_t)slot * S;
+  const double *ttgt = h->times + (size_t)head * S;
+  const double *a = h->mom + ((size_t)slot * 3 + w.moment) * S;
+  if (threadIdx.x == 0 && w.fn == PHX_WAKE_RESISTIVE_WALL) rw_prefixes(w, h->beta[slot], rwp);
+  __syncthreads();
+  double acc = 0.0;
+  if (w.dot_product) {
+    // dt[i, j] = t_target[j] - (t_source[i] + age)     (wake_kicks.py:81-85)
+    if (j < S) {
+      const double tj = ttgt[j];
+      for (int i = g; i < S; i += WK_GROUPS) {
+        const double si = tsrc[i] + age;
+        acc = acc + a[i] * wake_eval(w, tj - si, rwp);
       }
     }
-    red[threadIdx.x] = acc;
+  } else {
+    // dt = concat(t_tgt - s_src[-1], (t_tgt - s_src[0])[1:]) and
+    // np.convolve(moments, W, 'valid')[j] = sum_m a[m] W[j + S - 1 - m]
+    // (wake_kicks.py:109-116)
+    const double s_last = tsrc[S - 1] + age, s_first = tsrc[0] + age;
+    const int p0 = j0, p1 = min(j0 + WK_TILE - 1 + S - 1, 2 * S - 2);
+    for (int p = p0 + threadIdx.x; p <= p1; p += blockDim.x) {
+      const double dt = p <= S - 1 ? ttgt[p] - s_last : ttgt[p - S + 1] - s_first;
+      vbuf[p - p0] = wake_eval(w, dt, rwp);
+    }
     __syncthreads();
-    if (g == 0 && j < S) {
-      double r = red[jj];
-      for (int q = 1; q < WK_GROUPS; ++q) r = r + red[q * WK_TILE + jj];
-      part[((size_t)ki * T + t) * S + j] = r;
+    if (j < S) {
+      for (int m = g; m < S; m += WK_GROUPS) acc = acc + a[m] * vbuf[j + S - 1 - m - p0];
     }
   }
-  // last arriving block of this (tile, kick) column sums the turns
-  unsigned int *tk = tickets + (size_t)ki * gridDim.x + tile;
-  if (last_block(tk, gridDim.z)) {
-    if (threadIdx.x < WK_TILE && j < S) {
-      double acc = 0.0;
-      for (int q = 0; q < n_turns; ++q) acc = acc + part[((size_t)ki * T + q) * S + j];
-      out[(size_t)ki * S + j] = w.wake_factor * acc;
-    }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (g == 0 && j < S) {
+    double r = red[jj];
+    for (int q = 1; q < WK_GROUPS; ++q) r = r + red[q * WK_TILE + jj];
+    if (T == 1) out[(size_t)ki * S + j] = w.wake_factor * (0.0 + r);
+    else part[((size_t)ki * T + t) * S + j] = r;
   }
+}
+
+// Sum of the per-turn convolutions in turn order, times the wake factor.
+__device__ __forceinline__ void wake_conv_reduce(const phx_wake_hist *h, const WakeKicks &kk,
+                                                 int n_kicks, int S, const double *part, int T,
+                                                 double *out) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n_kicks * S) return;
+  const int ki = idx / S, j = idx % S;
+  const phx_wake_kick &w = kk.k[ki];
+  const int nf = h->n_filled;
+  const int n_turns = nf < w.n_turns_wake ? nf : w.n_turns_wake;
+  double acc = 0.0;
+  for (int q = 0; q < n_turns; ++q) acc = acc + part[((size_t)ki * T + q) * S + j];
+  out[(size_t)ki * S + j] = w.wake_factor * acc;
 }
 
 inline size_t wake_conv_lds_bytes(int S) { return (size_t)(WK_TILE + S - 1) * sizeof(double); }

@@ -13,16 +13,14 @@ __global__ void k_hist_push(phx_wake_hist *h, const double *times, const int32_t
 }
 
 __global__ void __launch_bounds__(PHX_BLOCK)
-    k_wake_conv(const phx_wake_hist *h, WakeKicks kk, int S, double *part, int T,
-                unsigned int *tickets, double *out) {
-  wake_conv_block(h, kk, S, part, T, tickets, out);
+    k_wake_conv(const phx_wake_hist *h, WakeKicks kk, int S, double *part, int T, double *out) {
+  wake_conv_block(h, kk, S, part, T, out);
 }
 
-struct KickDesc {
-  int n;
-  int multiply[PHX_MAX_KICKS];
-  int plane[PHX_MAX_KICKS];
-};
+__global__ void k_wake_conv_reduce(const phx_wake_hist *h, WakeKicks kk, int n_kicks, int S,
+                                   const double *part, int T, double *out) {
+  wake_conv_reduce(h, kk, n_kicks, S, part, T, out);
+}
 
 __global__ void __launch_bounds__(PHX_BLOCK)
     k_wake_apply(Coords6 co, double *xp, double *yp, double *dp, int64_t n, const int32_t *idx,
@@ -69,8 +67,6 @@ extern "C" int phx_wake_hist_push(phx_wake_hist *hist, const double *times,
   return PHX_OK;
 }
 
-// Workspace of phx_wake_convolve: tickets then per-turn partials; it lives at
-// the end of the history record's own allocation (see wake_conv_ws_bytes).
 static int check_kicks(const phx_wake_kick *kicks, int32_t n_kicks, const char *who) {
   if (!kicks || n_kicks < 1) return phx::fail(PHX_E_INVALID, "%s: no kicks", who);
   if (n_kicks > PHX_MAX_KICKS) return phx::fail(PHX_E_LIMIT, "%s: n_kicks=%d > %d", who, n_kicks, PHX_MAX_KICKS);
@@ -88,18 +84,14 @@ static int check_kicks(const phx_wake_kick *kicks, int32_t n_kicks, const char *
 }
 
 size_t phx_wake_conv_ws_bytes(int32_t n_kicks, int32_t n_turns, int32_t n_slices) {
-  const int tiles = (n_slices + WK_TILE - 1) / WK_TILE;
-  const size_t tb = ((size_t)PHX_MAX_KICKS * tiles * 4 + 255) / 256 * 256;
-  return tb + (size_t)n_kicks * n_turns * n_slices * sizeof(double);
+  return (size_t)n_kicks * n_turns * n_slices * sizeof(double) + 256;
 }
 
 int phx_wake_convolve_ws(const phx_wake_hist *hist, const phx_wake_kick *kicks, int32_t n_kicks,
                          int32_t n_slices, int32_t max_turns, double *kicks_out, void *ws,
                          hipStream_t s) {
   const int tiles = (n_slices + WK_TILE - 1) / WK_TILE;
-  const size_t tb = ((size_t)PHX_MAX_KICKS * tiles * 4 + 255) / 256 * 256;
-  unsigned int *tickets = reinterpret_cast<unsigned int *>(ws);
-  double *part = reinterpret_cast<double *>(reinterpret_cast<char *>(ws) + tb);
+  double *part = reinterpret_cast<double *>(ws);
   WakeKicks kk{};
   int T = 1;
   for (int i = 0; i < n_kicks; ++i) {
@@ -111,8 +103,14 @@ int phx_wake_convolve_ws(const phx_wake_hist *hist, const phx_wake_kick *kicks, 
   if (lds > phx_max_lds_bytes())
     return phx::fail(PHX_E_LIMIT, "phx_wake_convolve: n_slices=%d too large for LDS", n_slices);
   k_wake_conv<<<dim3(tiles, n_kicks, T), PHX_BLOCK, lds, s>>>(hist, kk, n_slices, part, T,
-                                                             tickets, kicks_out);
+                                                             kicks_out);
   PHX_CHECK_LAUNCH("phx_wake_convolve");
+  if (T > 1) {
+    const int tot = n_kicks * n_slices;
+    k_wake_conv_reduce<<<dim3((tot + 255) / 256), 256, 0, s>>>(hist, kk, n_kicks, n_slices, part,
+                                                              T, kicks_out);
+    PHX_CHECK_LAUNCH("phx_wake_convolve: reduce");
+  }
   return PHX_OK;
 }
 

@@ -114,13 +114,27 @@ def test_slicing_indices_counts_means(n_slices, n_sigma_z, z_cuts):
         assert np.max(np.abs(got - ref)) <= TOL * np.std(u), name
     ref = R.std_per_slice(sd, ob.x)
     assert np.max(np.abs(ss.sigma_x.cpu().numpy() - ref)) <= 1e-11 * np.std(ob.x)
+    counts = sd.n_macroparticles_per_slice
     ref = R.emittance_per_slice(sd, ob.x, ob.xp, ob.dp) * ob.betagamma
-    got = ss.epsn_x.cpu().numpy()
-    assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(ref)
+    _check_slice_emittance(ss.epsn_x.cpu().numpy(), ref, counts, 3)
     ref = R.emittance_per_slice(sd, ob.z, ob.dp) * 4 * np.pi * ob.p0 / 1.602176634e-19
-    got = ss.epsn_z.cpu().numpy()
-    assert np.max(np.abs(got - ref)) <= 1e-9 * np.max(ref)
+    _check_slice_emittance(ss.epsn_z.cpu().numpy(), ref, counts, 2)
     assert int(ss.n_macroparticles_per_slice.sum()) + len(ss.particles_outside_cuts) == 50000
+
+
+def _check_slice_emittance(got, ref, counts, n_vars):
+    """Slices with n <= 1 give exactly 0.  With 2 <= n <= n_vars (2 for the
+    effective emittance, 3 with the dispersion correction) the particles
+    span a degenerate covariance, the determinant is zero in exact
+    arithmetic and both sides return the square root of rounding noise (or
+    NaN for a negative one): only smallness is checked.  Above that the
+    emittances agree to 1e-10 of the largest one."""
+    assert np.all(got[counts <= 1] == 0) and np.all(ref[counts <= 1] == 0)
+    good = counts > n_vars
+    scale = np.max(ref[good])
+    deg = got[(counts >= 2) & (counts <= n_vars)]
+    assert np.all(np.isnan(deg) | (np.abs(deg) <= 1e-6 * scale))
+    assert np.max(np.abs(got[good] - ref[good])) <= 1e-10 * scale
 
 
 def test_slicing_edges():
