@@ -178,8 +178,15 @@ __device__ __forceinline__ void apply_wake_kicks(P6 &p, int32_t idx, int n_slice
                                                  const phx_wake_kick *kd, int n_kicks,
                                                  const double *__restrict__ kicks) {
   if (idx < 0 || idx >= n_slices) return;
-  for (int k = 0; k < n_kicks; ++k) {
-    double v = kicks[(size_t)k * n_slices + idx];
+  // gather all kick values first so their loads are in flight together
+  double kv[PHX_MAX_KICKS];
+#pragma unroll
+  for (int k = 0; k < PHX_MAX_KICKS; ++k)
+    if (k < n_kicks) kv[k] = kicks[(size_t)k * n_slices + idx];
+#pragma unroll
+  for (int k = 0; k < PHX_MAX_KICKS; ++k) {
+    if (k >= n_kicks) break;
+    double v = kv[k];
     if (kd[k].multiply == 0) v = v * p.x;
     else if (kd[k].multiply == 2) v = v * p.y;
     if (kd[k].plane == PHX_PLANE_XP) p.xp = p.xp + v;
@@ -216,10 +223,10 @@ __device__ __forceinline__ double red_ident(int i) {
 
 // Block-reduce NV slots held by every thread; thread t < NV then holds the
 // block result of slot t in the return value (other threads: undefined).
-// lds: __shared__ double[NV * PHX_BLOCK / PHX_WAVE].
-template <int NV, unsigned long long MINM, unsigned long long MAXM>
+// lds: __shared__ double[NV * NT / PHX_WAVE] for a block of NT threads.
+template <int NV, unsigned long long MINM, unsigned long long MAXM, int NT = PHX_BLOCK>
 __device__ __forceinline__ double block_reduce_slots(double (&v)[NV], double *lds) {
-  constexpr int NW = PHX_BLOCK / PHX_WAVE;
+  constexpr int NW = NT / PHX_WAVE;
   const int lane = threadIdx.x % PHX_WAVE, wave = threadIdx.x / PHX_WAVE;
 #pragma unroll
   for (int i = 0; i < NV; ++i) {

@@ -34,7 +34,7 @@ namespace {
 inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
 
 struct EngineWs {
-  double *partials;   // [chunks][EN_NV]
+  double *partials;   // [EN_NV][chunks] (slot-major)
   AccParts acc;       // H partial histograms, T = 2 (x, y)
   double *conv_part;  // [n_kicks][T][S]
 };
@@ -183,69 +183,129 @@ __global__ void __launch_bounds__(PHX_BLOCK) k_engine_pass(EngineArgs a) {
     }
   }
   const double r = block_reduce_slots<L::NV, L::MINM, L::MAXM>(v, lds);
-  if (threadIdx.x < L::NV) w.partials[(int64_t)blockIdx.x * EN_NV + threadIdx.x] = r;
+  if (threadIdx.x < L::NV) w.partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = r;
 }
 
-// One block: fold the chunk partials, then the per-turn bookkeeping.
-template <int MODE, bool REC, bool WAKE>
+// Fold of slots [I0, I1) of the chunk partials (slot-major, so the loads
+// are coalesced) into res[I0..I1).  The order is that of the standalone
+// fold (reduce_partials): the first PHX_BLOCK threads stride over the
+// chunks, waves reduce by shuffles, the waves are combined in order; so the
+// cuts match the standalone slicing bit for bit whatever the block size.
+// Slots are independent: folding a subset gives the same values.
+template <int I0, int I1, unsigned long long MINM, unsigned long long MAXM>
+__device__ __forceinline__ void fold_slots(const double *part, int64_t nc, double *res,
+                                           double *lds) {
+  constexpr int NV = I1 - I0, NW = PHX_BLOCK / PHX_WAVE;
+  const int lane = threadIdx.x % PHX_WAVE, wave = threadIdx.x / PHX_WAVE;
+  if (threadIdx.x < PHX_BLOCK) {
+    double acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = red_ident<MINM, MAXM>(I0 + i);
+    for (int64_t c = threadIdx.x; c < nc; c += PHX_BLOCK) {
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+        acc[i] = red_slot<MINM, MAXM>(I0 + i, acc[i], part[(size_t)(I0 + i) * nc + c]);
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double r = acc[i];
+#pragma unroll
+      for (int off = PHX_WAVE / 2; off > 0; off >>= 1)
+        r = red_slot<MINM, MAXM>(I0 + i, r, __shfl_down(r, off, PHX_WAVE));
+      if (lane == 0) lds[i * NW + wave] = r;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    const int i = threadIdx.x, k = I0 + i;
+    double out = lds[i * NW];
+    for (int w = 1; w < NW; ++w) {
+      const double b = lds[i * NW + w];
+      if ((MINM >> k) & 1ULL) out = fmin(out, b);
+      else if ((MAXM >> k) & 1ULL) out = fmax(out, b);
+      else out = out + b;
+    }
+    res[k] = out;
+  }
+  __syncthreads();
+}
+
+// One block: fold the chunk partials, then the per-turn bookkeeping (for
+// maps without a wake field, and for the flush).
+template <int MODE, bool REC>
 __global__ void __launch_bounds__(PHX_BLOCK) k_engine_fin(EngineArgs a) {
-  using L = PassLayout<REC, WAKE>;
+  using L = PassLayout<REC, false>;
   const phx_engine &e = a.e;
   __shared__ double lds[L::NV * (PHX_BLOCK / PHX_WAVE)];
   __shared__ double res[L::NV];
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
   const int turn = *e.turn_counter;
   const int64_t nc = (e.n + PHX_CHUNK - 1) / PHX_CHUNK;
-  double acc[L::NV];
-#pragma unroll
-  for (int i = 0; i < L::NV; ++i) acc[i] = red_ident<L::MINM, L::MAXM>(i);
-  for (int64_t c = threadIdx.x; c < nc; c += PHX_BLOCK) {
-    const double *pc = w.partials + c * EN_NV;
-#pragma unroll
-    for (int i = 0; i < L::NV; ++i) acc[i] = red_slot<L::MINM, L::MAXM>(i, acc[i], pc[i]);
-  }
-  const double rr = block_reduce_slots<L::NV, L::MINM, L::MAXM>(acc, lds);
-  if (threadIdx.x < L::NV) res[threadIdx.x] = rr;
-  __syncthreads();
+  fold_slots<0, L::NV, L::MINM, L::MAXM>(w.partials, nc, res, lds);
   if (REC && threadIdx.x < EN_STAT_NV)
     e.stats[(size_t)turn * EN_STAT_NV + threadIdx.x] = res[L::SOFF + threadIdx.x];
-  if (MODE == 0) {
-    const phx_turn_program &g = e.programs[e.turn_program[turn]];
-    if (WAKE) {
-      zblock_finalize(res, g.slicer, e.n, e.state, e.z_bins, turn);
-      if (threadIdx.x == 0) {
-        // WakeField.track: ages += C/(beta c), then the new entry at age 0
-        phx_wake_hist *h = e.hist;
-        const int cap = h->capacity, nf = h->n_filled, head = h->head;
-        for (int i = 0; i < nf; ++i) {
-          const int sl = ring_slot(head, i, cap);
-          h->age[sl] = h->age[sl] + g.age_increment;
-        }
-        const int nh = (head + 1) % cap;
-        h->head = nh;
-        h->n_filled = nf + 1 < cap ? nf + 1 : cap;
-        h->age[nh] = 0.0;
-        h->beta[nh] = g.beta;
-      }
-    }
-    if (threadIdx.x == 0) *e.turn_counter = turn + 1;
-  } else {
-    if (threadIdx.x == 0) *e.pending = 0;
+  if (threadIdx.x == 0) {
+    if (MODE == 0) *e.turn_counter = turn + 1;
+    else *e.pending = 0;
   }
 }
 
-// Slice histogram of the new state: counts, sum x, sum y per slice, one
-// LDS histogram per block stored as a partial.
+// Slice histogram of the new state.  Prologue: every block folds the z
+// block of the pass partials in the same order (identical results) into the
+// cuts of this turn; block 0 publishes them, the statistics row and the wake-history
+// bookkeeping (WakeField.track: ages += C/(beta c), then the new entry at
+// age 0, impedances/wakes.py:134-142).  Then counts, sum x and sum y per
+// slice in an LDS histogram, stored as this block's partial.
+template <bool REC>
 __global__ void __launch_bounds__(EN_HIST_THREADS) k_engine_hist(EngineArgs a) {
+  using L = PassLayout<REC, true>;
   extern __shared__ unsigned long long acc_lds[];
+  __shared__ double lds[L::NV * (PHX_BLOCK / PHX_WAVE)];
+  __shared__ double res[L::NV];
+  __shared__ double sh_tail, sh_width;
+  __shared__ int sh_ex, sh_ey;
   const phx_engine &e = a.e;
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
   const int S = e.n_slices;
-  const double tail = e.state->tail, width = e.state->width;
-  const int ex = fx_exponent(e.state->absmax[0]);
-  const int ey = fx_exponent(e.state->absmax[2]);
-  AccLds L = acc_lds_layout(acc_lds, S, 2);
-  acc_lds_zero(L, S, 2);
+  const int turn = *e.turn_counter;
+  const phx_turn_program &g = e.programs[e.turn_program[turn]];
+  const int64_t ncp = (e.n + PHX_CHUNK - 1) / PHX_CHUNK;
+  // every block needs the z block; only block 0 writes the statistics row
+  fold_slots<0, ZB_NV, L::MINM, L::MAXM>(w.partials, ncp, res, lds);
+  if constexpr (REC) {
+    if (blockIdx.x == 0) fold_slots<ZB_NV, L::NV, L::MINM, L::MAXM>(w.partials, ncp, res, lds);
+  }
+  if (threadIdx.x == 0) {
+    phx_slice_state st;
+    compute_cuts(g.slicer, e.n, res[0], res[1], res[2], res[3], res[4], st);
+    for (int i = 0; i < 6; ++i) st.absmax[i] = res[5 + i];
+    st.turn = turn;
+    st.pad = 0;
+    sh_tail = st.tail;
+    sh_width = st.width;
+    sh_ex = fx_exponent(st.absmax[0]);
+    sh_ey = fx_exponent(st.absmax[2]);
+    if (blockIdx.x == 0) {
+      *e.state = st;
+      phx_wake_hist *h = e.hist;
+      const int cap = h->capacity, nf = h->n_filled, head = h->head;
+      for (int i = 0; i < nf; ++i) {
+        const int sl = ring_slot(head, i, cap);
+        h->age[sl] = h->age[sl] + g.age_increment;
+      }
+      const int nh = (head + 1) % cap;
+      h->head = nh;
+      h->n_filled = nf + 1 < cap ? nf + 1 : cap;
+      h->age[nh] = 0.0;
+      h->beta[nh] = g.beta;
+    }
+  }
+  if (REC && blockIdx.x == 0 && threadIdx.x < EN_STAT_NV)
+    e.stats[(size_t)turn * EN_STAT_NV + threadIdx.x] = res[L::SOFF + threadIdx.x];
+  AccLds A = acc_lds_layout(acc_lds, S, 2);
+  acc_lds_zero(A, S, 2);  // ends with a barrier: the shared cuts are visible
+  const double tail = sh_tail, width = sh_width;
+  const int ex = sh_ex, ey = sh_ey;
   constexpr int CH = EN_HIST_THREADS * PHX_PPT;
   const int64_t nc = (e.n + CH - 1) / CH;
   for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
@@ -267,14 +327,14 @@ __global__ void __launch_bounds__(EN_HIST_THREADS) k_engine_hist(EngineArgs a) {
         const double f = slice_floor(zz[k], tail, width);
         if (f >= 0.0 && f < (double)S) {
           const int s = (int)f;
-          atomicAdd(&L.counts[s], 1u);
-          acc_add_value(L, S, 0, s, xx[k], ex, w.acc.fallback);
-          acc_add_value(L, S, 1, s, yy[k], ey, w.acc.fallback);
+          atomicAdd(&A.counts[s], 1u);
+          acc_add_value(A, S, 0, s, xx[k], ex, w.acc.fallback);
+          acc_add_value(A, S, 1, s, yy[k], ey, w.acc.fallback);
         }
       }
     }
   }
-  acc_lds_store(L, S, 2, w.acc, blockIdx.x);
+  acc_lds_store(A, S, 2, w.acc, blockIdx.x);
 }
 
 // Merge the partial histograms; means and the new history entry:
@@ -285,9 +345,14 @@ __global__ void __launch_bounds__(RD_SL * RD_G) k_engine_merge(EngineArgs a) {
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
   const int S = e.n_slices;
   SliceTotals tot;
-  if (!acc_merge(w.acc, S, 2, tot)) return;
-  const int s = blockIdx.x * RD_SL + threadIdx.x % RD_SL;
   const int turn = e.state->turn;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *e.turn_counter = turn + 1;
+  if (!acc_merge<2>(w.acc, S, tot)) return;
+  const int s = blockIdx.x * RD_SL + threadIdx.x % RD_SL;
+  // bin edges as np.arange(tail, head + 1e-7*width, width)
+  const double tail = e.state->tail, width = e.state->width;
+  e.z_bins[s] = arange_value(tail, width, s);
+  if (s == S - 1) e.z_bins[S] = arange_value(tail, width, S);
   const phx_turn_program &g = e.programs[e.turn_program[turn]];
   const int ex = fx_exponent(e.state->absmax[0]);
   const int ey = fx_exponent(e.state->absmax[2]);
@@ -303,7 +368,8 @@ __global__ void __launch_bounds__(RD_SL * RD_G) k_engine_merge(EngineArgs a) {
   e.slice_means[S + s] = my;
   phx_wake_hist *h = e.hist;
   const int nh = h->head;
-  const double zc = e.z_bins[s] + 0.5 * (e.z_bins[s + 1] - e.z_bins[s]);
+  const double zl = arange_value(tail, width, s), zr = arange_value(tail, width, s + 1);
+  const double zc = zl + 0.5 * (zr - zl);
   h->times[(size_t)nh * S + s] = zc / g.beta_c;
   double *mm = h->mom + (size_t)nh * 3 * S;
   mm[s] = cnt;
@@ -372,20 +438,17 @@ int launch_pass(const EngineArgs &a, bool has_wake, hipStream_t s) {
   const dim3 grid((unsigned)phx::n_chunks(e.n));
   const bool rec = e.record_stats != 0;
   if (MODE == 0 && has_wake) {
+    // the histogram kernel folds the partials of this pass
     if (rec) k_engine_pass<MODE, true, true><<<grid, PHX_BLOCK, 0, s>>>(a);
     else k_engine_pass<MODE, false, true><<<grid, PHX_BLOCK, 0, s>>>(a);
-  } else {
-    if (rec) k_engine_pass<MODE, true, false><<<grid, PHX_BLOCK, 0, s>>>(a);
-    else k_engine_pass<MODE, false, false><<<grid, PHX_BLOCK, 0, s>>>(a);
+    PHX_CHECK_LAUNCH("phx_engine: pass");
+    return PHX_OK;
   }
+  if (rec) k_engine_pass<MODE, true, false><<<grid, PHX_BLOCK, 0, s>>>(a);
+  else k_engine_pass<MODE, false, false><<<grid, PHX_BLOCK, 0, s>>>(a);
   PHX_CHECK_LAUNCH("phx_engine: pass");
-  if (MODE == 0 && has_wake) {
-    if (rec) k_engine_fin<MODE, true, true><<<1, PHX_BLOCK, 0, s>>>(a);
-    else k_engine_fin<MODE, false, true><<<1, PHX_BLOCK, 0, s>>>(a);
-  } else {
-    if (rec) k_engine_fin<MODE, true, false><<<1, PHX_BLOCK, 0, s>>>(a);
-    else k_engine_fin<MODE, false, false><<<1, PHX_BLOCK, 0, s>>>(a);
-  }
+  if (rec) k_engine_fin<MODE, true><<<1, PHX_BLOCK, 0, s>>>(a);
+  else k_engine_fin<MODE, false><<<1, PHX_BLOCK, 0, s>>>(a);
   PHX_CHECK_LAUNCH("phx_engine: finalize");
   return PHX_OK;
 }
@@ -394,7 +457,10 @@ int enqueue_stage(const EngineArgs &a, int stage, bool has_wake, hipStream_t s) 
   const phx_engine &e = a.e;
   if (stage == 0) return launch_pass<0>(a, has_wake, s);
   if (stage == 1) {
-    k_engine_hist<<<dim3(hist_blocks(e.n)), EN_HIST_THREADS, acc_lds_bytes(e.n_slices, 2), s>>>(a);
+    if (e.record_stats)
+      k_engine_hist<true><<<dim3(hist_blocks(e.n)), EN_HIST_THREADS, acc_lds_bytes(e.n_slices, 2), s>>>(a);
+    else
+      k_engine_hist<false><<<dim3(hist_blocks(e.n)), EN_HIST_THREADS, acc_lds_bytes(e.n_slices, 2), s>>>(a);
     PHX_CHECK_LAUNCH("phx_engine: hist");
     k_engine_merge<<<dim3((e.n_slices + RD_SL - 1) / RD_SL), RD_SL * RD_G, 0, s>>>(a);
     PHX_CHECK_LAUNCH("phx_engine: merge");

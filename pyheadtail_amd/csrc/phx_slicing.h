@@ -68,12 +68,12 @@ struct AccTerms {
 // Per-block partial histograms (no cross-block atomics: device-scope atomics
 // to one address from many blocks serialise, so each block stores its own
 // histogram and a slice-parallel kernel merges them).
-//   cnt[h][s] (u32), hi[h][t][s], lo[h][t][s] (u64), fallback[t][s] (f64,
-//   only for non-finite or out-of-scale values, merged with FP64 atomics).
+//   cnt[h][s] (u32), hl[h][t][s] = {hi, lo} (u64 pairs), fallback[t][s]
+//   (f64, only for non-finite or out-of-scale values, merged with FP64
+//   atomics).
 struct AccParts {
   unsigned int *cnt;
-  unsigned long long *hi;
-  unsigned long long *lo;
+  ulonglong2 *hl;
   double *fallback;
   int H;  // number of partial histograms (blocks)
 };
@@ -100,10 +100,8 @@ __host__ __device__ inline AccParts acc_parts_layout(void *base, int H, int S, i
   a.H = H;
   a.cnt = reinterpret_cast<unsigned int *>(p);
   p += ((size_t)H * S * 4 + 255) / 256 * 256;
-  a.hi = reinterpret_cast<unsigned long long *>(p);
-  p += (size_t)H * T * S * 8;
-  a.lo = reinterpret_cast<unsigned long long *>(p);
-  p += (size_t)H * T * S * 8;
+  a.hl = reinterpret_cast<ulonglong2 *>(p);
+  p += (size_t)H * T * S * 16;
   a.fallback = reinterpret_cast<double *>(p);
   return a;
 }
@@ -180,8 +178,10 @@ __device__ __forceinline__ void acc_lds_store(const AccLds &L, int S, int T, con
   for (int s = threadIdx.x; s < S; s += blockDim.x) {
     P.cnt[(size_t)h * S + s] = L.counts[s];
     for (int i = 0; i < T; ++i) {
-      P.hi[((size_t)h * T + i) * S + s] = L.hi[(size_t)i * S + s];
-      P.lo[((size_t)h * T + i) * S + s] = L.lo[(size_t)i * S + s];
+      ulonglong2 v;
+      v.x = L.hi[(size_t)i * S + s];
+      v.y = L.lo[(size_t)i * S + s];
+      P.hl[((size_t)h * T + i) * S + s] = v;
     }
   }
 }
@@ -189,47 +189,51 @@ __device__ __forceinline__ void acc_lds_store(const AccLds &L, int S, int T, con
 // Merge of the H partial histograms for a tile of RD_SL slices: RD_G thread
 // groups stride over the partials, then the groups are combined (integer
 // sums: order-free).  Thread g == 0 of slice s gets the totals.
-#define RD_SL 32
-#define RD_G 8
+#define RD_SL 64
+#define RD_G 16
 
 struct SliceTotals {
   unsigned int count;
   unsigned long long hi[PHX_MAX_TERMS], lo[PHX_MAX_TERMS];
 };
 
-__device__ __forceinline__ bool acc_merge(const AccParts &P, int S, int T, SliceTotals &out) {
-  __shared__ unsigned long long red[RD_G][RD_SL][2 * PHX_MAX_TERMS + 1];
+template <int T>
+__device__ __forceinline__ bool acc_merge(const AccParts &P, int S, SliceTotals &out) {
+  __shared__ unsigned long long red[RD_G][RD_SL];
+  constexpr int TA = T > 0 ? T : 1;
   const int sl = threadIdx.x % RD_SL, g = threadIdx.x / RD_SL;
   const int s = blockIdx.x * RD_SL + sl;
-  unsigned long long c = 0, hs[PHX_MAX_TERMS], ls[PHX_MAX_TERMS];
-  for (int i = 0; i < T; ++i) hs[i] = ls[i] = 0ULL;
+  unsigned long long v[2 * TA + 1];
+#pragma unroll
+  for (int i = 0; i < 2 * TA + 1; ++i) v[i] = 0ULL;
   if (s < S) {
     for (int h = g; h < P.H; h += RD_G) {
-      c += P.cnt[(size_t)h * S + s];
+      v[0] += P.cnt[(size_t)h * S + s];
+#pragma unroll
       for (int i = 0; i < T; ++i) {
-        hs[i] += P.hi[((size_t)h * T + i) * S + s];
-        ls[i] += P.lo[((size_t)h * T + i) * S + s];
+        const ulonglong2 q = P.hl[((size_t)h * T + i) * S + s];
+        v[1 + i] += q.x;
+        v[1 + TA + i] += q.y;
       }
     }
   }
-  red[g][sl][0] = c;
-  for (int i = 0; i < T; ++i) {
-    red[g][sl][1 + i] = hs[i];
-    red[g][sl][1 + PHX_MAX_TERMS + i] = ls[i];
+  // combine the groups one value at a time through a small LDS buffer
+#pragma unroll
+  for (int i = 0; i < 2 * TA + 1; ++i) {
+    if (i > 0 && i <= TA && i > T) continue;
+    if (i > TA && i - TA > T) continue;
+    red[g][sl] = v[i];
+    __syncthreads();
+    if (g == 0)
+      for (int q = 1; q < RD_G; ++q) v[i] += red[q][sl];
+    __syncthreads();
   }
-  __syncthreads();
   if (g != 0 || s >= S) return false;
-  for (int q = 1; q < RD_G; ++q) {
-    c += red[q][sl][0];
-    for (int i = 0; i < T; ++i) {
-      hs[i] += red[q][sl][1 + i];
-      ls[i] += red[q][sl][1 + PHX_MAX_TERMS + i];
-    }
-  }
-  out.count = (unsigned int)c;
+  out.count = (unsigned int)v[0];
+#pragma unroll
   for (int i = 0; i < T; ++i) {
-    out.hi[i] = hs[i];
-    out.lo[i] = ls[i];
+    out.hi[i] = v[1 + i];
+    out.lo[i] = v[1 + TA + i];
   }
   return true;
 }

@@ -233,14 +233,16 @@ __global__ void __launch_bounds__(PHX_BLOCK)
 
 // Merge the partial histograms: counts, exact sums and means per slice;
 // re-zero the FP64 fallback entries.
-__global__ void __launch_bounds__(PHX_BLOCK)
+template <int T>
+__global__ void __launch_bounds__(RD_SL * RD_G)
     k_slice_acc_merge(AccParts P, int S, AccTerms terms, const phx_slice_state *state,
                       int32_t *counts, double *sums, double *means) {
   SliceTotals tot;
-  if (!acc_merge(P, S, terms.n, tot)) return;
+  if (!acc_merge<T>(P, S, tot)) return;
   const int s = blockIdx.x * RD_SL + threadIdx.x % RD_SL;
   if (counts) counts[s] = (int32_t)tot.count;
-  for (int i = 0; i < terms.n; ++i) {
+#pragma unroll
+  for (int i = 0; i < T; ++i) {
     const int e = term_exponent(terms, i, state->absmax);
     const size_t k = (size_t)i * S + s;
     const double v = fx_value(tot.hi[i], tot.lo[i], e) + P.fallback[k];
@@ -301,8 +303,14 @@ extern "C" int phx_slice_accumulate(const double *const coords[6], int64_t n,
   k_slice_accumulate<<<dim3(H), PHX_BLOCK, lds_bytes, s>>>(co, n, state, S, t, idx_in, idx_out,
                                                            P);
   PHX_CHECK_LAUNCH("phx_slice_accumulate");
-  k_slice_acc_merge<<<dim3((S + RD_SL - 1) / RD_SL), RD_SL * RD_G, 0, s>>>(P, S, t, state, counts,
-                                                                          sums, means);
+  const dim3 mg((S + RD_SL - 1) / RD_SL);
+  switch (n_terms) {
+#define PHX_MG(TT) \
+    case TT: k_slice_acc_merge<TT><<<mg, RD_SL * RD_G, 0, s>>>(P, S, t, state, counts, sums, means); break;
+    PHX_MG(0) PHX_MG(1) PHX_MG(2) PHX_MG(3) PHX_MG(4) PHX_MG(5) PHX_MG(6) PHX_MG(7)
+    default: k_slice_acc_merge<8><<<mg, RD_SL * RD_G, 0, s>>>(P, S, t, state, counts, sums, means);
+#undef PHX_MG
+  }
   PHX_CHECK_LAUNCH("phx_slice_accumulate: merge");
   return PHX_OK;
 }

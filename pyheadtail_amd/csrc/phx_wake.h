@@ -6,7 +6,7 @@
 
 #include "phx_device.h"
 
-#define WK_TILE 64                   // target slices per block
+#define WK_TILE 16                   // target slices per block
 #define WK_GROUPS (PHX_BLOCK / WK_TILE)
 
 // Evaluation of the wake function with the reference's operation order.

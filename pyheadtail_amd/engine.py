@@ -135,16 +135,30 @@ class FusedOneTurnMap(Element):
                 prog.kick[i] = k.record(state)
         return prog
 
+    @staticmethod
+    def _state_key(state):
+        return (state._gamma, state._beta, state._betagamma, state._p0)
+
     def _schedule(self, beam, n_turns):
+        """Per-turn programs from the host-scalar recurrence.  Once a turn
+        leaves the scalars unchanged, every later turn has the same program
+        and the same scalars (the recurrence is deterministic), so the rest
+        of the table is filled without evaluating it."""
         state = ScalarState(beam)
         progs, table, index = [], np.empty(n_turns, dtype=np.int32), {}
-        for t in range(n_turns):
+        t = 0
+        while t < n_turns:
+            before = self._state_key(state)
             p = self._program(state)
             key = bytes(memoryview(p).cast("B"))
             if key not in index:
                 index[key] = len(progs)
                 progs.append(key)
             table[t] = index[key]
+            t += 1
+            if self._state_key(state) == before:
+                table[t:] = index[key]
+                break
         state.write_back(beam)
         return progs, table
 
