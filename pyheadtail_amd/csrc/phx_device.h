@@ -73,116 +73,184 @@ __device__ __forceinline__ void detune(const phx_detuner &d, const P6 &p, double
   }
 }
 
-__device__ __forceinline__ void apply_tseg(P6 &p, const phx_tseg &t) {
-  double M00, M01, M10, M11, M22, M23, M32, M33;
-  if (t.n_detuners == 0) {
-    M00 = t.M00; M01 = t.M01; M10 = t.M10; M11 = t.M11;
-    M22 = t.M22; M23 = t.M23; M32 = t.M32; M33 = t.M33;
-  } else {
-    double dphi_x = t.dQ_x, dphi_y = t.dQ_y;
-    for (int i = 0; i < t.n_detuners; ++i) {
-      double qx, qy;
-      detune(t.det[i], p, qx, qy);
-      dphi_x = dphi_x + qx;
-      dphi_y = dphi_y + qy;
+// The element functions act on N particles at once: the element's constants
+// are read once and the N independent particles interleave (instruction-level
+// parallelism for the long FP64 chains).  The arithmetic of each particle is
+// the same for every N.
+template <int N>
+__device__ __forceinline__ void apply_tseg_n(P6 (&p)[N], const phx_tseg &t) {
+  double M00[N], M01[N], M10[N], M11[N], M22[N], M23[N], M32[N], M33[N];
+  const int nd = t.n_detuners;
+  if (nd == 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      M00[k] = t.M00; M01[k] = t.M01; M10[k] = t.M10; M11[k] = t.M11;
+      M22[k] = t.M22; M23[k] = t.M23; M32[k] = t.M32; M33[k] = t.M33;
     }
-    dphi_x = dphi_x * PHX_TWO_PI;
-    dphi_y = dphi_y * PHX_TWO_PI;
-    double sx, cx, sy, cy;
-    sincos(dphi_x, &sx, &cx);
-    sincos(dphi_y, &sy, &cy);
-    M00 = t.I00 * cx + t.J00 * sx;
-    M01 = t.I01 * cx + t.J01 * sx;
-    M10 = t.I10 * cx + t.J10 * sx;
-    M11 = t.I11 * cx + t.J11 * sx;
-    M22 = t.I22 * cy + t.J22 * sy;
-    M23 = t.I23 * cy + t.J23 * sy;
-    M32 = t.I32 * cy + t.J32 * sy;
-    M33 = t.I33 * cy + t.J33 * sy;
+  } else {
+    double dphi_x[N], dphi_y[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      dphi_x[k] = t.dQ_x;
+      dphi_y[k] = t.dQ_y;
+    }
+    for (int i = 0; i < nd; ++i) {
+      const phx_detuner &d = t.det[i];
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        double qx, qy;
+        detune(d, p[k], qx, qy);
+        dphi_x[k] = dphi_x[k] + qx;
+        dphi_y[k] = dphi_y[k] + qy;
+      }
+    }
+    const double I00 = t.I00, I01 = t.I01, I10 = t.I10, I11 = t.I11;
+    const double I22 = t.I22, I23 = t.I23, I32 = t.I32, I33 = t.I33;
+    const double J00 = t.J00, J01 = t.J01, J10 = t.J10, J11 = t.J11;
+    const double J22 = t.J22, J23 = t.J23, J32 = t.J32, J33 = t.J33;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const double ax = dphi_x[k] * PHX_TWO_PI;
+      const double ay = dphi_y[k] * PHX_TWO_PI;
+      double sx, cx, sy, cy;
+      sincos(ax, &sx, &cx);
+      sincos(ay, &sy, &cy);
+      M00[k] = I00 * cx + J00 * sx;
+      M01[k] = I01 * cx + J01 * sx;
+      M10[k] = I10 * cx + J10 * sx;
+      M11[k] = I11 * cx + J11 * sx;
+      M22[k] = I22 * cy + J22 * sy;
+      M23[k] = I23 * cy + J23 * sy;
+      M32[k] = I32 * cy + J32 * sy;
+      M33[k] = I33 * cy + J33 * sy;
+    }
   }
-  if (t.with_dispersion) {
-    p.x = p.x - t.D_x_s0 * p.dp;
-    p.y = p.y - t.D_y_s0 * p.dp;
-  }
-  const double x = p.x, xp = p.xp, y = p.y, yp = p.yp;
-  p.x = M00 * x + M01 * xp;
-  p.xp = M10 * x + M11 * xp;
-  p.y = M22 * y + M23 * yp;
-  p.yp = M32 * y + M33 * yp;
-  if (t.with_dispersion) {
-    p.x = p.x + t.D_x_s1 * p.dp;
-    p.y = p.y + t.D_y_s1 * p.dp;
+  const bool disp = t.with_dispersion != 0;
+  const double dx0 = t.D_x_s0, dy0 = t.D_y_s0, dx1 = t.D_x_s1, dy1 = t.D_y_s1;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    P6 &q = p[k];
+    if (disp) {
+      q.x = q.x - dx0 * q.dp;
+      q.y = q.y - dy0 * q.dp;
+    }
+    const double x = q.x, xp = q.xp, y = q.y, yp = q.yp;
+    q.x = M00[k] * x + M01[k] * xp;
+    q.xp = M10[k] * x + M11[k] * xp;
+    q.y = M22[k] * y + M23[k] * yp;
+    q.yp = M32[k] * y + M33[k] * yp;
+    if (disp) {
+      q.x = q.x + dx1 * q.dp;
+      q.y = q.y + dy1 * q.dp;
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
 // Longitudinal maps (trackers/longitudinal_tracking.py).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void apply_long_seq(P6 &p, const phx_long_seq &s) {
+template <int N>
+__device__ __forceinline__ void apply_long_seq_n(P6 (&p)[N], const phx_long_seq &s) {
   for (int i = 0; i < s.n_ops; ++i) {
     const phx_long_op &o = s.op[i];
     if (o.type == PHX_LONG_DRIFT) {
       // Drift.track: z = beta_ratio*z - eta(dp)*dp*length   (:107-111)
-      p.z = o.beta_ratio * p.z - (o.eta0 * p.dp) * o.length;
+      const double br = o.beta_ratio, eta0 = o.eta0, len = o.length;
+#pragma unroll
+      for (int k = 0; k < N; ++k) p[k].z = br * p[k].z - (eta0 * p[k].dp) * len;
     } else {
       // Kick.track_with_dispersion / track_without_dispersion (:185-203)
-      if (o.with_dispersion) {
-        p.x = p.x - o.D_x * p.dp;
-        p.y = p.y - o.D_y * p.dp;
-      }
-      const double phi = (o.harmonic * ((o.two_pi * p.z) / o.circumference) + o.phi_offset) +
-                         o.phi_lock;
-      double delta_p = p.dp * o.p0_before;
-      delta_p = delta_p + (o.amplitude * sin(phi) - o.p_increment);
-      p.dp = delta_p / o.p0_after;
-      if (o.with_dispersion) {
-        p.x = p.x + o.D_x * p.dp;
-        p.y = p.y + o.D_y * p.dp;
+      const bool disp = o.with_dispersion != 0;
+      const double Dx = o.D_x, Dy = o.D_y, h = o.harmonic, tp = o.two_pi, C = o.circumference;
+      const double off = o.phi_offset, lock = o.phi_lock, p0b = o.p0_before, p0a = o.p0_after;
+      const double amp = o.amplitude, pinc = o.p_increment;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        P6 &q = p[k];
+        if (disp) {
+          q.x = q.x - Dx * q.dp;
+          q.y = q.y - Dy * q.dp;
+        }
+        const double phi = (h * ((tp * q.z) / C) + off) + lock;
+        double delta_p = q.dp * p0b;
+        delta_p = delta_p + (amp * sin(phi) - pinc);
+        q.dp = delta_p / p0a;
+        if (disp) {
+          q.x = q.x + Dx * q.dp;
+          q.y = q.y + Dy * q.dp;
+        }
       }
     }
   }
   if (s.shrink) {
-    p.x = p.x * s.shrink_factor;
-    p.xp = p.xp * s.shrink_factor;
-    p.y = p.y * s.shrink_factor;
-    p.yp = p.yp * s.shrink_factor;
+    const double f = s.shrink_factor;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      p[k].x = p[k].x * f;
+      p[k].xp = p[k].xp * f;
+      p[k].y = p[k].y * f;
+      p[k].yp = p[k].yp * f;
+    }
   }
 }
 
-__device__ __forceinline__ void apply_linmap(P6 &p, const phx_linmap &m) {
+template <int N>
+__device__ __forceinline__ void apply_linmap_n(P6 (&p)[N], const phx_linmap &m) {
   // LinearMap.track_with_dispersion -> track_without_dispersion (:731-759)
-  p.x = p.x - m.D_x * p.dp;
-  p.y = p.y - m.D_y * p.dp;
-  const double z0 = p.z, dp0 = p.dp;
-  p.z = z0 * m.cosQ - (m.longfac * dp0) * m.sinQ;
-  p.dp = dp0 * m.cosQ + (z0 / m.longfac) * m.sinQ;
-  p.x = p.x + m.D_x * p.dp;
-  p.y = p.y + m.D_y * p.dp;
+  const double Dx = m.D_x, Dy = m.D_y, cq = m.cosQ, sq = m.sinQ, lf = m.longfac;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    P6 &q = p[k];
+    q.x = q.x - Dx * q.dp;
+    q.y = q.y - Dy * q.dp;
+    const double z0 = q.z, dp0 = q.dp;
+    q.z = z0 * cq - (lf * dp0) * sq;
+    q.dp = dp0 * cq + (z0 / lf) * sq;
+    q.x = q.x + Dx * q.dp;
+    q.y = q.y + Dy * q.dp;
+  }
 }
 
-__device__ __forceinline__ void apply_program(P6 &p, const phx_turn_program &g) {
+template <int N>
+__device__ __forceinline__ void apply_program_n(P6 (&p)[N], const phx_turn_program &g) {
   for (int e = 0; e < g.n_el; ++e) {
     const int k = g.el_index[e];
     switch (g.el_type[e]) {
-      case PHX_EL_TSEG: apply_tseg(p, g.tseg[k]); break;
-      case PHX_EL_LONG: apply_long_seq(p, g.lseq[k]); break;
-      default: apply_linmap(p, g.lin[k]); break;
+      case PHX_EL_TSEG: apply_tseg_n<N>(p, g.tseg[k]); break;
+      case PHX_EL_LONG: apply_long_seq_n<N>(p, g.lseq[k]); break;
+      default: apply_linmap_n<N>(p, g.lin[k]); break;
     }
   }
+}
+
+__device__ __forceinline__ P6 (&one(P6 &p))[1] { return *reinterpret_cast<P6(*)[1]>(&p); }
+__device__ __forceinline__ void apply_tseg(P6 &p, const phx_tseg &t) { apply_tseg_n<1>(one(p), t); }
+__device__ __forceinline__ void apply_long_seq(P6 &p, const phx_long_seq &s) {
+  apply_long_seq_n<1>(one(p), s);
+}
+__device__ __forceinline__ void apply_linmap(P6 &p, const phx_linmap &m) {
+  apply_linmap_n<1>(one(p), m);
+}
+__device__ __forceinline__ void apply_program(P6 &p, const phx_turn_program &g) {
+  apply_program_n<1>(one(p), g);
 }
 
 // ---------------------------------------------------------------------------
 // Wake kick application (impedances/wake_kicks.py:160-361), kicks in order.
 // ---------------------------------------------------------------------------
+// Kick k of slice idx is kicks[k * ks + idx * ss] (ks = n_slices, ss = 1 for
+// the [kick][slice] layout of the C ABI; ks = 1, ss = n_kicks for the
+// slice-major copy the engine keeps in LDS).
 __device__ __forceinline__ void apply_wake_kicks(P6 &p, int32_t idx, int n_slices,
                                                  const phx_wake_kick *kd, int n_kicks,
-                                                 const double *__restrict__ kicks) {
+                                                 const double *__restrict__ kicks, int ks,
+                                                 int ss) {
   if (idx < 0 || idx >= n_slices) return;
   // gather all kick values first so their loads are in flight together
   double kv[PHX_MAX_KICKS];
 #pragma unroll
   for (int k = 0; k < PHX_MAX_KICKS; ++k)
-    if (k < n_kicks) kv[k] = kicks[(size_t)k * n_slices + idx];
+    if (k < n_kicks) kv[k] = kicks[(size_t)k * ks + (size_t)idx * ss];
 #pragma unroll
   for (int k = 0; k < PHX_MAX_KICKS; ++k) {
     if (k >= n_kicks) break;
@@ -196,17 +264,18 @@ __device__ __forceinline__ void apply_wake_kicks(P6 &p, int32_t idx, int n_slice
 }
 
 // ---------------------------------------------------------------------------
-// Deterministic reductions.  Within a wave: a fixed shuffle-down tree whose
-// result is taken from lane 0; within a block: the wave results combined in
-// wave order.  Across chunks: each thread folds a fixed stride of chunks in
-// increasing order, then the same block reduction.  For a given particle
-// count the association order is fixed, so results are bit-reproducible.
+// Deterministic reductions.  Within a block: every thread stores its slot
+// values in LDS; for each slot, 16 threads each combine a fixed stride of
+// the PHX_BLOCK values (lanes q, q+16, q+32, ...), then one thread combines
+// the 16 results in order.  Across chunks: each thread folds a fixed stride
+// of chunks in increasing order, then the same block reduction.  For a given
+// particle count the association order is fixed, so results are
+// bit-reproducible, and a slot's result does not depend on which other
+// slots are reduced with it.  (An LDS transpose instead of wave shuffles:
+// the shuffle tree is a chain of dependent cross-lane permutes per slot.)
 // The operation of slot i is min if bit i of MINM is set, max if bit i of
-// MAXM is set, else sum (masks are compile-time so the slots stay in VGPRs).
+// MAXM is set, else sum.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double red_min(double a, double b) { return fmin(a, b); }
-__device__ __forceinline__ double red_max(double a, double b) { return fmax(a, b); }
-
 template <unsigned long long MINM, unsigned long long MAXM>
 __device__ __forceinline__ double red_slot(int i, double a, double b) {
   if ((MINM >> i) & 1ULL) return fmin(a, b);
@@ -221,34 +290,53 @@ __device__ __forceinline__ double red_ident(int i) {
   return 0.0;
 }
 
-// Block-reduce NV slots held by every thread; thread t < NV then holds the
-// block result of slot t in the return value (other threads: undefined).
-// lds: __shared__ double[NV * NT / PHX_WAVE] for a block of NT threads.
-template <int NV, unsigned long long MINM, unsigned long long MAXM, int NT = PHX_BLOCK>
-__device__ __forceinline__ double block_reduce_slots(double (&v)[NV], double *lds) {
-  constexpr int NW = NT / PHX_WAVE;
-  const int lane = threadIdx.x % PHX_WAVE, wave = threadIdx.x / PHX_WAVE;
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    double r = v[i];
-#pragma unroll
-    for (int off = PHX_WAVE / 2; off > 0; off >>= 1)
-      r = red_slot<MINM, MAXM>(i, r, __shfl_down(r, off, PHX_WAVE));
-    if (lane == 0) lds[i * NW + wave] = r;
-  }
-  __syncthreads();
+#define BR_GROUP 16  // slots per LDS round
+#define BR_PARTS 16  // partial results per slot
+
+// LDS doubles needed by block_reduce_slots for NV slots.
+#define BR_LDS(NV) (((NV) < BR_GROUP ? (NV) : BR_GROUP) * (PHX_BLOCK + BR_PARTS))
+
+// Block-reduce NV slots held by the first PHX_BLOCK threads of the block
+// (every thread of the block must call it; larger blocks contribute only
+// their first PHX_BLOCK threads).  Thread t < NV gets the block result of
+// slot t as the return value (other threads: 0).  Slot t's operation is that
+// of bit OFF + t of the masks.  lds: __shared__ double[BR_LDS(NV)].
+template <int NV, unsigned long long MINM, unsigned long long MAXM, int OFF = 0>
+__device__ __forceinline__ double block_reduce_slots(const double (&v)[NV], double *lds) {
+  constexpr int G = NV < BR_GROUP ? NV : BR_GROUP;
+  constexpr int STRIDE = PHX_BLOCK / BR_PARTS;
+  double *part = lds + G * PHX_BLOCK;
+  const int t = threadIdx.x;
   double out = 0.0;
-  if (threadIdx.x < NV) {
-    const int i = threadIdx.x;
-    out = lds[i * NW];
-    for (int w = 1; w < NW; ++w) {
-      const double b = lds[i * NW + w];
-      if ((MINM >> i) & 1ULL) out = fmin(out, b);
-      else if ((MAXM >> i) & 1ULL) out = fmax(out, b);
-      else out = out + b;
+#pragma unroll
+  for (int g0 = 0; g0 < NV; g0 += G) {
+    if (t < PHX_BLOCK) {
+#pragma unroll
+      for (int i = 0; i < G; ++i)
+        if (g0 + i < NV) lds[i * PHX_BLOCK + t] = v[g0 + i];
     }
+    __syncthreads();
+    if (t < G * BR_PARTS) {
+      const int i = t / BR_PARTS, q = t % BR_PARTS;
+      if (g0 + i < NV) {
+        const int k = OFF + g0 + i;
+        const double *row = lds + i * PHX_BLOCK;
+        double r = row[q];
+#pragma unroll
+        for (int m = 1; m < STRIDE; ++m) r = red_slot<MINM, MAXM>(k, r, row[m * BR_PARTS + q]);
+        part[i * BR_PARTS + q] = r;
+      }
+    }
+    __syncthreads();
+    if (t >= g0 && t < g0 + G && t < NV) {
+      const int i = t - g0, k = OFF + t;
+      double r = part[i * BR_PARTS];
+#pragma unroll
+      for (int q = 1; q < BR_PARTS; ++q) r = red_slot<MINM, MAXM>(k, r, part[i * BR_PARTS + q]);
+      out = r;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   return out;
 }
 

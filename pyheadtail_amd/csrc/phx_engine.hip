@@ -70,7 +70,15 @@ struct EngineArgs {
   phx_engine e;
   int n_kicks;
   int T;
+  int kick_lds;  // the pass stages the kick table in LDS (it fits)
 };
+
+// Largest kick table the pass stages in LDS (n_kicks * n_slices doubles).
+#define EN_KICK_LDS_MAX (48 * 1024)
+
+inline size_t kick_lds_bytes(const phx_engine &e, int n_kicks) {
+  return (size_t)n_kicks * (e.n_slices > 0 ? e.n_slices : 0) * sizeof(double);
+}
 
 __device__ __forceinline__ P6 load_p(const phx_engine &e, int64_t j) {
   P6 p;
@@ -92,11 +100,12 @@ __device__ __forceinline__ void store_p(const phx_engine &e, int64_t j, const P6
   e.coords[5][j] = p.dp;
 }
 
-__device__ __forceinline__ void kick_pending(P6 &p, const phx_turn_program &g, double tail,
-                                             double width, int S, const double *kicks) {
+// nk = min(program kicks, engine kicks): the kick table holds the latter.
+__device__ __forceinline__ void kick_pending(P6 &p, const phx_turn_program &g, int nk,
+                                             double tail, double width, int S,
+                                             const double *kicks, int ks, int ss) {
   const double f = slice_floor(p.z, tail, width);
-  if (f >= 0.0 && f < (double)S)
-    apply_wake_kicks(p, (int32_t)f, S, g.kick, g.n_kicks, kicks);
+  if (f >= 0.0 && f < (double)S) apply_wake_kicks(p, (int32_t)f, S, g.kick, nk, kicks, ks, ss);
 }
 
 // stats slots: [0..6) sum u, [6..12) sum (u-u0), [12..33) sum (u-u0)(v-v0)
@@ -132,7 +141,7 @@ template <int MODE, bool REC, bool WAKE>
 __global__ void __launch_bounds__(PHX_BLOCK) k_engine_pass(EngineArgs a) {
   using L = PassLayout<REC, WAKE>;
   const phx_engine &e = a.e;
-  __shared__ double lds[L::NV * (PHX_BLOCK / PHX_WAVE)];
+  __shared__ double lds[BR_LDS(L::NV)];
   __shared__ double sh[8];
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
   const int turn = *e.turn_counter;
@@ -141,10 +150,22 @@ __global__ void __launch_bounds__(PHX_BLOCK) k_engine_pass(EngineArgs a) {
   const int pend = *e.pending;
   const double tail = e.state->tail, width = e.state->width;
   const int S = e.n_slices;
+  // The pending kick table, slice-major in LDS: each particle's kicks are
+  // adjacent and the gathers stay on-chip.
+  extern __shared__ double kick_lds[];
+  const int nk = g.n_kicks < a.n_kicks ? g.n_kicks : a.n_kicks;
+  const bool klds = pend && a.kick_lds;
+  if (klds) {
+    const int tot = nk * S;
+    for (int i = threadIdx.x; i < tot; i += PHX_BLOCK) {
+      const int k = i / S, s = i - k * S;
+      kick_lds[s * nk + k] = e.kicks[i];
+    }
+  }
   if (threadIdx.x == 0) {
     // particle 0 sets the shifts of the shifted sums (stats.pyx:cov)
     P6 p = load_p(e, 0);
-    if (pend) kick_pending(p, g, tail, width, S, e.kicks);
+    if (pend) kick_pending(p, g, nk, tail, width, S, e.kicks, S, 1);
     sh[0] = p.x; sh[1] = p.xp; sh[2] = p.y; sh[3] = p.yp; sh[4] = p.z; sh[5] = p.dp;
     if (MODE == 0) apply_program(p, g);
     sh[6] = p.z;
@@ -158,21 +179,32 @@ __global__ void __launch_bounds__(PHX_BLOCK) k_engine_pass(EngineArgs a) {
   for (int i = 0; i < 6; ++i) shs[i] = sh[i];
   const double zshift = sh[6];
   const int64_t base = (int64_t)blockIdx.x * PHX_CHUNK + threadIdx.x;
+  // Loads first, stores last: no global store precedes the reads of the
+  // program, so they stay scalar loads read once per block.
   P6 p[PHX_PPT];
 #pragma unroll
   for (int k = 0; k < PHX_PPT; ++k) {
     const int64_t j = base + (int64_t)k * PHX_BLOCK;
     if (j < e.n) p[k] = load_p(e, j);
+    else p[k] = P6{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   }
+  if (pend) {
 #pragma unroll
-  for (int k = 0; k < PHX_PPT; ++k) {
-    const int64_t j = base + (int64_t)k * PHX_BLOCK;
-    if (j < e.n) {
-      if (pend) kick_pending(p[k], g, tail, width, S, e.kicks);
-      if (REC) stats_accumulate(&v[L::SOFF], p[k], shs);
-      if (MODE == 0) apply_program(p[k], g);
-      if (MODE == 0 || pend) store_p(e, j, p[k]);
-      if (WAKE) {
+    for (int k = 0; k < PHX_PPT; ++k) {
+      if (klds) kick_pending(p[k], g, nk, tail, width, S, kick_lds, 1, nk);
+      else kick_pending(p[k], g, nk, tail, width, S, e.kicks, S, 1);
+    }
+  }
+  if (REC) {
+#pragma unroll
+    for (int k = 0; k < PHX_PPT; ++k)
+      if (base + (int64_t)k * PHX_BLOCK < e.n) stats_accumulate(&v[L::SOFF], p[k], shs);
+  }
+  if (MODE == 0) apply_program_n<PHX_PPT>(p, g);
+  if (WAKE) {
+#pragma unroll
+    for (int k = 0; k < PHX_PPT; ++k) {
+      if (base + (int64_t)k * PHX_BLOCK < e.n) {
         double zb[ZB_NV];
 #pragma unroll
         for (int i = 0; i < ZB_NV; ++i) zb[i] = v[i];
@@ -182,51 +214,39 @@ __global__ void __launch_bounds__(PHX_BLOCK) k_engine_pass(EngineArgs a) {
       }
     }
   }
+  if (MODE == 0 || pend) {
+#pragma unroll
+    for (int k = 0; k < PHX_PPT; ++k) {
+      const int64_t j = base + (int64_t)k * PHX_BLOCK;
+      if (j < e.n) store_p(e, j, p[k]);
+    }
+  }
   const double r = block_reduce_slots<L::NV, L::MINM, L::MAXM>(v, lds);
   if (threadIdx.x < L::NV) w.partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = r;
 }
 
 // Fold of slots [I0, I1) of the chunk partials (slot-major, so the loads
-// are coalesced) into res[I0..I1).  The order is that of the standalone
-// fold (reduce_partials): the first PHX_BLOCK threads stride over the
-// chunks, waves reduce by shuffles, the waves are combined in order; so the
-// cuts match the standalone slicing bit for bit whatever the block size.
-// Slots are independent: folding a subset gives the same values.
+// are coalesced) into res[I0..I1), in the order of the standalone fold
+// (reduce_partials): the first PHX_BLOCK threads stride over the chunks,
+// then the block reduction; so the cuts match the standalone slicing bit for
+// bit whatever the block size.  Slots are independent: folding a subset
+// gives the same values.
 template <int I0, int I1, unsigned long long MINM, unsigned long long MAXM>
 __device__ __forceinline__ void fold_slots(const double *part, int64_t nc, double *res,
                                            double *lds) {
-  constexpr int NV = I1 - I0, NW = PHX_BLOCK / PHX_WAVE;
-  const int lane = threadIdx.x % PHX_WAVE, wave = threadIdx.x / PHX_WAVE;
-  if (threadIdx.x < PHX_BLOCK) {
-    double acc[NV];
+  constexpr int NV = I1 - I0;
+  double acc[NV];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) acc[i] = red_ident<MINM, MAXM>(I0 + i);
+  for (int i = 0; i < NV; ++i) acc[i] = red_ident<MINM, MAXM>(I0 + i);
+  if (threadIdx.x < PHX_BLOCK) {
     for (int64_t c = threadIdx.x; c < nc; c += PHX_BLOCK) {
 #pragma unroll
       for (int i = 0; i < NV; ++i)
         acc[i] = red_slot<MINM, MAXM>(I0 + i, acc[i], part[(size_t)(I0 + i) * nc + c]);
     }
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      double r = acc[i];
-#pragma unroll
-      for (int off = PHX_WAVE / 2; off > 0; off >>= 1)
-        r = red_slot<MINM, MAXM>(I0 + i, r, __shfl_down(r, off, PHX_WAVE));
-      if (lane == 0) lds[i * NW + wave] = r;
-    }
   }
-  __syncthreads();
-  if (threadIdx.x < NV) {
-    const int i = threadIdx.x, k = I0 + i;
-    double out = lds[i * NW];
-    for (int w = 1; w < NW; ++w) {
-      const double b = lds[i * NW + w];
-      if ((MINM >> k) & 1ULL) out = fmin(out, b);
-      else if ((MAXM >> k) & 1ULL) out = fmax(out, b);
-      else out = out + b;
-    }
-    res[k] = out;
-  }
+  const double r = block_reduce_slots<NV, MINM, MAXM, I0>(acc, lds);
+  if (threadIdx.x < NV) res[I0 + threadIdx.x] = r;
   __syncthreads();
 }
 
@@ -236,7 +256,7 @@ template <int MODE, bool REC>
 __global__ void __launch_bounds__(PHX_BLOCK) k_engine_fin(EngineArgs a) {
   using L = PassLayout<REC, false>;
   const phx_engine &e = a.e;
-  __shared__ double lds[L::NV * (PHX_BLOCK / PHX_WAVE)];
+  __shared__ double lds[BR_LDS(L::NV)];
   __shared__ double res[L::NV];
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
   const int turn = *e.turn_counter;
@@ -260,7 +280,7 @@ template <bool REC>
 __global__ void __launch_bounds__(EN_HIST_THREADS) k_engine_hist(EngineArgs a) {
   using L = PassLayout<REC, true>;
   extern __shared__ unsigned long long acc_lds[];
-  __shared__ double lds[L::NV * (PHX_BLOCK / PHX_WAVE)];
+  __shared__ double lds[BR_LDS(L::NV)];
   __shared__ double res[L::NV];
   __shared__ double sh_tail, sh_width;
   __shared__ int sh_ex, sh_ey;
@@ -432,20 +452,27 @@ int check_engine(const phx_engine *e, int *n_kicks, int *T) {
   return PHX_OK;
 }
 
+EngineArgs make_args(const phx_engine *e, int nk, int T) {
+  EngineArgs a{*e, nk, T, 0};
+  a.kick_lds = nk > 0 && e->n_slices > 0 && kick_lds_bytes(*e, nk) <= EN_KICK_LDS_MAX;
+  return a;
+}
+
 template <int MODE>
 int launch_pass(const EngineArgs &a, bool has_wake, hipStream_t s) {
   const phx_engine &e = a.e;
   const dim3 grid((unsigned)phx::n_chunks(e.n));
   const bool rec = e.record_stats != 0;
+  const size_t kl = a.kick_lds ? kick_lds_bytes(e, a.n_kicks) : 0;
   if (MODE == 0 && has_wake) {
     // the histogram kernel folds the partials of this pass
-    if (rec) k_engine_pass<MODE, true, true><<<grid, PHX_BLOCK, 0, s>>>(a);
-    else k_engine_pass<MODE, false, true><<<grid, PHX_BLOCK, 0, s>>>(a);
+    if (rec) k_engine_pass<MODE, true, true><<<grid, PHX_BLOCK, kl, s>>>(a);
+    else k_engine_pass<MODE, false, true><<<grid, PHX_BLOCK, kl, s>>>(a);
     PHX_CHECK_LAUNCH("phx_engine: pass");
     return PHX_OK;
   }
-  if (rec) k_engine_pass<MODE, true, false><<<grid, PHX_BLOCK, 0, s>>>(a);
-  else k_engine_pass<MODE, false, false><<<grid, PHX_BLOCK, 0, s>>>(a);
+  if (rec) k_engine_pass<MODE, true, false><<<grid, PHX_BLOCK, kl, s>>>(a);
+  else k_engine_pass<MODE, false, false><<<grid, PHX_BLOCK, kl, s>>>(a);
   PHX_CHECK_LAUNCH("phx_engine: pass");
   if (rec) k_engine_fin<MODE, true><<<1, PHX_BLOCK, 0, s>>>(a);
   else k_engine_fin<MODE, false><<<1, PHX_BLOCK, 0, s>>>(a);
@@ -487,7 +514,7 @@ int enqueue_turn(const EngineArgs &a, bool has_wake, hipStream_t s) {
 
 int check_lds(const phx_engine *e) {
   if (e->n_slices > 0 && (acc_lds_bytes(e->n_slices, 2) > phx_max_lds_bytes() ||
-                          wake_conv_lds_bytes(e->n_slices) > phx_max_lds_bytes()))
+                          wake_conv_lds_bytes(e->n_slices) + WK_STATIC_LDS > phx_max_lds_bytes()))
     return phx::fail(PHX_E_LIMIT, "phx_engine: n_slices=%d exceeds LDS", e->n_slices);
   return PHX_OK;
 }
@@ -504,7 +531,7 @@ extern "C" int phx_engine_reset(const phx_engine *e, phx_stream_t stream) {
   int nk, T;
   int rc = check_engine(e, &nk, &T);
   if (rc) return rc;
-  EngineArgs a{*e, nk, T};
+  EngineArgs a = make_args(e, nk, T);
   const size_t bytes = engine_ws_bytes(e->n, e->n_slices, nk, T);
   k_engine_reset<<<256, 256, 0, phx::as_stream(stream)>>>(a, bytes);
   PHX_CHECK_LAUNCH("phx_engine_reset");
@@ -519,7 +546,7 @@ extern "C" int phx_engine_turns(const phx_engine *e, int32_t n_turns, phx_stream
   rc = check_lds(e);
   if (rc) return rc;
   const bool has_wake = e->n_slices > 0 && nk > 0;
-  EngineArgs a{*e, nk, T};
+  EngineArgs a = make_args(e, nk, T);
   for (int t = 0; t < n_turns; ++t) {
     rc = enqueue_turn(a, has_wake, phx::as_stream(stream));
     if (rc) return rc;
@@ -536,7 +563,7 @@ extern "C" int phx_engine_stage(const phx_engine *e, int32_t stage, phx_stream_t
   const bool has_wake = e->n_slices > 0 && nk > 0;
   if (stage < 0 || stage > 2 || (stage > 0 && !has_wake))
     return phx::fail(PHX_E_INVALID, "phx_engine_stage: bad stage %d", stage);
-  EngineArgs a{*e, nk, T};
+  EngineArgs a = make_args(e, nk, T);
   return enqueue_stage(a, stage, has_wake, phx::as_stream(stream));
 }
 
@@ -544,7 +571,7 @@ extern "C" int phx_engine_flush(const phx_engine *e, phx_stream_t stream) {
   int nk, T;
   int rc = check_engine(e, &nk, &T);
   if (rc) return rc;
-  EngineArgs a{*e, nk, T};
+  EngineArgs a = make_args(e, nk, T);
   return launch_pass<1>(a, false, phx::as_stream(stream));
 }
 

@@ -187,10 +187,12 @@ __device__ __forceinline__ void acc_lds_store(const AccLds &L, int S, int T, con
 }
 
 // Merge of the H partial histograms for a tile of RD_SL slices: RD_G thread
-// groups stride over the partials, then the groups are combined (integer
-// sums: order-free).  Thread g == 0 of slice s gets the totals.
-#define RD_SL 64
-#define RD_G 16
+// groups stride over the partials, then the group sums are combined through
+// LDS, RD_V values per round (integer sums: order-free).  Thread g == 0 of
+// slice s gets the totals.
+#define RD_SL 16
+#define RD_G 64
+#define RD_V 5
 
 struct SliceTotals {
   unsigned int count;
@@ -199,41 +201,48 @@ struct SliceTotals {
 
 template <int T>
 __device__ __forceinline__ bool acc_merge(const AccParts &P, int S, SliceTotals &out) {
-  __shared__ unsigned long long red[RD_G][RD_SL];
-  constexpr int TA = T > 0 ? T : 1;
+  constexpr int NVAL = 2 * T + 1;
+  __shared__ unsigned long long red[RD_V][RD_G][RD_SL];
+  __shared__ unsigned long long tot[NVAL][RD_SL];
   const int sl = threadIdx.x % RD_SL, g = threadIdx.x / RD_SL;
   const int s = blockIdx.x * RD_SL + sl;
-  unsigned long long v[2 * TA + 1];
+  // v[0] = count, v[1 + 2i] = hi of term i, v[2 + 2i] = lo of term i
+  unsigned long long v[NVAL];
 #pragma unroll
-  for (int i = 0; i < 2 * TA + 1; ++i) v[i] = 0ULL;
+  for (int i = 0; i < NVAL; ++i) v[i] = 0ULL;
   if (s < S) {
     for (int h = g; h < P.H; h += RD_G) {
       v[0] += P.cnt[(size_t)h * S + s];
 #pragma unroll
       for (int i = 0; i < T; ++i) {
         const ulonglong2 q = P.hl[((size_t)h * T + i) * S + s];
-        v[1 + i] += q.x;
-        v[1 + TA + i] += q.y;
+        v[1 + 2 * i] += q.x;
+        v[2 + 2 * i] += q.y;
       }
     }
   }
-  // combine the groups one value at a time through a small LDS buffer
 #pragma unroll
-  for (int i = 0; i < 2 * TA + 1; ++i) {
-    if (i > 0 && i <= TA && i > T) continue;
-    if (i > TA && i - TA > T) continue;
-    red[g][sl] = v[i];
+  for (int r0 = 0; r0 < NVAL; r0 += RD_V) {
+#pragma unroll
+    for (int i = 0; i < RD_V; ++i)
+      if (r0 + i < NVAL) red[i][g][sl] = v[r0 + i];
     __syncthreads();
-    if (g == 0)
-      for (int q = 1; q < RD_G; ++q) v[i] += red[q][sl];
+    if (threadIdx.x < RD_V * RD_SL) {
+      const int i = threadIdx.x / RD_SL, c = threadIdx.x % RD_SL;
+      if (r0 + i < NVAL) {
+        unsigned long long acc = 0ULL;
+        for (int q = 0; q < RD_G; ++q) acc += red[i][q][c];
+        tot[r0 + i][c] = acc;
+      }
+    }
     __syncthreads();
   }
   if (g != 0 || s >= S) return false;
-  out.count = (unsigned int)v[0];
+  out.count = (unsigned int)tot[0][sl];
 #pragma unroll
   for (int i = 0; i < T; ++i) {
-    out.hi[i] = v[1 + i];
-    out.lo[i] = v[1 + TA + i];
+    out.hi[i] = tot[1 + 2 * i][sl];
+    out.lo[i] = tot[2 + 2 * i][sl];
   }
   return true;
 }

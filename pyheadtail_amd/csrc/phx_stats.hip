@@ -31,7 +31,7 @@ template <int K>
 __global__ void __launch_bounds__(PHX_BLOCK)
     k_bunch_sums(Arrays6 arr, int64_t n, double *partials) {
   using L = BunchLayout<K>;
-  __shared__ double lds[L::NV * (PHX_BLOCK / PHX_WAVE)];
+  __shared__ double lds[BR_LDS(L::NV)];
   double shift[K];
 #pragma unroll
   for (int i = 0; i < K; ++i) shift[i] = arr.a[i][0];
@@ -74,7 +74,7 @@ template <int K>
 __global__ void __launch_bounds__(PHX_BLOCK)
     k_bunch_sums_fin(const double *partials, int64_t n_chunks, double *out) {
   using L = BunchLayout<K>;
-  __shared__ double lds[L::NV * (PHX_BLOCK / PHX_WAVE)];
+  __shared__ double lds[BR_LDS(L::NV)];
   __shared__ double res[L::NV];
   reduce_partials<L::NV, L::MINM, L::MAXM>(partials, n_chunks, res, lds);
   if (threadIdx.x < L::NV) out[threadIdx.x] = res[threadIdx.x];
@@ -125,7 +125,7 @@ extern "C" int phx_bunch_sums(const double *const *arrays, int32_t k, int64_t n,
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(PHX_BLOCK)
     k_slice_cuts(Coords6 co, int64_t n, double *partials) {
-  __shared__ double lds[ZB_NV * (PHX_BLOCK / PHX_WAVE)];
+  __shared__ double lds[BR_LDS(ZB_NV)];
   const double shift = co.c[4][0];
   double v[ZB_NV];
 #pragma unroll
@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(PHX_BLOCK)
 __global__ void __launch_bounds__(PHX_BLOCK)
     k_slice_cuts_fin(const double *partials, int64_t n_chunks, phx_slicer cfg, int64_t n,
                      phx_slice_state *state, double *z_bins) {
-  __shared__ double lds[ZB_NV * (PHX_BLOCK / PHX_WAVE)];
+  __shared__ double lds[BR_LDS(ZB_NV)];
   __shared__ double res[ZB_NV];
   reduce_partials<ZB_NV, ZB_MINM, ZB_MAXM>(partials, n_chunks, res, lds);
   zblock_finalize(res, cfg, n, state, z_bins, -1);

@@ -107,6 +107,8 @@ struct WakeKicks {
 
 __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const WakeKicks &kk,
                                                 int S, double *part, int T, double *out) {
+  // LDS: [WK_TILE + S - 1] wake values (convolution form) or [S] source
+  // times (dot-product form), then [S] source moments
   extern __shared__ double vbuf[];
   __shared__ double red[PHX_BLOCK];
   __shared__ double rwp[3];
@@ -123,17 +125,18 @@ __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const Wa
   const double *tsrc = h->times + (size_t)slot * S;
   const double *ttgt = h->times + (size_t)head * S;
   const double *a = h->mom + ((size_t)slot * 3 + w.moment) * S;
+  double *abuf = vbuf + (WK_TILE + S - 1);
+  for (int i = threadIdx.x; i < S; i += blockDim.x) abuf[i] = a[i];
   if (threadIdx.x == 0 && w.fn == PHX_WAKE_RESISTIVE_WALL) rw_prefixes(w, h->beta[slot], rwp);
   __syncthreads();
   double acc = 0.0;
   if (w.dot_product) {
     // dt[i, j] = t_target[j] - (t_source[i] + age)     (wake_kicks.py:81-85)
+    for (int i = threadIdx.x; i < S; i += blockDim.x) vbuf[i] = tsrc[i] + age;
+    __syncthreads();
     if (j < S) {
       const double tj = ttgt[j];
-      for (int i = g; i < S; i += WK_GROUPS) {
-        const double si = tsrc[i] + age;
-        acc = acc + a[i] * wake_eval(w, tj - si, rwp);
-      }
+      for (int i = g; i < S; i += WK_GROUPS) acc = acc + abuf[i] * wake_eval(w, tj - vbuf[i], rwp);
     }
   } else {
     // dt = concat(t_tgt - s_src[-1], (t_tgt - s_src[0])[1:]) and
@@ -147,7 +150,7 @@ __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const Wa
     }
     __syncthreads();
     if (j < S) {
-      for (int m = g; m < S; m += WK_GROUPS) acc = acc + a[m] * vbuf[j + S - 1 - m - p0];
+      for (int m = g; m < S; m += WK_GROUPS) acc = acc + abuf[m] * vbuf[j + S - 1 - m - p0];
     }
   }
   red[threadIdx.x] = acc;
@@ -175,4 +178,6 @@ __device__ __forceinline__ void wake_conv_reduce(const phx_wake_hist *h, const W
   out[(size_t)ki * S + j] = w.wake_factor * acc;
 }
 
-inline size_t wake_conv_lds_bytes(int S) { return (size_t)(WK_TILE + S - 1) * sizeof(double); }
+// Dynamic LDS of wake_conv_block; the static part is PHX_BLOCK + 3 doubles.
+inline size_t wake_conv_lds_bytes(int S) { return (size_t)(WK_TILE + 2 * S - 1) * sizeof(double); }
+#define WK_STATIC_LDS ((PHX_BLOCK + 3) * sizeof(double))

@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(PHX_BLOCK)
     p.yp = yp ? yp[j] : 0.0;
     p.dp = dp ? dp[j] : 0.0;
     p.z = 0.0;
-    apply_wake_kicks(p, s, S, kd.k, n_kicks, kicks);
+    apply_wake_kicks(p, s, S, kd.k, n_kicks, kicks, S, 1);
     if (xp) xp[j] = p.xp;
     if (yp) yp[j] = p.yp;
     if (dp) dp[j] = p.dp;
@@ -100,7 +100,7 @@ int phx_wake_convolve_ws(const phx_wake_hist *hist, const phx_wake_kick *kicks, 
     T = tk > T ? tk : T;
   }
   const size_t lds = wake_conv_lds_bytes(n_slices);
-  if (lds > phx_max_lds_bytes())
+  if (lds + WK_STATIC_LDS > phx_max_lds_bytes())
     return phx::fail(PHX_E_LIMIT, "phx_wake_convolve: n_slices=%d too large for LDS", n_slices);
   k_wake_conv<<<dim3(tiles, n_kicks, T), PHX_BLOCK, lds, s>>>(hist, kk, n_slices, part, T,
                                                              kicks_out);
