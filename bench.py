@@ -43,7 +43,25 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-turns", type=int, default=100)
     ap.add_argument("--graph-turns", type=int, default=10)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of the pass's algorithmic bytes "
+                         "(48 B read + 48 B written per macroparticle), for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
+
+
+def copy_calibration(n, reps=5):
+    """Plain device-to-device copy of 6 FP64 arrays of n elements (the bytes a
+    turn pass must move), repeated reps times."""
+    import torch
+    torch.cuda.set_device(0)
+    src = torch.rand((6, n), dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": src.numel() * 8,
+                                           "bytes_written": dst.numel() * 8, "reps": reps}}))
 
 
 def cpu_baseline(n, turns):
@@ -96,6 +114,9 @@ def main():
               "(independent bunches would shard across GPUs without communication). "
               "Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.n)
+        return
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_turns)
 
     import numpy as np

@@ -11,8 +11,21 @@
 
 #define PHX_BLOCK 256
 #define PHX_WAVE 64
-#define PHX_PPT 4
+#define PHX_PPT 2
 #define PHX_CHUNK (PHX_BLOCK * PHX_PPT)
+// Reducing particle passes run a persistent grid of at most PHX_GRID blocks:
+// block b handles the particle groups b, b + G, b + 2G, ... of PHX_CHUNK
+// particles (thread t: particles g*PHX_CHUNK + k*PHX_BLOCK + t, k < PHX_PPT),
+// so block b's partial covers a fixed set of particles in a fixed order.
+#define PHX_GRID 512
+
+__host__ __device__ inline int64_t phx_n_groups(int64_t n) {
+  return n <= 0 ? 0 : (n + PHX_CHUNK - 1) / PHX_CHUNK;
+}
+__host__ __device__ inline int phx_pass_grid(int64_t n) {
+  const int64_t g = phx_n_groups(n);
+  return (int)(g < PHX_GRID ? g : PHX_GRID);
+}
 // NumPy's 2.*np.pi as a double
 #define PHX_TWO_PI 6.283185307179586
 #define PHX_PI 3.141592653589793
@@ -20,6 +33,17 @@
 struct P6 {
   double x, xp, y, yp, z, dp;
 };
+
+// Workgroup barrier for data exchanged through LDS only.  __syncthreads()
+// also waits for every outstanding global load and store of the wave (its
+// release fence covers global memory); this one waits for LDS operations
+// only, so particle loads and stores stay in flight across it.  Every
+// barrier of this library protects LDS data only.
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 __device__ __forceinline__ double p6_get(const P6 &p, int c) {
   switch (c) {
@@ -315,7 +339,7 @@ __device__ __forceinline__ double block_reduce_slots(const double (&v)[NV], doub
       for (int i = 0; i < G; ++i)
         if (g0 + i < NV) lds[i * PHX_BLOCK + t] = v[g0 + i];
     }
-    __syncthreads();
+    lds_sync();
     if (t < G * BR_PARTS) {
       const int i = t / BR_PARTS, q = t % BR_PARTS;
       if (g0 + i < NV) {
@@ -327,7 +351,7 @@ __device__ __forceinline__ double block_reduce_slots(const double (&v)[NV], doub
         part[i * BR_PARTS + q] = r;
       }
     }
-    __syncthreads();
+    lds_sync();
     if (t >= g0 && t < g0 + G && t < NV) {
       const int i = t - g0, k = OFF + t;
       double r = part[i * BR_PARTS];
@@ -335,7 +359,7 @@ __device__ __forceinline__ double block_reduce_slots(const double (&v)[NV], doub
       for (int q = 1; q < BR_PARTS; ++q) r = red_slot<MINM, MAXM>(k, r, part[i * BR_PARTS + q]);
       out = r;
     }
-    __syncthreads();
+    lds_sync();
   }
   return out;
 }
@@ -355,7 +379,7 @@ __device__ __forceinline__ void reduce_partials(const double *part, int64_t n_ch
   }
   const double r = block_reduce_slots<NV, MINM, MAXM>(acc, lds);
   if (threadIdx.x < NV) res[threadIdx.x] = r;
-  __syncthreads();
+  lds_sync();
 }
 
 // ---------------------------------------------------------------------------

@@ -34,7 +34,7 @@ namespace {
 inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
 
 struct EngineWs {
-  double *partials;   // [EN_NV][chunks] (slot-major)
+  double *partials;   // [EN_NV][pass grid] (slot-major)
   AccParts acc;       // H partial histograms, T = 2 (x, y)
   double *conv_part;  // [n_kicks][T][S]
 };
@@ -47,7 +47,7 @@ __host__ __device__ inline int hist_blocks(int64_t n) {
 __host__ __device__ inline EngineWs engine_ws(void *ws, int64_t n, int S) {
   EngineWs w;
   char *p = reinterpret_cast<char *>(ws);
-  const int64_t nc = (n + PHX_CHUNK - 1) / PHX_CHUNK;
+  const int64_t nc = phx_pass_grid(n);
   const int Se = S > 0 ? S : 1;
   w.partials = reinterpret_cast<double *>(p);
   p += ((size_t)nc * EN_NV * 8 + 255) / 256 * 256;
@@ -59,7 +59,7 @@ __host__ __device__ inline EngineWs engine_ws(void *ws, int64_t n, int S) {
 }
 
 size_t engine_ws_bytes(int64_t n, int S, int n_kicks, int T) {
-  const int64_t nc = (n + PHX_CHUNK - 1) / PHX_CHUNK;
+  const int64_t nc = phx_pass_grid(n);
   const int Se = S > 0 ? S : 1;
   size_t b = round256((size_t)nc * EN_NV * 8) + acc_parts_bytes(hist_blocks(n), Se, 2);
   b += (size_t)(n_kicks > 0 ? n_kicks : 1) * T * Se * 8;
@@ -73,8 +73,9 @@ struct EngineArgs {
   int kick_lds;  // the pass stages the kick table in LDS (it fits)
 };
 
-// Largest kick table the pass stages in LDS (n_kicks * n_slices doubles).
-#define EN_KICK_LDS_MAX (48 * 1024)
+// Largest kick table the pass stages in LDS, per wave (n_kicks * n_slices
+// doubles; four copies per block).
+#define EN_KICK_LDS_MAX (8 * 1024)
 
 inline size_t kick_lds_bytes(const phx_engine &e, int n_kicks) {
   return (size_t)n_kicks * (e.n_slices > 0 ? e.n_slices : 0) * sizeof(double);
@@ -137,91 +138,124 @@ struct PassLayout {
 };
 
 // MODE 0: a turn; MODE 1: flush (pending kick + final statistics only).
+// Persistent grid (phx_pass_grid blocks): each block walks its particle
+// groups with the next group's coordinates loaded while the current one is
+// computed, so the particle traffic overlaps the FP64 work.
+// No workgroup barrier or fence precedes the last read of the turn program
+// and the coordinate arrays are declared non-aliasing (__restrict__
+// arguments): otherwise the compiler must assume the program may have been
+// written and reloads it with vector loads that wait on the particle loads.
+// So the kick table is staged per wave (a wave needs no barrier to read what
+// its own lanes wrote to LDS) and particle 0 is computed by every lane.  Dynamic LDS: max(per-wave kick tables, block reduction).
 template <int MODE, bool REC, bool WAKE>
-__global__ void __launch_bounds__(PHX_BLOCK) k_engine_pass(EngineArgs a) {
+__global__ void __launch_bounds__(PHX_BLOCK)
+    k_engine_pass(EngineArgs a, double *__restrict__ x, double *__restrict__ xp,
+                  double *__restrict__ y, double *__restrict__ yp, double *__restrict__ z,
+                  double *__restrict__ dp, const phx_turn_program *__restrict__ programs,
+                  const double *__restrict__ kicks) {
   using L = PassLayout<REC, WAKE>;
   const phx_engine &e = a.e;
-  __shared__ double lds[BR_LDS(L::NV)];
-  __shared__ double sh[8];
+  extern __shared__ double pass_lds[];
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
+  const int64_t n = e.n;
   const int turn = *e.turn_counter;
   const int prog_turn = MODE == 0 ? turn : (turn > 0 ? turn - 1 : 0);
-  const phx_turn_program &g = e.programs[e.turn_program[prog_turn]];
+  const phx_turn_program &g = programs[e.turn_program[prog_turn]];
   const int pend = *e.pending;
   const double tail = e.state->tail, width = e.state->width;
   const int S = e.n_slices;
-  // The pending kick table, slice-major in LDS: each particle's kicks are
-  // adjacent and the gathers stay on-chip.
-  extern __shared__ double kick_lds[];
+  // This wave's copy of the pending kick table, slice-major: each
+  // particle's kicks are adjacent and the gathers stay on-chip.
   const int nk = g.n_kicks < a.n_kicks ? g.n_kicks : a.n_kicks;
   const bool klds = pend && a.kick_lds;
+  const int lane = threadIdx.x % PHX_WAVE;
+  double *ktab = pass_lds + (size_t)(threadIdx.x / PHX_WAVE) * nk * S;
   if (klds) {
     const int tot = nk * S;
-    for (int i = threadIdx.x; i < tot; i += PHX_BLOCK) {
+    for (int i = lane; i < tot; i += PHX_WAVE) {
       const int k = i / S, s = i - k * S;
-      kick_lds[s * nk + k] = e.kicks[i];
+      ktab[s * nk + k] = kicks[i];
     }
   }
-  if (threadIdx.x == 0) {
-    // particle 0 sets the shifts of the shifted sums (stats.pyx:cov)
-    P6 p = load_p(e, 0);
-    if (pend) kick_pending(p, g, nk, tail, width, S, e.kicks, S, 1);
-    sh[0] = p.x; sh[1] = p.xp; sh[2] = p.y; sh[3] = p.yp; sh[4] = p.z; sh[5] = p.dp;
-    if (MODE == 0) apply_program(p, g);
-    sh[6] = p.z;
+  const int64_t ng = phx_n_groups(n);
+  int64_t gi = blockIdx.x;
+  P6 cur[PHX_PPT], nxt[PHX_PPT];
+#pragma unroll
+  for (int k = 0; k < PHX_PPT; ++k) {
+    const int64_t j = gi * PHX_CHUNK + threadIdx.x + (int64_t)k * PHX_BLOCK;
+    cur[k] = j < n ? P6{x[j], xp[j], y[j], yp[j], z[j], dp[j]}
+                   : P6{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    nxt[k] = P6{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   }
-  __syncthreads();
+  // Particle 0 sets the shifts of the shifted sums (stats.pyx:cov): its
+  // state after the pending kick, and its z after this turn's elements.
+  // Every lane computes it (uniform values; once per block).
+  double src[7];
+  {
+    P6 q = P6{x[0], xp[0], y[0], yp[0], z[0], dp[0]};
+    if (pend) kick_pending(q, g, nk, tail, width, S, kicks, S, 1);
+    src[0] = q.x; src[1] = q.xp; src[2] = q.y; src[3] = q.yp; src[4] = q.z; src[5] = q.dp;
+    if (MODE == 0) apply_program(q, g);
+    src[6] = q.z;
+  }
   double v[L::NV];
 #pragma unroll
   for (int i = 0; i < L::NV; ++i) v[i] = red_ident<L::MINM, L::MAXM>(i);
   double shs[6];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) shs[i] = sh[i];
-  const double zshift = sh[6];
-  const int64_t base = (int64_t)blockIdx.x * PHX_CHUNK + threadIdx.x;
-  // Loads first, stores last: no global store precedes the reads of the
-  // program, so they stay scalar loads read once per block.
-  P6 p[PHX_PPT];
+  for (int i = 0; i < 6; ++i) shs[i] = src[i];
+  const double zshift = src[6];
+  for (; gi < ng; gi += gridDim.x) {
+    const int64_t base = gi * PHX_CHUNK + threadIdx.x;
+    const int64_t gn = gi + gridDim.x;
+    if (gn < ng) {
 #pragma unroll
-  for (int k = 0; k < PHX_PPT; ++k) {
-    const int64_t j = base + (int64_t)k * PHX_BLOCK;
-    if (j < e.n) p[k] = load_p(e, j);
-    else p[k] = P6{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  }
-  if (pend) {
-#pragma unroll
-    for (int k = 0; k < PHX_PPT; ++k) {
-      if (klds) kick_pending(p[k], g, nk, tail, width, S, kick_lds, 1, nk);
-      else kick_pending(p[k], g, nk, tail, width, S, e.kicks, S, 1);
-    }
-  }
-  if (REC) {
-#pragma unroll
-    for (int k = 0; k < PHX_PPT; ++k)
-      if (base + (int64_t)k * PHX_BLOCK < e.n) stats_accumulate(&v[L::SOFF], p[k], shs);
-  }
-  if (MODE == 0) apply_program_n<PHX_PPT>(p, g);
-  if (WAKE) {
-#pragma unroll
-    for (int k = 0; k < PHX_PPT; ++k) {
-      if (base + (int64_t)k * PHX_BLOCK < e.n) {
-        double zb[ZB_NV];
-#pragma unroll
-        for (int i = 0; i < ZB_NV; ++i) zb[i] = v[i];
-        zblock_accumulate(zb, p[k], zshift);
-#pragma unroll
-        for (int i = 0; i < ZB_NV; ++i) v[i] = zb[i];
+      for (int k = 0; k < PHX_PPT; ++k) {
+        const int64_t j = gn * PHX_CHUNK + threadIdx.x + (int64_t)k * PHX_BLOCK;
+        if (j < n) nxt[k] = P6{x[j], xp[j], y[j], yp[j], z[j], dp[j]};
       }
     }
-  }
-  if (MODE == 0 || pend) {
+    if (pend) {
 #pragma unroll
-    for (int k = 0; k < PHX_PPT; ++k) {
-      const int64_t j = base + (int64_t)k * PHX_BLOCK;
-      if (j < e.n) store_p(e, j, p[k]);
+      for (int k = 0; k < PHX_PPT; ++k) {
+        if (klds) kick_pending(cur[k], g, nk, tail, width, S, ktab, 1, nk);
+        else kick_pending(cur[k], g, nk, tail, width, S, kicks, S, 1);
+      }
     }
+    if (REC) {
+#pragma unroll
+      for (int k = 0; k < PHX_PPT; ++k)
+        if (base + (int64_t)k * PHX_BLOCK < n) stats_accumulate(&v[L::SOFF], cur[k], shs);
+    }
+    if (MODE == 0) apply_program_n<PHX_PPT>(cur, g);
+    if (WAKE) {
+#pragma unroll
+      for (int k = 0; k < PHX_PPT; ++k) {
+        if (base + (int64_t)k * PHX_BLOCK < n) {
+          double zb[ZB_NV];
+#pragma unroll
+          for (int i = 0; i < ZB_NV; ++i) zb[i] = v[i];
+          zblock_accumulate(zb, cur[k], zshift);
+#pragma unroll
+          for (int i = 0; i < ZB_NV; ++i) v[i] = zb[i];
+        }
+      }
+    }
+    if (MODE == 0 || pend) {
+#pragma unroll
+      for (int k = 0; k < PHX_PPT; ++k) {
+        const int64_t j = base + (int64_t)k * PHX_BLOCK;
+        if (j < n) {
+          x[j] = cur[k].x; xp[j] = cur[k].xp; y[j] = cur[k].y;
+          yp[j] = cur[k].yp; z[j] = cur[k].z; dp[j] = cur[k].dp;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PHX_PPT; ++k) cur[k] = nxt[k];
   }
-  const double r = block_reduce_slots<L::NV, L::MINM, L::MAXM>(v, lds);
+  lds_sync();  // the reduction reuses the kick tables' LDS
+  const double r = block_reduce_slots<L::NV, L::MINM, L::MAXM>(v, pass_lds);
   if (threadIdx.x < L::NV) w.partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = r;
 }
 
@@ -247,7 +281,7 @@ __device__ __forceinline__ void fold_slots(const double *part, int64_t nc, doubl
   }
   const double r = block_reduce_slots<NV, MINM, MAXM, I0>(acc, lds);
   if (threadIdx.x < NV) res[I0 + threadIdx.x] = r;
-  __syncthreads();
+  lds_sync();
 }
 
 // One block: fold the chunk partials, then the per-turn bookkeeping (for
@@ -260,7 +294,7 @@ __global__ void __launch_bounds__(PHX_BLOCK) k_engine_fin(EngineArgs a) {
   __shared__ double res[L::NV];
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
   const int turn = *e.turn_counter;
-  const int64_t nc = (e.n + PHX_CHUNK - 1) / PHX_CHUNK;
+  const int64_t nc = phx_pass_grid(e.n);
   fold_slots<0, L::NV, L::MINM, L::MAXM>(w.partials, nc, res, lds);
   if (REC && threadIdx.x < EN_STAT_NV)
     e.stats[(size_t)turn * EN_STAT_NV + threadIdx.x] = res[L::SOFF + threadIdx.x];
@@ -289,7 +323,7 @@ __global__ void __launch_bounds__(EN_HIST_THREADS) k_engine_hist(EngineArgs a) {
   const int S = e.n_slices;
   const int turn = *e.turn_counter;
   const phx_turn_program &g = e.programs[e.turn_program[turn]];
-  const int64_t ncp = (e.n + PHX_CHUNK - 1) / PHX_CHUNK;
+  const int64_t ncp = phx_pass_grid(e.n);
   // every block needs the z block; only block 0 writes the statistics row
   fold_slots<0, ZB_NV, L::MINM, L::MAXM>(w.partials, ncp, res, lds);
   if constexpr (REC) {
@@ -458,21 +492,33 @@ EngineArgs make_args(const phx_engine *e, int nk, int T) {
   return a;
 }
 
+template <bool REC, bool WAKE>
+size_t pass_lds_bytes(const EngineArgs &a) {
+  const size_t tables = a.kick_lds ? (PHX_BLOCK / PHX_WAVE) * kick_lds_bytes(a.e, a.n_kicks) : 0;
+  constexpr int NV = PassLayout<REC, WAKE>::NV;
+  const size_t red = BR_LDS(NV) * sizeof(double);
+  return tables > red ? tables : red;
+}
+
 template <int MODE>
 int launch_pass(const EngineArgs &a, bool has_wake, hipStream_t s) {
   const phx_engine &e = a.e;
-  const dim3 grid((unsigned)phx::n_chunks(e.n));
+  const dim3 grid((unsigned)phx_pass_grid(e.n));
   const bool rec = e.record_stats != 0;
-  const size_t kl = a.kick_lds ? kick_lds_bytes(e, a.n_kicks) : 0;
+  double *const *c = e.coords;
+#define PHX_PASS(R, W)                                                                   \
+  k_engine_pass<MODE, R, W><<<grid, PHX_BLOCK, pass_lds_bytes<R, W>(a), s>>>(            \
+      a, c[0], c[1], c[2], c[3], c[4], c[5], e.programs, e.kicks)
   if (MODE == 0 && has_wake) {
     // the histogram kernel folds the partials of this pass
-    if (rec) k_engine_pass<MODE, true, true><<<grid, PHX_BLOCK, kl, s>>>(a);
-    else k_engine_pass<MODE, false, true><<<grid, PHX_BLOCK, kl, s>>>(a);
+    if (rec) PHX_PASS(true, true);
+    else PHX_PASS(false, true);
     PHX_CHECK_LAUNCH("phx_engine: pass");
     return PHX_OK;
   }
-  if (rec) k_engine_pass<MODE, true, false><<<grid, PHX_BLOCK, kl, s>>>(a);
-  else k_engine_pass<MODE, false, false><<<grid, PHX_BLOCK, kl, s>>>(a);
+  if (rec) PHX_PASS(true, false);
+  else PHX_PASS(false, false);
+#undef PHX_PASS
   PHX_CHECK_LAUNCH("phx_engine: pass");
   if (rec) k_engine_fin<MODE, true><<<1, PHX_BLOCK, 0, s>>>(a);
   else k_engine_fin<MODE, false><<<1, PHX_BLOCK, 0, s>>>(a);

@@ -6,6 +6,7 @@
 #include <stdarg.h>
 
 #include "../../include/phx.h"
+#include "phx_device.h"
 
 namespace phx {
 
@@ -14,13 +15,12 @@ int fail(int code, const char *fmt, ...);
 
 inline hipStream_t as_stream(phx_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
-inline int64_t n_chunks(int64_t n) { return n <= 0 ? 0 : (n + (256 * 4) - 1) / (256 * 4); }
+inline int64_t n_chunks(int64_t n) { return phx_n_groups(n); }
 
 // Byte layout of the shared workspace of the reduction kernels.
 struct WsLayout {
   static constexpr size_t kTicketBytes = 256;         // reserved head of the workspace
-  static constexpr size_t kPartialSlots = 64;         // doubles per chunk partial
-  static size_t partial_bytes(int64_t n) { return (size_t)n_chunks(n) * kPartialSlots * 8; }
+
 };
 
 }  // namespace phx

@@ -49,10 +49,10 @@ __device__ __forceinline__ void zblock_finalize(const double *res, const phx_sli
     sh_tail = st.tail;
     sh_width = st.width;
   }
-  __syncthreads();
+  lds_sync();
   for (int i = threadIdx.x; i <= cfg.n_slices; i += blockDim.x)
     z_bins[i] = arange_value(sh_tail, sh_width, i);
-  __syncthreads();
+  lds_sync();
 }
 
 // ---------------------------------------------------------------------------
@@ -86,8 +86,7 @@ struct AccLds {
 
 // Standalone workspace: [256 B reserved][chunk partials][partial histograms]
 inline size_t acc_partials_bytes(int64_t n) {
-  const int64_t nc = n <= 0 ? 0 : (n + PHX_CHUNK - 1) / PHX_CHUNK;
-  return ((size_t)nc * 64 * sizeof(double) + 255) / 256 * 256;
+  return ((size_t)phx_pass_grid(n) * 64 * sizeof(double) + 255) / 256 * 256;
 }
 
 inline size_t acc_parts_bytes(int H, int S, int T) {
@@ -122,7 +121,7 @@ __device__ __forceinline__ void acc_lds_zero(const AccLds &L, int S, int T) {
     L.lo[i] = 0ULL;
   }
   for (int i = threadIdx.x; i < S; i += blockDim.x) L.counts[i] = 0u;
-  __syncthreads();
+  lds_sync();
 }
 
 // Bound of |term| from max|u| (centres are per-slice means, |c| <= max|u|).
@@ -174,7 +173,7 @@ __device__ __forceinline__ void acc_add_lds(const AccLds &L, int S, const AccTer
 // included, so the merge needs no prior clearing).
 __device__ __forceinline__ void acc_lds_store(const AccLds &L, int S, int T, const AccParts &P,
                                               int h) {
-  __syncthreads();
+  lds_sync();
   for (int s = threadIdx.x; s < S; s += blockDim.x) {
     P.cnt[(size_t)h * S + s] = L.counts[s];
     for (int i = 0; i < T; ++i) {
@@ -226,7 +225,7 @@ __device__ __forceinline__ bool acc_merge(const AccParts &P, int S, SliceTotals 
 #pragma unroll
     for (int i = 0; i < RD_V; ++i)
       if (r0 + i < NVAL) red[i][g][sl] = v[r0 + i];
-    __syncthreads();
+    lds_sync();
     if (threadIdx.x < RD_V * RD_SL) {
       const int i = threadIdx.x / RD_SL, c = threadIdx.x % RD_SL;
       if (r0 + i < NVAL) {
@@ -235,7 +234,7 @@ __device__ __forceinline__ bool acc_merge(const AccParts &P, int S, SliceTotals 
         tot[r0 + i][c] = acc;
       }
     }
-    __syncthreads();
+    lds_sync();
   }
   if (g != 0 || s >= S) return false;
   out.count = (unsigned int)tot[0][sl];

@@ -38,8 +38,9 @@ __global__ void __launch_bounds__(PHX_BLOCK)
   double v[L::NV];
 #pragma unroll
   for (int i = 0; i < L::NV; ++i) v[i] = red_ident<L::MINM, L::MAXM>(i);
-  const int64_t c = blockIdx.x;
-  const int64_t base = c * PHX_CHUNK;
+  const int64_t ng = phx_n_groups(n);
+  for (int64_t gi = blockIdx.x; gi < ng; gi += gridDim.x) {
+  const int64_t base = gi * PHX_CHUNK;
 #pragma unroll
   for (int k = 0; k < PHX_PPT; ++k) {
     const int64_t j = base + (int64_t)k * PHX_BLOCK + threadIdx.x;
@@ -65,8 +66,9 @@ __global__ void __launch_bounds__(PHX_BLOCK)
       }
     }
   }
+  }
   const double r = block_reduce_slots<L::NV, L::MINM, L::MAXM>(v, lds);
-  if (threadIdx.x < L::NV) partials[c * L::NV + threadIdx.x] = r;
+  if (threadIdx.x < L::NV) partials[(int64_t)blockIdx.x * L::NV + threadIdx.x] = r;
 }
 
 // One block folds the chunk partials in a fixed order.
@@ -96,7 +98,7 @@ extern "C" int phx_bunch_sums(const double *const *arrays, int32_t k, int64_t n,
     a.a[i] = arrays[i];
   }
   double *partials = reinterpret_cast<double *>(reinterpret_cast<char *>(ws) + phx::WsLayout::kTicketBytes);
-  const int64_t nc = phx::n_chunks(n);
+  const int64_t nc = phx_pass_grid(n);
   const dim3 grid((unsigned)nc);
   hipStream_t s = phx::as_stream(stream);
 #define PHX_BS(KK)                                                              \
@@ -130,8 +132,9 @@ __global__ void __launch_bounds__(PHX_BLOCK)
   double v[ZB_NV];
 #pragma unroll
   for (int i = 0; i < ZB_NV; ++i) v[i] = red_ident<ZB_MINM, ZB_MAXM>(i);
-  const int64_t c = blockIdx.x;
-  const int64_t base = c * PHX_CHUNK;
+  const int64_t ng = phx_n_groups(n);
+  for (int64_t gi = blockIdx.x; gi < ng; gi += gridDim.x) {
+  const int64_t base = gi * PHX_CHUNK;
 #pragma unroll
   for (int k = 0; k < PHX_PPT; ++k) {
     const int64_t j = base + (int64_t)k * PHX_BLOCK + threadIdx.x;
@@ -146,8 +149,9 @@ __global__ void __launch_bounds__(PHX_BLOCK)
       zblock_accumulate(v, p, shift);
     }
   }
+  }
   const double r = block_reduce_slots<ZB_NV, ZB_MINM, ZB_MAXM>(v, lds);
-  if (threadIdx.x < ZB_NV) partials[c * ZB_NV + threadIdx.x] = r;
+  if (threadIdx.x < ZB_NV) partials[(int64_t)blockIdx.x * ZB_NV + threadIdx.x] = r;
 }
 
 __global__ void __launch_bounds__(PHX_BLOCK)
@@ -172,7 +176,7 @@ extern "C" int phx_slice_cuts(const double *const coords[6], int64_t n, const ph
   Coords6 co{};
   for (int i = 0; i < 6; ++i) co.c[i] = coords[i];
   double *partials = reinterpret_cast<double *>(reinterpret_cast<char *>(ws) + phx::WsLayout::kTicketBytes);
-  const int64_t nc = phx::n_chunks(n);
+  const int64_t nc = phx_pass_grid(n);
   hipStream_t s = phx::as_stream(stream);
   k_slice_cuts<<<dim3((unsigned)nc), PHX_BLOCK, 0, s>>>(co, n, partials);
   k_slice_cuts_fin<<<1, PHX_BLOCK, 0, s>>>(partials, nc, *cfg, n, state, z_bins);

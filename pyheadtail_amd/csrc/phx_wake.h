@@ -82,7 +82,7 @@ __device__ __forceinline__ void hist_push_device(phx_wake_hist *h, const double 
     h->beta[nh] = beta;
     sh_head = nh;
   }
-  __syncthreads();
+  lds_sync();
   const int nh = sh_head;
   double *tt = h->times + (size_t)nh * S;
   double *mm = h->mom + (size_t)nh * 3 * S;
@@ -93,7 +93,7 @@ __device__ __forceinline__ void hist_push_device(phx_wake_hist *h, const double 
     mm[S + s] = mean_x ? c * mean_x[s] : 0.0;
     mm[2 * S + s] = mean_y ? c * mean_y[s] : 0.0;
   }
-  __syncthreads();
+  lds_sync();
 }
 
 // One block = (target tile, kick, turn).  With one history slot the block
@@ -128,12 +128,12 @@ __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const Wa
   double *abuf = vbuf + (WK_TILE + S - 1);
   for (int i = threadIdx.x; i < S; i += blockDim.x) abuf[i] = a[i];
   if (threadIdx.x == 0 && w.fn == PHX_WAKE_RESISTIVE_WALL) rw_prefixes(w, h->beta[slot], rwp);
-  __syncthreads();
+  lds_sync();
   double acc = 0.0;
   if (w.dot_product) {
     // dt[i, j] = t_target[j] - (t_source[i] + age)     (wake_kicks.py:81-85)
     for (int i = threadIdx.x; i < S; i += blockDim.x) vbuf[i] = tsrc[i] + age;
-    __syncthreads();
+    lds_sync();
     if (j < S) {
       const double tj = ttgt[j];
       for (int i = g; i < S; i += WK_GROUPS) acc = acc + abuf[i] * wake_eval(w, tj - vbuf[i], rwp);
@@ -148,13 +148,13 @@ __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const Wa
       const double dt = p <= S - 1 ? ttgt[p] - s_last : ttgt[p - S + 1] - s_first;
       vbuf[p - p0] = wake_eval(w, dt, rwp);
     }
-    __syncthreads();
+    lds_sync();
     if (j < S) {
       for (int m = g; m < S; m += WK_GROUPS) acc = acc + abuf[m] * vbuf[j + S - 1 - m - p0];
     }
   }
   red[threadIdx.x] = acc;
-  __syncthreads();
+  lds_sync();
   if (g == 0 && j < S) {
     double r = red[jj];
     for (int q = 1; q < WK_GROUPS; ++q) r = r + red[q * WK_TILE + jj];

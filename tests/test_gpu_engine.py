@@ -130,3 +130,4 @@ def test_engine_multiturn_wake_equals_elements():
     ga, gb = to_numpy(a), to_numpy(b)
     for k in COORDS:
         np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)
+
