@@ -98,7 +98,7 @@ def build_machine(p0_ref):
     from pyheadtail_amd.machines.synchrotron import Synchrotron
     from pyheadtail_amd.particles.slicing import UniformBinSlicer
     from pyheadtail_amd.impedances.wakes import WakeField, CircularResonator
-    mach = Synchrotron(optics_mode='smooth', charge=e, mass=m_p, gamma=27.7,
+    mach = Synchrotron(optics_mode='smooth', charge=e, mass=m_p, p0=p0_ref,
                        circumference=6911.5, n_segments=1, beta_x=54.6, beta_y=54.6,
                        D_x=0., D_y=0., accQ_x=20.13, accQ_y=20.18, Qp_x=5., Qp_y=5.,
                        app_x=1e-10 * p0_ref, app_y=1e-10 * p0_ref, app_xy=-5e-11 * p0_ref,
@@ -129,7 +129,7 @@ def main():
     from oracle import workloads as W
     from tests.machines import to_product
 
-    p0 = W.p0_of(W.GAMMA)
+    p0 = W.P0
     mach = build_machine(p0)
     wake = WakeField(UniformBinSlicer(args.slices, n_sigma_z=3), CircularResonator(1e6, 1e9, 1))
     one_turn = list(mach.one_turn_map) + [wake]

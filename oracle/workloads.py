@@ -11,7 +11,8 @@ machines/synchrotron.py:309-364) using the closed form of a stationary
 single-harmonic bucket above transition.
 
 Workload parameters:
-  protons, gamma = 27.7, C = 6911.5 m, beta_x = beta_y = 54.6 m, alpha = 0,
+  protons, p0 = sqrt(27.7^2 - 1) m_p c (gamma from the reference's p0
+  setter), C = 6911.5 m, beta_x = beta_y = 54.6 m, alpha = 0,
   D = 0, eps_n = 2.5e-6 m, Q_x = 20.13, Q_y = 20.18, alpha_c = 1.9e-3;
   linear:    LinearMap with Q_s = 0.017, sigma_z = 0.23 m, sigma_dp = 1e-3
              drawn as stated (no longitudinal matching), 50 slices over
@@ -28,7 +29,17 @@ from scipy.constants import c, e, m_p
 
 from . import refpath as R
 
-GAMMA = 27.7
+# The reference's machines take the momentum: Synchrotron(p0=...) sets gamma
+# through its p0 setter, 1/(c m) sqrt(p0^2 + m^2 c^2) (machines/synchrotron.py:
+# 278-280), which gives 27.700000000000003 for the nominal gamma 27.7.
+P0 = np.sqrt(27.7**2 - 1) * m_p * c
+
+
+def gamma_from_p0(p0, mass=m_p):
+    return 1 / (c * mass) * np.sqrt(p0**2 + mass**2 * c**2)
+
+
+GAMMA = gamma_from_p0(P0)
 CIRCUMFERENCE = 6911.5
 BETA_X = BETA_Y = 54.6
 EPSN = 2.5e-6
@@ -127,7 +138,7 @@ def linear_map_oracle(n_slices=50):
 
 
 def nonlinear_map_oracle(n_slices=500, n_turns_wake=1, p0=None):
-    p0 = p0_of(GAMMA) if p0 is None else p0
+    p0 = P0 if p0 is None else p0
     seg = R.smooth_segment(Q_X, Q_Y, BETA_X, BETA_Y, Qp_x=5., Qp_y=5.,
                            app_x=1e-10 * p0, app_y=1e-10 * p0, app_xy=-5e-11 * p0)
     rf = R.RFSystemsOracle(CIRCUMFERENCE, [H_RF], [V_RF], [0.], [ALPHA_C], GAMMA, e)

@@ -1,0 +1,280 @@
+"""Writes the golden vectors tests/golden/*.npz by running the reference
+PyHEADTAIL CPU path in this container (never on the GPU box, which has no
+copy of the reference).  The fixtures hold inputs and outputs only.
+
+Usage, from the repository root:
+
+    make -C oracle                       # builds the reference's Cython
+                                         # statistics module into oracle/_ref
+    python tests/golden/make_golden.py [REFERENCE_ROOT]
+
+REFERENCE_ROOT defaults to /root/reference (the directory holding the
+PyHEADTAIL package).  The script imports the reference package from there
+and its compiled statistics module (cobra_functions/stats.pyx) from
+oracle/_ref, with bytecode writing off (the reference tree is read-only).
+
+Fixtures (all float64 unless noted; seeds fixed; 'gamma' is the bunch's):
+  stats.npz      cov / std / emittance of cobra_functions.stats on seeded
+                 arrays (n = 1000, 2, 1), bunch-level and per-slice
+                 statistics of a SliceSet (UniformBinSlicer(10, n_sigma_z=2))
+  linear.npz     config-1 shape: ParticleGenerator bunch (N = 1000,
+                 sigma_z = 0.23, sigma_dp = 1e-3 as stated), Synchrotron
+                 one-turn map with LinearMap (Q_s = 0.017), 10 turns; per
+                 turn UniformBinSlicer(50, n_sigma_z=3) counts / mean_x /
+                 mean_y / (tail, head, width), coordinates after turns
+                 1, 2 and 10
+  nonlinear.npz  config-2 shape: Synchrotron.generate_6D_Gaussian_bunch
+                 (N = 1000, separatrix cut) of the non-linear RF machine
+                 with Q' = 5 and amplitude detuning, plus
+                 WakeField(UniformBinSlicer(50, n_sigma_z=3),
+                 CircularResonator(1e6, 1e9, 1)), 5 turns: bunch statistics
+                 and the wake slicing per turn, coordinates after turns 1, 5
+  elements.npz   one turn of the config-2 machine element by element (after
+                 RFSystems, after the transverse segment, after the
+                 WakeField), RFBucket Q_s, beta_z, sigma_dp and the
+                 separatrix acceptance (margin 0.05) on an 81 x 81 grid
+  multiturn.npz  config-3 shape at small size: linear longitudinal map,
+                 WakeField(UniformBinSlicer(20, n_sigma_z=3),
+                 CircularResonator(1e6, 1e9, 1, n_turns_wake=4),
+                 CircularResistiveWall(0.02, C, 5.9e7, 1e-12, n_turns_wake=4)),
+                 8 turns (dot-product path, moving bins): cuts per turn,
+                 coordinates after turns 1, 4, 8
+"""
+import glob
+import importlib.util
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+
+
+def load_reference(ref_root):
+    sys.dont_write_bytecode = True
+    so = glob.glob(os.path.join(ROOT, 'oracle', '_ref', 'stats*.so'))
+    if not so:
+        raise SystemExit("build the statistics module first: make -C oracle")
+    spec = importlib.util.spec_from_file_location('PyHEADTAIL.cobra_functions.stats', so[0])
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    sys.modules['PyHEADTAIL.cobra_functions.stats'] = mod
+    sys.path.insert(0, ref_root)
+    import PyHEADTAIL  # noqa: F401
+    return mod
+
+
+COORDS = ('x', 'xp', 'y', 'yp', 'z', 'dp')
+# turns (1-based) whose full coordinates are kept
+KEEP_LINEAR = (1, 2, 10)
+KEEP_NONLINEAR = (1, 5)
+KEEP_MULTITURN = (1, 4, 8)
+GAMMA, C, BETA, EPSN = 27.7, 6911.5, 54.6, 2.5e-6
+QX, QY, ALPHA_C = 20.13, 20.18, 1.9e-3
+
+
+def snapshot(bunch):
+    return np.stack([np.array(getattr(bunch, k), dtype=np.float64) for k in COORDS])
+
+
+def machine(longitudinal_mode, **kw):
+    from scipy.constants import c, e, m_p
+    from PyHEADTAIL.machines.synchrotron import Synchrotron
+    p0 = np.sqrt(GAMMA**2 - 1) * m_p * c
+    args = dict(optics_mode='smooth', charge=e, mass=m_p, p0=p0, circumference=C,
+                n_segments=1, alpha_x=0., beta_x=BETA, D_x=0., alpha_y=0., beta_y=BETA,
+                D_y=0., accQ_x=QX, accQ_y=QY, alpha_mom_compaction=ALPHA_C,
+                longitudinal_mode=longitudinal_mode)
+    args.update(kw)
+    return Synchrotron(**args), p0
+
+
+def gen_stats(cp):
+    from scipy.constants import c, e, m_p
+    from PyHEADTAIL.particles.generators import ParticleGenerator, gaussian2D
+    from PyHEADTAIL.particles.slicing import UniformBinSlicer
+    rng = np.random.RandomState(1)
+    out = {}
+    for n in (1000, 2, 1):
+        a, b, d = rng.normal(1., 2., n), rng.normal(-3., 0.5, n), rng.normal(0., 1e-3, n)
+        out['a%d' % n], out['b%d' % n], out['d%d' % n] = a, b, d
+        out['cov%d' % n] = np.array([cp.cov(a, b), cp.cov(a, a)])
+        out['std%d' % n] = np.array([cp.std(a)])
+        out['emit%d' % n] = np.array([cp.emittance(a, b, None), cp.emittance(a, b, d)])
+    np.random.seed(7)
+    beam = ParticleGenerator(macroparticlenumber=400, intensity=1e11, charge=e, mass=m_p,
+                             circumference=C, gamma=GAMMA,
+                             distribution_x=gaussian2D(1e-8), alpha_x=0., beta_x=BETA,
+                             D_x=0., distribution_y=gaussian2D(1e-8), alpha_y=0.,
+                             beta_y=BETA, D_y=0.,
+                             distribution_z=gaussian2D(1e-3)).generate()
+    out['beam'] = snapshot(beam)
+    out['gamma'] = np.array([beam.gamma])
+    out['beam_stats'] = np.array([beam.mean_x(), beam.mean_y(), beam.mean_z(),
+                                  beam.sigma_x(), beam.sigma_y(), beam.sigma_z(),
+                                  beam.sigma_dp(), beam.epsn_x(), beam.epsn_y(),
+                                  beam.epsn_z()])
+    ss = beam.get_slices(UniformBinSlicer(10, n_sigma_z=2),
+                         statistics=['mean_x', 'mean_y', 'sigma_x', 'sigma_dp',
+                                     'epsn_x', 'epsn_z', 'eff_epsn_x'])
+    out['ss_z_bins'] = np.array(ss.z_bins)
+    out['ss_idx'] = np.array(ss.slice_index_of_particle, dtype=np.int32)
+    out['ss_counts'] = np.array(ss.n_macroparticles_per_slice, dtype=np.int32)
+    for k in ('mean_x', 'mean_y', 'sigma_x', 'sigma_dp', 'epsn_x', 'epsn_z', 'eff_epsn_x'):
+        out['ss_' + k] = np.array(getattr(ss, k))
+    return out
+
+
+def slice_record(ss):
+    zb = np.array(ss.z_bins)
+    return (np.array(ss.n_macroparticles_per_slice, dtype=np.int32),
+            np.array([zb[0], zb[-1], zb[1] - zb[0]]))
+
+
+def gen_linear():
+    from scipy.constants import e, m_p
+    from PyHEADTAIL.particles.generators import (ParticleGenerator, gaussian2D,
+                                                 gaussian2D_asymmetrical)
+    from PyHEADTAIL.particles.slicing import UniformBinSlicer
+    mach, p0 = machine('linear', Q_s=0.017)
+    np.random.seed(12345)
+    eps = EPSN / np.sqrt(GAMMA**2 - 1)
+    beam = ParticleGenerator(macroparticlenumber=1000, intensity=1.5e11, charge=e, mass=m_p,
+                             circumference=C, gamma=GAMMA,
+                             distribution_x=gaussian2D(eps), alpha_x=0., beta_x=BETA, D_x=0.,
+                             distribution_y=gaussian2D(eps), alpha_y=0., beta_y=BETA, D_y=0.,
+                             distribution_z=gaussian2D_asymmetrical(sigma_u=0.23,
+                                                                    sigma_up=1e-3)).generate()
+    out = {'initial': snapshot(beam),
+           'order': np.array([type(el).__name__ for el in mach.one_turn_map]),
+           'gamma': np.array([beam.gamma])}
+    slicer = UniformBinSlicer(50, n_sigma_z=3)
+    coords, counts, means, cuts = [], [], [], []
+    for turn in range(1, 11):
+        for el in mach.one_turn_map:
+            el.track(beam)
+        ss = beam.get_slices(slicer, statistics=['mean_x', 'mean_y'])
+        cnt, cut = slice_record(ss)
+        if turn in KEEP_LINEAR:
+            coords.append(snapshot(beam))
+        counts.append(cnt)
+        cuts.append(cut)
+        means.append(np.stack([np.array(ss.mean_x), np.array(ss.mean_y)]))
+    out.update(coords=np.stack(coords), coord_turns=np.array(KEEP_LINEAR),
+               counts=np.stack(counts), means=np.stack(means), cuts=np.stack(cuts))
+    return out
+
+
+def gen_nonlinear():
+    from PyHEADTAIL.impedances.wakes import WakeField, CircularResonator
+    from PyHEADTAIL.particles.slicing import UniformBinSlicer
+    from scipy.constants import c, m_p
+    p0 = np.sqrt(GAMMA**2 - 1) * m_p * c
+    mach, _ = machine('non-linear', Qp_x=5., Qp_y=5., app_x=1e-10 * p0, app_y=1e-10 * p0,
+                      app_xy=-5e-11 * p0, h_RF=4620, V_RF=4.5e6, dphi_RF=0., p_increment=0.)
+    np.random.seed(12345)
+    beam = mach.generate_6D_Gaussian_bunch(1000, 1.5e11, EPSN, EPSN, 0.23)
+    wake = WakeField(UniformBinSlicer(50, n_sigma_z=3), CircularResonator(1e6, 1e9, 1))
+    one_turn = list(mach.one_turn_map) + [wake]
+    out = {'initial': snapshot(beam),
+           'order': np.array([type(el).__name__ for el in one_turn]),
+           'gamma': np.array([beam.gamma])}
+    coords, stats, counts, cuts = [], [], [], []
+    for turn in range(1, 6):
+        for el in one_turn:
+            el.track(beam)
+        cnt, cut = slice_record(wake.slice_set_deque[0])
+        if turn in KEEP_NONLINEAR:
+            coords.append(snapshot(beam))
+        counts.append(cnt)
+        cuts.append(cut)
+        stats.append([beam.mean_x(), beam.mean_y(), beam.mean_z(), beam.sigma_z(),
+                      beam.epsn_x(), beam.epsn_y(), beam.epsn_z()])
+    out.update(coords=np.stack(coords), coord_turns=np.array(KEEP_NONLINEAR),
+               stats=np.array(stats), counts=np.stack(counts), cuts=np.stack(cuts))
+    return out
+
+
+def gen_elements():
+    """Element by element through one turn of the config-2 machine, and the
+    quantities that fix its bunch: RFBucket Q_s, sigma_dp, the separatrix
+    acceptance on a grid."""
+    from scipy.constants import c, m_p
+    from PyHEADTAIL.impedances.wakes import WakeField, CircularResonator
+    from PyHEADTAIL.particles.slicing import UniformBinSlicer
+    p0 = np.sqrt(GAMMA**2 - 1) * m_p * c
+    mach, _ = machine('non-linear', Qp_x=5., Qp_y=5., app_x=1e-10 * p0, app_y=1e-10 * p0,
+                      app_xy=-5e-11 * p0, h_RF=4620, V_RF=4.5e6, dphi_RF=0., p_increment=0.)
+    bucket = mach.longitudinal_map.get_bucket(gamma=mach.gamma, mass=mach.mass,
+                                              charge=mach.charge)
+    eta = mach.longitudinal_map.alpha_array[0] - mach.gamma ** -2
+    beta_z = np.abs(eta) * mach.circumference / 2.0 / np.pi / bucket.Q_s
+    zz, dd = np.meshgrid(np.linspace(-0.8, 0.8, 81), np.linspace(-4e-3, 4e-3, 81))
+    acc = bucket.make_is_accepted(margin=0.05)(zz.ravel(), dd.ravel())
+    out = {'Q_s': np.array([bucket.Q_s]), 'beta_z': np.array([beta_z]),
+           'sigma_dp': np.array([0.23 / beta_z]), 'grid_z': zz.ravel(),
+           'grid_dp': dd.ravel(), 'accepted': np.array(acc, dtype=np.int8)}
+    np.random.seed(12345)
+    beam = mach.generate_6D_Gaussian_bunch(1000, 1.5e11, EPSN, EPSN, 0.23)
+    wake = WakeField(UniformBinSlicer(50, n_sigma_z=3), CircularResonator(1e6, 1e9, 1))
+    out['initial'] = snapshot(beam)
+    out['gamma'] = np.array([beam.gamma])
+    for name, el in (('after_rf', mach.one_turn_map[0]), ('after_transverse',
+                                                          mach.one_turn_map[1]),
+                     ('after_wake', wake)):
+        el.track(beam)
+        out[name] = snapshot(beam)
+    out['p0_after'] = np.array([beam.p0, beam.gamma])
+    return out
+
+
+def gen_multiturn():
+    from PyHEADTAIL.impedances.wakes import (WakeField, CircularResonator,
+                                             CircularResistiveWall)
+    from PyHEADTAIL.particles.generators import (ParticleGenerator, gaussian2D,
+                                                 gaussian2D_asymmetrical)
+    from scipy.constants import e, m_p
+    from PyHEADTAIL.particles.slicing import UniformBinSlicer
+    mach, p0 = machine('linear', Q_s=0.017)
+    np.random.seed(2024)
+    eps = EPSN / np.sqrt(GAMMA**2 - 1)
+    beam = ParticleGenerator(macroparticlenumber=1000, intensity=1.5e11, charge=e, mass=m_p,
+                             circumference=C, gamma=GAMMA,
+                             distribution_x=gaussian2D(eps), alpha_x=0., beta_x=BETA, D_x=0.,
+                             distribution_y=gaussian2D(eps), alpha_y=0., beta_y=BETA, D_y=0.,
+                             distribution_z=gaussian2D_asymmetrical(sigma_u=0.23,
+                                                                    sigma_up=1e-3)).generate()
+    wake = WakeField(UniformBinSlicer(20, n_sigma_z=3),
+                     CircularResonator(1e6, 1e9, 1, n_turns_wake=4),
+                     CircularResistiveWall(0.02, C, 5.9e7, 1e-12, n_turns_wake=4))
+    one_turn = list(mach.one_turn_map) + [wake]
+    out = {'initial': snapshot(beam),
+           'order': np.array([type(el).__name__ for el in one_turn]),
+           'gamma': np.array([beam.gamma])}
+    coords, cuts = [], []
+    for turn in range(1, 9):
+        for el in one_turn:
+            el.track(beam)
+        if turn in KEEP_MULTITURN:
+            coords.append(snapshot(beam))
+        cuts.append(slice_record(wake.slice_set_deque[0])[1])
+    out.update(coords=np.stack(coords), coord_turns=np.array(KEEP_MULTITURN),
+               cuts=np.stack(cuts))
+    return out
+
+
+def main():
+    ref = sys.argv[1] if len(sys.argv) > 1 else '/root/reference'
+    cp = load_reference(ref)
+    for name, fn in (('stats', lambda: gen_stats(cp)), ('linear', gen_linear),
+                     ('nonlinear', gen_nonlinear), ('multiturn', gen_multiturn),
+                     ('elements', gen_elements)):
+        out = fn()
+        path = os.path.join(HERE, name + '.npz')
+        np.savez_compressed(path, **out)
+        print("wrote %s (%d bytes)" % (path, os.path.getsize(path)))
+
+
+if __name__ == '__main__':
+    main()

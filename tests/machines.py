@@ -44,7 +44,7 @@ def nonlinear_machine(n_slices=500, n_turns_wake=1, p0=None, wake=True):
     from pyheadtail_amd.trackers.longitudinal_tracking import RFSystems
     from pyheadtail_amd.particles.slicing import UniformBinSlicer
     from pyheadtail_amd.impedances.wakes import WakeField, CircularResonator
-    p0 = W.p0_of(W.GAMMA) if p0 is None else p0
+    p0 = W.P0 if p0 is None else p0
     rf = RFSystems(W.CIRCUMFERENCE, np.atleast_1d(W.H_RF), np.atleast_1d(W.V_RF),
                    np.atleast_1d(0.), np.atleast_1d(W.ALPHA_C), W.GAMMA, p_increment=0.,
                    charge=e, mass=m_p)

@@ -1,0 +1,164 @@
+"""The oracle against the golden vectors written by the reference itself
+(tests/golden/make_golden.py runs PyHEADTAIL's CPU path in the build
+container).  These pin the oracle: the statistics functions, the seeded bunch
+generators (draw order, matching, separatrix cut), the slicing, and whole
+turns of the three workload shapes, including the multi-turn dot-product wake
+with moving bins."""
+import os
+
+import numpy as np
+import pytest
+from scipy.constants import c, e, m_p
+
+from oracle import refpath as R
+from oracle import workloads as W
+
+GOLD = os.path.join(os.path.dirname(__file__), 'golden')
+COORDS = ('x', 'xp', 'y', 'yp', 'z', 'dp')
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name + '.npz'))
+
+
+def bunch_from(arr, gamma, n_particles=1.5e11):
+    n = arr.shape[1]
+    return R.Bunch(n, n_particles / n, e, m_p, W.CIRCUMFERENCE, gamma,
+                   {k: arr[i] for i, k in enumerate(COORDS)})
+
+
+def snapshot(b):
+    return np.stack([getattr(b, k) for k in COORDS])
+
+
+def rel_err(a, ref):
+    scale = np.sqrt(np.mean(ref * ref, axis=-1, keepdims=True))
+    scale[scale == 0] = 1.
+    return float(np.max(np.abs(a - ref) / scale))
+
+
+def test_statistics_functions_bit_exact():
+    g = gold('stats')
+    for n in (1000, 2, 1):
+        a, b, d = g['a%d' % n], g['b%d' % n], g['d%d' % n]
+        # NaN (a negative determinant from rounding) must be NaN on both sides
+        np.testing.assert_array_equal([R.cov(a, b), R.cov(a, a)], g['cov%d' % n])
+        np.testing.assert_array_equal([R.std(a)], g['std%d' % n])
+        np.testing.assert_array_equal([R.emittance(a, b, None), R.emittance(a, b, d)],
+                                      g['emit%d' % n])
+
+
+def test_bunch_and_slice_statistics():
+    g = gold('stats')
+    b = bunch_from(g['beam'], g['gamma'][0], 1e11)
+    st = R.bunch_statistics(b)
+    keys = ('mean_x', 'mean_y', 'mean_z', 'sigma_x', 'sigma_y', 'sigma_z', 'sigma_dp',
+            'epsn_x', 'epsn_y', 'epsn_z')
+    for k, ref in zip(keys, g['beam_stats']):
+        assert st[k] == ref, k
+    sd = R.slice_uniform(b, 10, n_sigma_z=2)
+    np.testing.assert_array_equal(sd.z_bins, g['ss_z_bins'])
+    np.testing.assert_array_equal(sd.slice_index_of_particle, g['ss_idx'])
+    np.testing.assert_array_equal(sd.n_macroparticles_per_slice, g['ss_counts'])
+    np.testing.assert_allclose(R.mean_per_slice(sd, b.x), g['ss_mean_x'], rtol=1e-13,
+                               atol=1e-13 * np.std(b.x))
+    np.testing.assert_allclose(R.mean_per_slice(sd, b.y), g['ss_mean_y'], rtol=1e-13,
+                               atol=1e-13 * np.std(b.y))
+    np.testing.assert_allclose(R.std_per_slice(sd, b.x), g['ss_sigma_x'], rtol=1e-12)
+    np.testing.assert_allclose(R.std_per_slice(sd, b.dp), g['ss_sigma_dp'], rtol=1e-12)
+    bg = b.betagamma
+    np.testing.assert_allclose(R.emittance_per_slice(sd, b.x, b.xp, b.dp) * bg,
+                               g['ss_epsn_x'], rtol=1e-9)
+    np.testing.assert_allclose(R.emittance_per_slice(sd, b.x, b.xp) * bg,
+                               g['ss_eff_epsn_x'], rtol=1e-9)
+    epsz = R.emittance_per_slice(sd, b.z, b.dp) * b.p0 / e * 4 * np.pi
+    np.testing.assert_allclose(epsz, g['ss_epsn_z'], rtol=1e-9)
+
+
+def test_generators_reproduce_reference_draws():
+    """Draw order, matching and (non-linear RF) the separatrix cut."""
+    np.testing.assert_array_equal(snapshot(W.linear_bunch(1000, seed=12345)),
+                                  gold('linear')['initial'])
+    np.testing.assert_array_equal(snapshot(W.nonlinear_bunch(1000, seed=12345)),
+                                  gold('nonlinear')['initial'])
+
+
+def test_linear_workload_turns():
+    g = gold('linear')
+    assert list(g['order']) == ['LinearMap', 'TransverseSegmentMap']
+    b = bunch_from(g['initial'], g['gamma'][0])
+    m = W.linear_map_oracle()
+    keep = list(g['coord_turns'])
+    for turn in range(1, 11):
+        for el in m:
+            el.track(b)
+        sd = R.slice_uniform(b, 50, n_sigma_z=3)
+        np.testing.assert_array_equal(sd.n_macroparticles_per_slice, g['counts'][turn - 1])
+        np.testing.assert_array_equal([sd.z_bins[0], sd.z_bins[-1],
+                                       sd.z_bins[1] - sd.z_bins[0]], g['cuts'][turn - 1])
+        np.testing.assert_array_equal(R.mean_per_slice(sd, b.x), g['means'][turn - 1][0])
+        np.testing.assert_array_equal(R.mean_per_slice(sd, b.y), g['means'][turn - 1][1])
+        if turn in keep:
+            np.testing.assert_array_equal(snapshot(b), g['coords'][keep.index(turn)])
+
+
+def test_nonlinear_workload_turns():
+    g = gold('nonlinear')
+    assert list(g['order']) == ['RFSystems', 'TransverseSegmentMap', 'WakeField']
+    b = bunch_from(g['initial'], g['gamma'][0])
+    m = W.nonlinear_map_oracle(n_slices=50)
+    keep = list(g['coord_turns'])
+    for turn in range(1, 6):
+        for el in m:
+            el.track(b)
+        sd = m[2].slices[0]
+        np.testing.assert_array_equal(sd.n_macroparticles_per_slice, g['counts'][turn - 1])
+        np.testing.assert_array_equal([sd.z_bins[0], sd.z_bins[-1],
+                                       sd.z_bins[1] - sd.z_bins[0]], g['cuts'][turn - 1])
+        st = R.bunch_statistics(b)
+        got = [st[k] for k in ('mean_x', 'mean_y', 'mean_z', 'sigma_z', 'epsn_x', 'epsn_y',
+                               'epsn_z')]
+        np.testing.assert_array_equal(got, g['stats'][turn - 1])
+        if turn in keep:
+            np.testing.assert_array_equal(snapshot(b), g['coords'][keep.index(turn)])
+
+
+def multiturn_oracle(T=4, n_slices=20):
+    lin, seg = W.linear_map_oracle()
+    wr = R.resonator_transverse(1., 1e6, 1e9, 1.)
+    ww = R.resistive_wall_transverse(1., 0.02, W.CIRCUMFERENCE, 5.9e7, 1e-12)
+    kicks = [R.Kick('DipoleX', wr, T), R.Kick('DipoleY', wr, T),
+             R.Kick('DipoleX', ww, T), R.Kick('DipoleY', ww, T)]
+    return [lin, seg, R.WakeFieldOracle(n_slices, kicks, n_sigma_z=3)]
+
+
+def test_multiturn_dot_product_wake():
+    g = gold('multiturn')
+    assert list(g['order']) == ['LinearMap', 'TransverseSegmentMap', 'WakeField']
+    b = bunch_from(g['initial'], g['gamma'][0])
+    m = multiturn_oracle()
+    keep = list(g['coord_turns'])
+    for turn in range(1, 9):
+        for el in m:
+            el.track(b)
+        sd = m[2].slices[0]
+        np.testing.assert_array_equal([sd.z_bins[0], sd.z_bins[-1],
+                                       sd.z_bins[1] - sd.z_bins[0]], g['cuts'][turn - 1])
+        if turn in keep:
+            np.testing.assert_array_equal(snapshot(b), g['coords'][keep.index(turn)])
+
+
+def test_config2_elements_and_bucket():
+    """One turn of the config-2 machine element by element, the bucket's
+    Q_s and the separatrix acceptance (margin 0.05) on a grid."""
+    g = gold('elements')
+    assert W.q_s_single_harmonic(W.GAMMA, W.ALPHA_C, W.H_RF, W.V_RF) == g['Q_s'][0]
+    assert W.nonlinear_sigma_dp() == g['sigma_dp'][0]
+    acc = W.stationary_acceptance(W.GAMMA, W.ALPHA_C, W.H_RF, W.V_RF, W.CIRCUMFERENCE, 0.05)
+    np.testing.assert_array_equal(acc(g['grid_z'], g['grid_dp']), g['accepted'].astype(bool))
+    b = bunch_from(g['initial'], g['gamma'][0])
+    m = W.nonlinear_map_oracle(n_slices=50)
+    for name, el in zip(('after_rf', 'after_transverse', 'after_wake'), m):
+        el.track(b)
+        np.testing.assert_array_equal(snapshot(b), g[name], err_msg=name)
+    assert (b.p0, b.gamma) == tuple(g['p0_after'])

@@ -49,7 +49,7 @@ def test_transverse_with_detuning():
     _print_case("TransverseSegmentMap chroma+amplitude")
     ob = W.nonlinear_bunch(20000, seed=3)
     pb = to_product(ob)
-    p0 = W.p0_of(W.GAMMA)
+    p0 = W.P0
     smooth_transverse(5., 1e-10 * p0, -5e-11 * p0)[0].track(pb)
     R.smooth_segment(W.Q_X, W.Q_Y, W.BETA_X, W.BETA_Y, Qp_x=5., Qp_y=5., app_x=1e-10 * p0,
                      app_y=1e-10 * p0, app_xy=-5e-11 * p0).track(ob)
