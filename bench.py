@@ -92,6 +92,24 @@ def cpu_baseline(n, turns):
                        % (n, turns, dt))
 
 
+def pass_traffic(n):
+    """HBM bytes per launch of the particle pass from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (profiles/pass_traffic.json), each scaled by
+    what the counter reads for a plain copy of known size; None if the
+    figures are for another size."""
+    path = os.path.join(ROOT, "profiles", "pass_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("macroparticles") != n:
+        return None
+    read = t["fetch_kib"] * 1024 * t["copy_bytes"] / (t["copy_fetch_kib"] * 1024)
+    written = t["write_kib"] * 1024 * t["copy_bytes"] / (t["copy_write_kib"] * 1024)
+    return read + written
+
+
 def build_machine(p0_ref):
     import numpy as np
     from scipy.constants import e, m_p
@@ -175,7 +193,7 @@ def main():
                    "macroparticles": args.n, "n_slices": args.slices, "turns": args.steps,
                    "graph_turns": args.graph_turns},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": pass_traffic(args.n),
                      "kernel": "k_engine_pass (fused particle pass)",
                      "algorithmic_bytes_per_launch": BYTES_PER_MP_PASS * args.n},
         "stage_ms": {"pass": stage_ms[0], "hist": stage_ms[1], "conv": stage_ms[2]},

@@ -42,7 +42,7 @@ enum {
 #define PHX_MAX_DETUNERS 4       /* detuners per transverse segment */
 #define PHX_MAX_CHROMA 8         /* chromaticity polynomial order */
 #define PHX_MAX_LONG_OPS 10      /* drifts + kicks in one longitudinal sequence */
-#define PHX_MAX_KICKS 8          /* wake kicks in one wake field */
+#define PHX_MAX_KICKS 16         /* wake kicks in one call / turn program */
 #define PHX_MAX_TERMS 8          /* per-slice accumulated terms in one pass */
 #define PHX_MAX_PROGRAM 16       /* local elements fused into one particle pass */
 #define PHX_MAX_STATS 16         /* per-turn bunch statistics recorded by a pass */

@@ -20,7 +20,7 @@ LIB_PATH = os.environ.get("PHX_LIB", os.path.join(_HERE, "libphx.so"))
 MAX_DETUNERS = 4
 MAX_CHROMA = 8
 MAX_LONG_OPS = 10
-MAX_KICKS = 8
+MAX_KICKS = 16
 MAX_TERMS = 8
 MAX_PROGRAM = 16
 

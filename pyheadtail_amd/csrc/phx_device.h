@@ -270,20 +270,24 @@ __device__ __forceinline__ void apply_wake_kicks(P6 &p, int32_t idx, int n_slice
                                                  const double *__restrict__ kicks, int ks,
                                                  int ss) {
   if (idx < 0 || idx >= n_slices) return;
-  // gather all kick values first so their loads are in flight together
-  double kv[PHX_MAX_KICKS];
+  // kick values gathered four at a time (loads in flight together, few
+  // registers held), then applied in list order
+  for (int k0 = 0; k0 < n_kicks; k0 += 4) {
+    double kv[4];
 #pragma unroll
-  for (int k = 0; k < PHX_MAX_KICKS; ++k)
-    if (k < n_kicks) kv[k] = kicks[(size_t)k * ks + (size_t)idx * ss];
+    for (int q = 0; q < 4; ++q)
+      if (k0 + q < n_kicks) kv[q] = kicks[(size_t)(k0 + q) * ks + (size_t)idx * ss];
 #pragma unroll
-  for (int k = 0; k < PHX_MAX_KICKS; ++k) {
-    if (k >= n_kicks) break;
-    double v = kv[k];
-    if (kd[k].multiply == 0) v = v * p.x;
-    else if (kd[k].multiply == 2) v = v * p.y;
-    if (kd[k].plane == PHX_PLANE_XP) p.xp = p.xp + v;
-    else if (kd[k].plane == PHX_PLANE_YP) p.yp = p.yp + v;
-    else p.dp = p.dp + v;
+    for (int q = 0; q < 4; ++q) {
+      if (k0 + q >= n_kicks) break;
+      const phx_wake_kick &d = kd[k0 + q];
+      double v = kv[q];
+      if (d.multiply == 0) v = v * p.x;
+      else if (d.multiply == 2) v = v * p.y;
+      if (d.plane == PHX_PLANE_XP) p.xp = p.xp + v;
+      else if (d.plane == PHX_PLANE_YP) p.yp = p.yp + v;
+      else p.dp = p.dp + v;
+    }
   }
 }
 

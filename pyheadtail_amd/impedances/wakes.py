@@ -250,8 +250,6 @@ class WakeField(Element):
         self.wake_kicks = []
         for source in wake_sources:
             self.wake_kicks.extend(source.get_wake_kicks(self.slicer))
-        if len(self.wake_kicks) > _phx.MAX_KICKS:
-            raise ValueError("at most %d wake kicks per WakeField" % _phx.MAX_KICKS)
         n_turns_wake_max = max(source.n_turns_wake for source in wake_sources)
         self.n_turns_wake_max = n_turns_wake_max
         self.slice_set_deque = deque([], maxlen=n_turns_wake_max)
@@ -271,5 +269,9 @@ class WakeField(Element):
             self._history = WakeHistory(slice_set.n_slices, self.n_turns_wake_max)
         self._history.push_sliceset(slice_set, age_increment)
         records = [k.record(bunch) for k in self.wake_kicks]
-        kicks = self._history.convolve(records)
-        apply_kicks(bunch, slice_set, records, kicks)
+        # every kick convolves the moments of this slicing, so groups of at
+        # most MAX_KICKS applied in list order give the same result as one
+        for i in range(0, len(records), _phx.MAX_KICKS):
+            group = records[i:i + _phx.MAX_KICKS]
+            kicks = self._history.convolve(group)
+            apply_kicks(bunch, slice_set, group, kicks)

@@ -33,6 +33,13 @@ Fixtures (all float64 unless noted; seeds fixed; 'gamma' is the bunch's):
                  RFSystems, after the transverse segment, after the
                  WakeField), RFBucket Q_s, beta_z, sigma_dp and the
                  separatrix acceptance (margin 0.05) on an 81 x 81 grid
+  extra_wakes.npz  the kick types and regimes the workloads leave out
+                 (quadrupolar and constant-z kicks; resonators with Q = 0.5
+                 and switch_Z, and Q = 0.3 with one turn of memory;
+                 parallel-plates resistive wall) on a 500-particle linear
+                 machine, one WakeField over fixed cuts (convolution) and one
+                 with n_sigma_z and two turns of memory (dot product);
+                 coordinates after each of 3 turns
   multiturn.npz  config-3 shape at small size: linear longitudinal map,
                  WakeField(UniformBinSlicer(20, n_sigma_z=3),
                  CircularResonator(1e6, 1e9, 1, n_turns_wake=4),
@@ -264,12 +271,59 @@ def gen_multiturn():
     return out
 
 
+def extra_wakes(sources):
+    """Per kick type and resonator regime: linear machine, 30 slices over
+    fixed cuts (np.convolve path) then n_sigma_z (dot product with T = 2)."""
+    from PyHEADTAIL.impedances.wakes import WakeField
+    from PyHEADTAIL.particles.generators import (ParticleGenerator, gaussian2D,
+                                                 gaussian2D_asymmetrical)
+    from PyHEADTAIL.particles.slicing import UniformBinSlicer
+    from scipy.constants import e, m_p
+    mach, _ = machine('linear', Q_s=0.017)
+    np.random.seed(77)
+    eps = EPSN / np.sqrt(GAMMA**2 - 1)
+    beam = ParticleGenerator(macroparticlenumber=500, intensity=1.5e11, charge=e, mass=m_p,
+                             circumference=C, gamma=GAMMA,
+                             distribution_x=gaussian2D(eps), alpha_x=0., beta_x=BETA, D_x=0.,
+                             distribution_y=gaussian2D(eps), alpha_y=0., beta_y=BETA, D_y=0.,
+                             distribution_z=gaussian2D_asymmetrical(sigma_u=0.23,
+                                                                    sigma_up=1e-3)).generate()
+    out = {'initial': snapshot(beam), 'gamma': np.array([beam.gamma])}
+    wakes = [WakeField(UniformBinSlicer(30, z_cuts=(-0.6, 0.6)), *sources(1)),
+             WakeField(UniformBinSlicer(30, n_sigma_z=3), *sources(2))]
+    coords = []
+    for turn in range(3):
+        for el in list(mach.one_turn_map) + wakes:
+            el.track(beam)
+        coords.append(snapshot(beam))
+    out['coords'] = np.stack(coords)
+    out['kicks'] = np.array([type(k).__name__ for w in wakes for k in w.wake_kicks])
+    return out
+
+
+def gen_extra_wakes():
+    """The kick types and wake regimes the workloads leave out: quadrupolar
+    and constant-z kicks, resonator Q < 1/2, Q = 1/2, longitudinal resonator,
+    parallel-plates resistive wall."""
+    from PyHEADTAIL.impedances.wakes import (Resonator, ParallelHorizontalPlatesResistiveWall)
+
+    # Q < 1/2 only with one turn of memory: for past turns the reference's
+    # exp(alpha dt) * sinh(omegabar dt) is 0 * inf = NaN
+    def sources(T):
+        res = [Resonator(1e6, 1.2e9, 0.5, 0.7, 1., 0.3, -0.3, 1, n_turns_wake=T),
+               ParallelHorizontalPlatesResistiveWall(0.02, C, 5.9e7, 1e-12, n_turns_wake=T)]
+        if T == 1:
+            res.insert(0, Resonator(2e6, 7e8, 0.3, 1., 0.8, -0.4, 0.4, 1, n_turns_wake=T))
+        return res
+    return extra_wakes(sources)
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else '/root/reference'
     cp = load_reference(ref)
     for name, fn in (('stats', lambda: gen_stats(cp)), ('linear', gen_linear),
                      ('nonlinear', gen_nonlinear), ('multiturn', gen_multiturn),
-                     ('elements', gen_elements)):
+                     ('elements', gen_elements), ('extra_wakes', gen_extra_wakes)):
         out = fn()
         path = os.path.join(HERE, name + '.npz')
         np.savez_compressed(path, **out)

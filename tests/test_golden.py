@@ -162,3 +162,43 @@ def test_config2_elements_and_bucket():
         el.track(b)
         np.testing.assert_array_equal(snapshot(b), g[name], err_msg=name)
     assert (b.p0, b.gamma) == tuple(g['p0_after'])
+
+
+def extra_wakes_oracle():
+    """The two WakeFields of the extra_wakes fixture (see make_golden.py)."""
+    pp = (np.pi**2 / 24., np.pi**2 / 12., -np.pi**2 / 24., np.pi**2 / 24.)
+
+    def resonator(T, z_cuts, Rs, f, Q, yx1, yy1, yx2, yy2):
+        return [R.Kick('DipoleX', R.resonator_transverse(yx1, Rs, f, Q), T, z_cuts=z_cuts),
+                R.Kick('QuadX', R.resonator_transverse(yx2, Rs, f, Q), T, z_cuts=z_cuts),
+                R.Kick('DipoleY', R.resonator_transverse(yy1, Rs, f, Q), T, z_cuts=z_cuts),
+                R.Kick('QuadY', R.resonator_transverse(yy2, Rs, f, Q), T, z_cuts=z_cuts),
+                R.Kick('ConstantZ', R.resonator_longitudinal(Rs, f, Q), T, z_cuts=z_cuts)]
+
+    def sources(T, z_cuts):
+        kicks = []
+        if T == 1:
+            kicks += resonator(T, z_cuts, 2e6, 7e8, 0.3, 1., 0.8, -0.4, 0.4)
+        kicks += resonator(T, z_cuts, 1e6, 1.2e9, 0.5, 0.7, 1., 0.3, -0.3)
+        for kind, y in (('DipoleX', pp[0]), ('QuadX', pp[2]), ('DipoleY', pp[1]),
+                        ('QuadY', pp[3])):
+            wake = R.resistive_wall_transverse(y, 0.02, W.CIRCUMFERENCE, 5.9e7, 1e-12)
+            kicks.append(R.Kick(kind, wake, T, z_cuts=z_cuts))
+        return kicks
+    return [R.WakeFieldOracle(30, sources(1, (-0.6, 0.6)), z_cuts=(-0.6, 0.6)),
+            R.WakeFieldOracle(30, sources(2, None), n_sigma_z=3)]
+
+
+def test_extra_wake_kinds():
+    """Quadrupolar and constant-z kicks, resonators with Q = 0.3 and 0.5 and
+    switch_Z, parallel-plates resistive wall, fixed cuts and n_sigma_z."""
+    g = gold('extra_wakes')
+    b = bunch_from(g['initial'], g['gamma'][0])
+    lin, seg = W.linear_map_oracle()
+    wakes = extra_wakes_oracle()
+    assert [k.kind for w in wakes for k in w.kicks] == [
+        n.replace('WakeKick', '').replace('Quadrupole', 'Quad') for n in g['kicks']]
+    for turn in range(3):
+        for el in [lin, seg] + wakes:
+            el.track(b)
+        np.testing.assert_array_equal(snapshot(b), g['coords'][turn])

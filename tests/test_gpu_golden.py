@@ -180,3 +180,37 @@ def test_config2_bunch_generation_vs_reference():
     pb = mach.generate_6D_Gaussian_bunch(1000, 1.5e11, W.EPSN, W.EPSN, W.SIGMA_Z)
     np.testing.assert_array_equal(snap(pb), g['initial'])
     assert pb.gamma == g['gamma'][0]
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_extra_wake_kinds_vs_reference(fused):
+    _case("quadrupolar / constant-z kicks, Q = 0.3 and 0.5, parallel-plates RW, fixed cuts "
+          "and n_sigma_z, fused=%s" % fused)
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.impedances.wakes import (WakeField, Resonator,
+                                                 ParallelHorizontalPlatesResistiveWall)
+    g = gold('extra_wakes')
+
+    def sources(T):
+        res = [Resonator(1e6, 1.2e9, 0.5, 0.7, 1., 0.3, -0.3, 1, n_turns_wake=T),
+               ParallelHorizontalPlatesResistiveWall(0.02, W.CIRCUMFERENCE, 5.9e7, 1e-12,
+                                                     n_turns_wake=T)]
+        if T == 1:
+            res.insert(0, Resonator(2e6, 7e8, 0.3, 1., 0.8, -0.4, 0.4, 1, n_turns_wake=T))
+        return res
+    mach = machine('linear', Q_s=0.017)
+    wakes = [WakeField(UniformBinSlicer(30, z_cuts=(-0.6, 0.6)), *sources(1)),
+             WakeField(UniformBinSlicer(30, n_sigma_z=3), *sources(2))]
+    assert [type(k).__name__ for w in wakes for k in w.wake_kicks] == list(g['kicks'])
+    pb = product_bunch(g['initial'], g['gamma'][0])
+    if fused:
+        # the engine takes one WakeField at the end of the map: the first one
+        # runs as an element, the rest of the turn fused
+        from pyheadtail_amd.engine import FusedOneTurnMap
+        eng = FusedOneTurnMap([wakes[1]], graph_turns=0)
+    for turn in range(3):
+        for el in list(mach.one_turn_map) + wakes[:1 if fused else 2]:
+            el.track(pb)
+        if fused:
+            eng.track_turns(pb, 1)
+        assert rel(snap(pb), g['coords'][turn]) <= 1e-10, turn
