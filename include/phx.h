@@ -409,16 +409,18 @@ int64_t phx_engine_ws_bytes(int64_t n, int32_t n_slices, int32_t n_kicks,
 /* Device-side reset of the engine state (turn counter, pending flag,
    workspace accumulators, wake history).  No memset nodes. */
 int phx_engine_reset(const phx_engine *e, phx_stream_t stream);
-/* Enqueue n_turns fused turns.  Each turn is three launches: the particle
+/* Enqueue n_turns fused turns.  Each turn is four launches: the particle
    pass (pending wake kick, local elements, z moments), the slice histogram
-   and the wake convolution.  The wake kick of the last turn stays pending
-   until phx_engine_flush. */
+   (cuts, per-block histograms), the histogram merge (slice means, history
+   entry) and the wake convolution; without a wake field, the pass and a
+   one-block finalize.  The wake kick of the last turn stays pending until
+   phx_engine_flush. */
 int phx_engine_turns(const phx_engine *e, int32_t n_turns, phx_stream_t stream);
 /* Applies a pending wake kick (and records the final statistics row). */
 int phx_engine_flush(const phx_engine *e, phx_stream_t stream);
-/* Enqueue one stage of a turn: 0 particle pass, 1 slice histogram, 2 wake
-   convolution (stages 1-2 need a wake field).  Running 0, 1, 2 in order is
-   one turn; used to time the stages separately. */
+/* Enqueue one stage of a turn: 0 particle pass, 1 slice histogram and
+   merge, 2 wake convolution (stages 1-2 need a wake field).  Running 0, 1, 2
+   in order is one turn; used to time the stages separately. */
 int phx_engine_stage(const phx_engine *e, int32_t stage, phx_stream_t stream);
 
 /* HIP graph of n_turns fused turns (captured on an internal stream);

@@ -393,22 +393,50 @@ def rf_kick(b, circumference, harmonic, voltage, phi_offset=0, phi_lock=0, p_inc
 
 
 class RFSystemsOracle(object):
-    """RFSystems: drift(C/2), kicks, drift(C/2); phase lock of the
-    non-accelerating kicks (:332-412, 556-593); no p_increment here."""
+    """RFSystems (:280-665): drift(C/2), kicks, drift(C/2) (:362-369); the
+    accelerating kick is the first; phase lock of the other kicks by
+    -h_k/h_0 * phi_s with phi_s = arcsin(p_increment beta c / (|q| V_0))
+    (:556-575, 530-542); with p_increment the transverse coordinates shrink
+    by sqrt(betagamma_before / betagamma_after) (:516-530, 545-548); the
+    last drift gets shrinkage_p_increment = p_increment only with
+    shrink_longitudinal (:451-456)."""
 
     def __init__(self, circumference, harmonic_list, voltage_list, phi_offset_list,
-                 alpha_array, gamma_reference, charge):
+                 alpha_array, gamma_reference, charge, p_increment=0, D_x=0, D_y=0,
+                 phase_lock=True, shrink_transverse=True, shrink_longitudinal=False):
         self.C = circumference
         self.h, self.V, self.dphi = list(harmonic_list), list(voltage_list), list(phi_offset_list)
         self.alpha_array = alpha_array
-        # phi_s = arcsin(0) = 0 without acceleration: phi_lock = 0 - h/h0*0
-        self.phi_lock = [0] + [0 - hk / self.h[0] * 0 for hk in self.h[1:]]
+        self.p_increment = p_increment
+        self.D = (D_x, D_y)
+        self.shrink_transverse = shrink_transverse
+        self.shrink_longitudinal = shrink_longitudinal
+        self.phi_lock = [0] * len(self.h)
+        if phase_lock:
+            phi_s = self.phi_s(gamma_reference, charge)
+            for k in range(1, len(self.h)):
+                self.phi_lock[k] -= self.h[k] / self.h[0] * phi_s
+
+    def phi_s(self, gamma, charge):
+        beta = np.sqrt(1 - gamma**-2)
+        if self.p_increment == 0 and self.V[0] == 0:
+            return 0
+        deltaE = self.p_increment * beta * c
+        return np.arcsin(deltaE / (np.abs(charge) * self.V[0]))
 
     def track(self, b):
+        betagamma_old = b.betagamma
         drift(b, self.alpha_array, 0.5 * self.C)
-        for h, V, dphi, lock in zip(self.h, self.V, self.dphi, self.phi_lock):
-            rf_kick(b, self.C, h, V, dphi, lock)
-        drift(b, self.alpha_array, 0.5 * self.C)
+        for k, (h, V, dphi, lock) in enumerate(zip(self.h, self.V, self.dphi, self.phi_lock)):
+            rf_kick(b, self.C, h, V, dphi, lock, self.p_increment if k == 0 else 0, *self.D)
+        drift(b, self.alpha_array, 0.5 * self.C,
+              self.p_increment if self.shrink_longitudinal else 0)
+        if self.p_increment and self.shrink_transverse:
+            f = np.sqrt(betagamma_old / b.betagamma)
+            b.x = b.x * f
+            b.xp = b.xp * f
+            b.y = b.y * f
+            b.yp = b.yp * f
 
 
 class LinearMapOracle(object):

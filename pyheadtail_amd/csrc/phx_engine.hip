@@ -1,19 +1,22 @@
-// Fused one-turn map.  One turn of [local elements..., WakeField]:
-//   1. particle pass (chunk grid): apply the wake kick left pending by the
+// Fused one-turn map.  One turn of [local elements..., WakeField] is four
+// launches:
+//   1. particle pass (persistent grid, the next particle group loading while
+//      the current one computes): apply the wake kick left pending by the
 //      previous turn (it needs only z and the previous cuts, both unchanged
 //      since), record the bunch moments of that end-of-turn state, run the
 //      local elements (longitudinal maps, transverse segments), write the six
-//      coordinates once (96 B per macroparticle), and store per-chunk partial
-//      sums of the new z moments;
-//   2. finalize (one block): fold the partials in a fixed order into the
-//      slicing cuts and bin edges, the statistics row and the wake-history
-//      bookkeeping (ages, ring head);
-//   3. slice histogram: each block keeps an LDS histogram of counts and
-//      fixed-point sums of x and y (24 B per macroparticle) and stores it;
-//   4. merge (slice-parallel): exact totals, slice means and the new
-//      history entry (times, moments);
-//   5. wake convolution over the history (and, with several turns of
-//      memory, the in-order sum over turns).
+//      coordinates once (96 B per macroparticle), and store per-block
+//      partial sums of the new z moments;
+//   2. slice histogram: every block first folds the pass partials in the
+//      same fixed order into the cuts (block 0 also writes the statistics row
+//      and advances the wake-history ring), then keeps an LDS histogram of
+//      counts and fixed-point sums of x and y (24 B per macroparticle) and
+//      stores it as its partial;
+//   3. merge (slice-parallel): exact totals, slice means, bin edges and the
+//      new history entry (times, moments);
+//   4. wake convolution over the history (and, with several turns of memory,
+//      the in-order sum over turns).
+// Without a wake field a turn is the pass and a one-block finalize.
 // No stage uses cross-block atomics or fences: kernel boundaries order the
 // stages, which keeps every reduction deterministic and avoids the
 // serialisation of device-scope atomics.  The kick of turn t is applied in

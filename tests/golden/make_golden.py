@@ -40,6 +40,11 @@ Fixtures (all float64 unless noted; seeds fixed; 'gamma' is the bunch's):
                  machine, one WakeField over fixed cuts (convolution) and one
                  with n_sigma_z and two turns of memory (dot product);
                  coordinates after each of 3 turns
+  variants.npz   branches the workloads leave out: two transverse segments
+                 with alpha != 0 and dispersion, third-order chromaticity,
+                 two-harmonic RFSystems (phase lock, dispersion, p_increment
+                 with transverse shrinking); 500 particles, coordinates and
+                 (gamma, p0) after each of 5 turns
   multiturn.npz  config-3 shape at small size: linear longitudinal map,
                  WakeField(UniformBinSlicer(20, n_sigma_z=3),
                  CircularResonator(1e6, 1e9, 1, n_turns_wake=4),
@@ -318,12 +323,55 @@ def gen_extra_wakes():
     return extra_wakes(sources)
 
 
+def gen_variants():
+    """Branches the workloads leave out: two transverse segments with
+    alpha != 0 and dispersion, third-order chromaticity, a two-harmonic
+    RFSystems with phase lock, dispersion and acceleration (p_increment, with
+    the transverse shrinking), 5 turns."""
+    from scipy.constants import e, m_p
+    from PyHEADTAIL.particles.generators import (ParticleGenerator, gaussian2D,
+                                                 gaussian2D_asymmetrical)
+    from PyHEADTAIL.trackers.transverse_tracking import TransverseMap
+    from PyHEADTAIL.trackers.detuners import Chromaticity, AmplitudeDetuning
+    from PyHEADTAIL.trackers.longitudinal_tracking import RFSystems
+    np.random.seed(99)
+    eps = EPSN / np.sqrt(GAMMA**2 - 1)
+    beam = ParticleGenerator(macroparticlenumber=500, intensity=1.5e11, charge=e, mass=m_p,
+                             circumference=C, gamma=GAMMA,
+                             distribution_x=gaussian2D(eps), alpha_x=0.4, beta_x=50., D_x=1.2,
+                             distribution_y=gaussian2D(eps), alpha_y=-0.2, beta_y=60., D_y=0.3,
+                             distribution_z=gaussian2D_asymmetrical(sigma_u=0.2,
+                                                                    sigma_up=8e-4)).generate()
+    p0 = beam.p0
+    s = np.array([0., 0.5 * C, C])
+    tm = TransverseMap(s=s, alpha_x=np.array([0.4, -0.3, 0.4]),
+                       beta_x=np.array([50., 35., 50.]), D_x=np.array([1.2, 0.7, 1.2]),
+                       alpha_y=np.array([-0.2, 0.5, -0.2]), beta_y=np.array([60., 45., 60.]),
+                       D_y=np.array([0.3, 0.1, 0.3]), accQ_x=QX, accQ_y=QY,
+                       detuners=[Chromaticity([5., 120., -3000.], [-2., 50., 800.]),
+                                 AmplitudeDetuning(2e-10 * p0, 1e-10 * p0, -5e-11 * p0)])
+    rf = RFSystems(C, [4620, 9240], [4.5e6, 1e6], [0., np.pi], [ALPHA_C], GAMMA,
+                   p_increment=2e-21, D_x=0.7, D_y=0.1, charge=e, mass=m_p)
+    one_turn = [tm[0], rf, tm[1]]
+    out = {'initial': snapshot(beam), 'gamma': np.array([beam.gamma]),
+           'order': np.array([type(el).__name__ for el in one_turn])}
+    coords, gam = [], []
+    for turn in range(5):
+        for el in one_turn:
+            el.track(beam)
+        coords.append(snapshot(beam))
+        gam.append([beam.gamma, beam.p0])
+    out.update(coords=np.stack(coords), gamma_p0=np.array(gam))
+    return out
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else '/root/reference'
     cp = load_reference(ref)
     for name, fn in (('stats', lambda: gen_stats(cp)), ('linear', gen_linear),
                      ('nonlinear', gen_nonlinear), ('multiturn', gen_multiturn),
-                     ('elements', gen_elements), ('extra_wakes', gen_extra_wakes)):
+                     ('elements', gen_elements), ('extra_wakes', gen_extra_wakes),
+                     ('variants', gen_variants)):
         out = fn()
         path = os.path.join(HERE, name + '.npz')
         np.savez_compressed(path, **out)

@@ -202,3 +202,39 @@ def test_extra_wake_kinds():
         for el in [lin, seg] + wakes:
             el.track(b)
         np.testing.assert_array_equal(snapshot(b), g['coords'][turn])
+
+
+def test_dispersion_chromaticity_acceleration_variants():
+    """Two segments with alpha != 0 and dispersion (the last segment closes
+    on the optics of the first boundary), third-order chromaticity and
+    amplitude detuning scaled by the segment's phase advance, a
+    two-harmonic RFSystems with phase lock, dispersion and p_increment."""
+    g = gold('variants')
+    assert list(g['order']) == ['TransverseSegmentMap', 'RFSystems', 'TransverseSegmentMap']
+    b = bunch_from(g['initial'], g['gamma'][0])
+    C = W.CIRCUMFERENCE
+    s = np.array([0., 0.5 * C, C])
+    ax, bx, Dx = [0.4, -0.3, 0.4], [50., 35., 50.], [1.2, 0.7, 1.2]
+    ay, by, Dy = [-0.2, 0.5, -0.2], [60., 45., 60.], [0.3, 0.1, 0.3]
+    Qp_x, Qp_y = [5., 120., -3000.], [-2., 50., 800.]
+    p0 = b.p0
+    app = (2e-10 * p0, 1e-10 * p0, -5e-11 * p0)
+    seg_len = np.diff(s) / s[-1]
+    dQ_x, dQ_y = W.Q_X * seg_len, W.Q_Y * seg_len
+    dmu_x, dmu_y = dQ_x / W.Q_X, dQ_y / W.Q_Y
+    segs = []
+    for k in range(2):
+        s0, s1 = k, (k + 1) % 2
+        dets = [R.Chroma([q * dmu_x[k] for q in Qp_x], [q * dmu_y[k] for q in Qp_y]),
+                R.Amplitude(app[0] * dmu_x[k], app[1] * dmu_y[k], app[2] * dmu_x[k],
+                            app[2] * dmu_y[k], ax[s0], bx[s0], ay[s0], by[s0])]
+        segs.append(R.TransverseSegment(ax[s0], bx[s0], Dx[s0], ax[s1], bx[s1], Dx[s1],
+                                        ay[s0], by[s0], Dy[s0], ay[s1], by[s1], Dy[s1],
+                                        dQ_x[k], dQ_y[k], dets))
+    rf = R.RFSystemsOracle(C, [4620, 9240], [4.5e6, 1e6], [0., np.pi], [W.ALPHA_C],
+                           g['gamma'][0], e, p_increment=2e-21, D_x=0.7, D_y=0.1)
+    for turn in range(5):
+        for el in (segs[0], rf, segs[1]):
+            el.track(b)
+        np.testing.assert_array_equal(snapshot(b), g['coords'][turn])
+        assert (b.gamma, b.p0) == tuple(g['gamma_p0'][turn])

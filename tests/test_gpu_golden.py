@@ -214,3 +214,37 @@ def test_extra_wake_kinds_vs_reference(fused):
         if fused:
             eng.track_turns(pb, 1)
         assert rel(snap(pb), g['coords'][turn]) <= 1e-10, turn
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_dispersion_chromaticity_acceleration_vs_reference(fused):
+    _case("two segments with alpha and dispersion, third-order chromaticity, two-harmonic "
+          "RF with acceleration, fused=%s" % fused)
+    from pyheadtail_amd.trackers.transverse_tracking import TransverseMap
+    from pyheadtail_amd.trackers.detuners import Chromaticity, AmplitudeDetuning
+    from pyheadtail_amd.trackers.longitudinal_tracking import RFSystems
+    g = gold('variants')
+    pb = product_bunch(g['initial'], g['gamma'][0])
+    C = W.CIRCUMFERENCE
+    p0 = pb.p0
+    tm = TransverseMap(s=np.array([0., 0.5 * C, C]), alpha_x=np.array([0.4, -0.3, 0.4]),
+                       beta_x=np.array([50., 35., 50.]), D_x=np.array([1.2, 0.7, 1.2]),
+                       alpha_y=np.array([-0.2, 0.5, -0.2]), beta_y=np.array([60., 45., 60.]),
+                       D_y=np.array([0.3, 0.1, 0.3]), accQ_x=W.Q_X, accQ_y=W.Q_Y,
+                       detuners=[Chromaticity([5., 120., -3000.], [-2., 50., 800.]),
+                                 AmplitudeDetuning(2e-10 * p0, 1e-10 * p0, -5e-11 * p0)])
+    rf = RFSystems(C, [4620, 9240], [4.5e6, 1e6], [0., np.pi], [W.ALPHA_C], g['gamma'][0],
+                   p_increment=2e-21, D_x=0.7, D_y=0.1, charge=e, mass=m_p)
+    one_turn = [tm[0], rf, tm[1]]
+    assert order(one_turn) == list(g['order'])
+    if fused:
+        from pyheadtail_amd.engine import FusedOneTurnMap
+        eng = FusedOneTurnMap(one_turn, graph_turns=0)
+    for turn in range(5):
+        if fused:
+            eng.track_turns(pb, 1)
+        else:
+            for el in one_turn:
+                el.track(pb)
+        assert rel(snap(pb), g['coords'][turn]) <= 1e-10, turn
+        assert (pb.gamma, pb.p0) == tuple(g['gamma_p0'][turn])
