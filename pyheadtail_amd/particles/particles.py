@@ -27,44 +27,29 @@ from ..general.element import Printing
 COORDS = ("x", "xp", "y", "yp", "z", "dp")
 
 
+def _kinematics_of_gamma(gamma, mass):
+    """(beta, betagamma, p0) derived from gamma as the reference's gamma
+    setter does (particles/particles.py:74-103)."""
+    betagamma = np.sqrt(gamma**2 - 1)
+    return np.sqrt(1 - gamma**-2), betagamma, betagamma * mass * c
+
+
 class Kinematics(object):
-    """gamma, beta, betagamma and p0 with the reference's setter arithmetic
-    (particles/particles.py:74-103).  Needs self.mass."""
+    """gamma, beta, betagamma and p0, each settable; setting one re-derives
+    the others through gamma with the reference's expressions, so host
+    scalars stay bit-identical to the reference's.  Needs self.mass."""
 
-    @property
-    def gamma(self):
-        return self._gamma
+    def _set_gamma(self, gamma):
+        self._gamma = gamma
+        self._beta, self._betagamma, self._p0 = _kinematics_of_gamma(gamma, self.mass)
 
-    @gamma.setter
-    def gamma(self, value):
-        self._gamma = value
-        self._beta = np.sqrt(1 - self.gamma**-2)
-        self._betagamma = np.sqrt(self.gamma**2 - 1)
-        self._p0 = self.betagamma * self.mass * c
-
-    @property
-    def beta(self):
-        return self._beta
-
-    @beta.setter
-    def beta(self, value):
-        self.gamma = 1. / np.sqrt(1 - value**2)
-
-    @property
-    def betagamma(self):
-        return self._betagamma
-
-    @betagamma.setter
-    def betagamma(self, value):
-        self.gamma = np.sqrt(value**2 + 1)
-
-    @property
-    def p0(self):
-        return self._p0
-
-    @p0.setter
-    def p0(self, value):
-        self.gamma = np.sqrt(1 + (value / (self.mass * c))**2)
+    gamma = property(lambda self: self._gamma, _set_gamma)
+    beta = property(lambda self: self._beta,
+                    lambda self, v: self._set_gamma(1. / np.sqrt(1 - v**2)))
+    betagamma = property(lambda self: self._betagamma,
+                         lambda self, v: self._set_gamma(np.sqrt(v**2 + 1)))
+    p0 = property(lambda self: self._p0,
+                  lambda self, v: self._set_gamma(np.sqrt(1 + (v / (self.mass * c))**2)))
 
 
 class ScalarState(Kinematics):
@@ -308,42 +293,6 @@ class Particles(Kinematics, Printing):
         sigma22 = cov_up2 - term_up2_dp
         return np.sqrt(sigma11 * sigma22 - sigma12 * sigma12)
 
-    def mean_x(self, **kwargs):
-        return self._mean(self.x)
-
-    def mean_xp(self, **kwargs):
-        return self._mean(self.xp)
-
-    def mean_y(self, **kwargs):
-        return self._mean(self.y)
-
-    def mean_yp(self, **kwargs):
-        return self._mean(self.yp)
-
-    def mean_z(self, **kwargs):
-        return self._mean(self.z)
-
-    def mean_dp(self, **kwargs):
-        return self._mean(self.dp)
-
-    def sigma_x(self, **kwargs):
-        return self._std(self.x)
-
-    def sigma_y(self, **kwargs):
-        return self._std(self.y)
-
-    def sigma_z(self, **kwargs):
-        return self._std(self.z)
-
-    def sigma_xp(self, **kwargs):
-        return self._std(self.xp)
-
-    def sigma_yp(self, **kwargs):
-        return self._std(self.yp)
-
-    def sigma_dp(self, **kwargs):
-        return self._std(self.dp)
-
     def effective_normalized_emittance_x(self, **kwargs):
         return self._emittance(self.x, self.xp, None) * self.betagamma
 
@@ -361,3 +310,19 @@ class Particles(Kinematics, Printing):
 
     def epsn_z(self, **kwargs):
         return self.effective_normalized_emittance_z(**kwargs)
+
+
+def _add_moment_methods(cls):
+    """mean_<u>() and sigma_<u>() for the six coordinates."""
+    for name in COORDS:
+        def mean(self, _name=name, **kwargs):
+            return self._mean(getattr(self, _name))
+
+        def sigma(self, _name=name, **kwargs):
+            return self._std(getattr(self, _name))
+        mean.__name__, sigma.__name__ = 'mean_' + name, 'sigma_' + name
+        setattr(cls, mean.__name__, mean)
+        setattr(cls, sigma.__name__, sigma)
+
+
+_add_moment_methods(Particles)
