@@ -18,7 +18,6 @@ reference's formula on the host (numpy); fill(record) writes the host-side
 constants of that formula into a phx_wake_kick record for the device.
 The WakeField keeps its SliceSet history on the device (WakeHistory).
 """
-from abc import ABCMeta, abstractmethod
 from collections import deque
 
 import numpy as np
@@ -115,20 +114,42 @@ class ResistiveWallWake(object):
         k.sqrt_z0c = np.sqrt(self.Z_0 * c)
 
 
-class WakeSource(Printing, metaclass=ABCMeta):
-    @abstractmethod
+# Yokoya factors (X1, Y1, X2, Y2) of the standard chamber geometries
+_PI2 = np.pi**2
+YOKOYA_FACTORS = {
+    'circular': (1., 1., 0., 0.),
+    'horizontal plates': (_PI2 / 24., _PI2 / 12., -_PI2 / 24., _PI2 / 24.),
+    'vertical plates': (_PI2 / 12., _PI2 / 24., _PI2 / 24., -_PI2 / 24.),
+}
+
+# (Yokoya attribute, kick class), in the order the kicks are produced
+_TRANSVERSE_KICKS = (('Yokoya_X1', DipoleWakeKickX), ('Yokoya_X2', QuadrupoleWakeKickX),
+                     ('Yokoya_Y1', DipoleWakeKickY), ('Yokoya_Y2', QuadrupoleWakeKickY))
+
+
+class WakeSource(Printing):
+    """A source of wake kicks: one kick per non-zero Yokoya factor, each
+    with its own wake function object."""
+
     def get_wake_kicks(self, slicer):
-        pass
+        kicks = [kick_class(self.function_transverse(getattr(self, factor)), slicer,
+                            self.n_turns_wake)
+                 for factor, kick_class in _TRANSVERSE_KICKS if getattr(self, factor)]
+        if getattr(self, 'switch_Z', False):
+            kicks.append(ConstantWakeKickZ(self.function_longitudinal(), slicer,
+                                           self.n_turns_wake))
+        return kicks
+
+    def _set_yokoya(self, X1, Y1, X2, Y2):
+        self.Yokoya_X1, self.Yokoya_Y1, self.Yokoya_X2, self.Yokoya_Y2 = X1, Y1, X2, Y2
 
 
 class Resonator(WakeSource):
     def __init__(self, R_shunt, frequency, Q, Yokoya_X1, Yokoya_Y1, Yokoya_X2, Yokoya_Y2,
                  switch_Z, n_turns_wake=1, *args, **kwargs):
         self.R_shunt, self.frequency, self.Q = R_shunt, frequency, Q
-        self.Yokoya_X1, self.Yokoya_X2 = Yokoya_X1, Yokoya_X2
-        self.Yokoya_Y1, self.Yokoya_Y2 = Yokoya_Y1, Yokoya_Y2
-        self.switch_Z = switch_Z
-        self.n_turns_wake = n_turns_wake
+        self._set_yokoya(Yokoya_X1, Yokoya_Y1, Yokoya_X2, Yokoya_Y2)
+        self.switch_Z, self.n_turns_wake = switch_Z, n_turns_wake
 
     def function_transverse(self, Yokoya_factor):
         return ResonatorTransverseWake(Yokoya_factor, self.R_shunt, self.frequency, self.Q)
@@ -136,98 +157,51 @@ class Resonator(WakeSource):
     def function_longitudinal(self):
         return ResonatorLongitudinalWake(self.R_shunt, self.frequency, self.Q)
 
-    def get_wake_kicks(self, slicer):
-        kicks = []
-        if self.Yokoya_X1:
-            kicks.append(DipoleWakeKickX(self.function_transverse(self.Yokoya_X1), slicer,
-                                         self.n_turns_wake))
-        if self.Yokoya_X2:
-            kicks.append(QuadrupoleWakeKickX(self.function_transverse(self.Yokoya_X2), slicer,
-                                             self.n_turns_wake))
-        if self.Yokoya_Y1:
-            kicks.append(DipoleWakeKickY(self.function_transverse(self.Yokoya_Y1), slicer,
-                                         self.n_turns_wake))
-        if self.Yokoya_Y2:
-            kicks.append(QuadrupoleWakeKickY(self.function_transverse(self.Yokoya_Y2), slicer,
-                                             self.n_turns_wake))
-        if self.switch_Z:
-            kicks.append(ConstantWakeKickZ(self.function_longitudinal(), slicer,
-                                           self.n_turns_wake))
-        return kicks
-
-
-class CircularResonator(Resonator):
-    def __init__(self, R_shunt, frequency, Q, n_turns_wake=1, *args, **kwargs):
-        super(CircularResonator, self).__init__(R_shunt, frequency, Q, 1., 1., 0., 0., False,
-                                                n_turns_wake, *args, **kwargs)
-
-
-class ParallelHorizontalPlatesResonator(Resonator):
-    def __init__(self, R_shunt, frequency, Q, n_turns_wake=1, *args, **kwargs):
-        super(ParallelHorizontalPlatesResonator, self).__init__(
-            R_shunt, frequency, Q, np.pi**2 / 24., np.pi**2 / 12., -np.pi**2 / 24.,
-            np.pi**2 / 24., False, n_turns_wake, *args, **kwargs)
-
-
-ParallelPlatesResonator = ParallelHorizontalPlatesResonator
-
-
-class ParallelVerticalPlatesResonator(Resonator):
-    def __init__(self, R_shunt, frequency, Q, n_turns_wake=1, *args, **kwargs):
-        super(ParallelVerticalPlatesResonator, self).__init__(
-            R_shunt, frequency, Q, np.pi**2 / 12., np.pi**2 / 24., np.pi**2 / 24.,
-            -np.pi**2 / 24., False, n_turns_wake, *args, **kwargs)
-
 
 class ResistiveWall(WakeSource):
     def __init__(self, pipe_radius, resistive_wall_length, conductivity, dt_min, Yokoya_X1,
                  Yokoya_Y1, Yokoya_X2, Yokoya_Y2, n_turns_wake=1, *args, **kwargs):
         self.pipe_radius = np.array([pipe_radius]).flatten()
-        self.resistive_wall_length = resistive_wall_length
-        self.conductivity = conductivity
-        self.dt_min = dt_min
-        self.Yokoya_X1, self.Yokoya_Y1 = Yokoya_X1, Yokoya_Y1
-        self.Yokoya_X2, self.Yokoya_Y2 = Yokoya_X2, Yokoya_Y2
-        self.n_turns_wake = n_turns_wake
+        self.resistive_wall_length, self.conductivity = resistive_wall_length, conductivity
+        self.dt_min, self.n_turns_wake = dt_min, n_turns_wake
+        self._set_yokoya(Yokoya_X1, Yokoya_Y1, Yokoya_X2, Yokoya_Y2)
 
     def function_transverse(self, Yokoya_factor):
         return ResistiveWallWake(Yokoya_factor, self.pipe_radius, self.resistive_wall_length,
                                  self.conductivity, self.dt_min)
 
-    def get_wake_kicks(self, slicer):
-        kicks = []
-        if self.Yokoya_X1:
-            kicks.append(DipoleWakeKickX(self.function_transverse(self.Yokoya_X1), slicer,
-                                         self.n_turns_wake))
-        if self.Yokoya_X2:
-            kicks.append(QuadrupoleWakeKickX(self.function_transverse(self.Yokoya_X2), slicer,
-                                             self.n_turns_wake))
-        if self.Yokoya_Y1:
-            kicks.append(DipoleWakeKickY(self.function_transverse(self.Yokoya_Y1), slicer,
-                                         self.n_turns_wake))
-        if self.Yokoya_Y2:
-            kicks.append(QuadrupoleWakeKickY(self.function_transverse(self.Yokoya_Y2), slicer,
-                                             self.n_turns_wake))
-        return kicks
+
+class CircularResonator(Resonator):
+    def __init__(self, R_shunt, frequency, Q, n_turns_wake=1, *args, **kwargs):
+        Resonator.__init__(self, R_shunt, frequency, Q, *YOKOYA_FACTORS['circular'], False,
+                           n_turns_wake, *args, **kwargs)
+
+
+class ParallelHorizontalPlatesResonator(Resonator):
+    def __init__(self, R_shunt, frequency, Q, n_turns_wake=1, *args, **kwargs):
+        Resonator.__init__(self, R_shunt, frequency, Q, *YOKOYA_FACTORS['horizontal plates'],
+                           False, n_turns_wake, *args, **kwargs)
+
+
+class ParallelVerticalPlatesResonator(Resonator):
+    def __init__(self, R_shunt, frequency, Q, n_turns_wake=1, *args, **kwargs):
+        Resonator.__init__(self, R_shunt, frequency, Q, *YOKOYA_FACTORS['vertical plates'],
+                           False, n_turns_wake, *args, **kwargs)
 
 
 class CircularResistiveWall(ResistiveWall):
     def __init__(self, pipe_radius, resistive_wall_length, conductivity, dt_min,
                  n_turns_wake=1, *args, **kwargs):
-        super(CircularResistiveWall, self).__init__(
-            pipe_radius, resistive_wall_length, conductivity, dt_min, 1., 1., 0., 0.,
-            n_turns_wake, *args, **kwargs)
+        ResistiveWall.__init__(self, pipe_radius, resistive_wall_length, conductivity, dt_min,
+                               *YOKOYA_FACTORS['circular'], n_turns_wake, *args, **kwargs)
 
 
 class ParallelHorizontalPlatesResistiveWall(ResistiveWall):
     def __init__(self, pipe_radius, resistive_wall_length, conductivity, dt_min,
                  n_turns_wake=1, *args, **kwargs):
-        super(ParallelHorizontalPlatesResistiveWall, self).__init__(
-            pipe_radius, resistive_wall_length, conductivity, dt_min, np.pi**2 / 24.,
-            np.pi**2 / 12., -np.pi**2 / 24., np.pi**2 / 24., n_turns_wake, *args, **kwargs)
-
-
-ParallelPlatesResistiveWall = ParallelHorizontalPlatesResistiveWall
+        ResistiveWall.__init__(self, pipe_radius, resistive_wall_length, conductivity, dt_min,
+                               *YOKOYA_FACTORS['horizontal plates'], n_turns_wake, *args,
+                               **kwargs)
 
 
 class ParallelVerticalPlatesResistiveWall(ResistiveWall):
@@ -237,9 +211,13 @@ class ParallelVerticalPlatesResistiveWall(ResistiveWall):
 
     def __init__(self, pipe_radius, resistive_wall_length, conductivity, dt_min,
                  n_turns_wake=1, *args, **kwargs):
-        super(ParallelVerticalPlatesResistiveWall, self).__init__(
-            pipe_radius, resistive_wall_length, conductivity, dt_min, np.pi**2 / 12.,
-            np.pi**2 / 24., np.pi**2 / 24., -np.pi**2 / 24., n_turns_wake, *args, **kwargs)
+        ResistiveWall.__init__(self, pipe_radius, resistive_wall_length, conductivity, dt_min,
+                               *YOKOYA_FACTORS['vertical plates'], n_turns_wake, *args,
+                               **kwargs)
+
+
+ParallelPlatesResonator = ParallelHorizontalPlatesResonator
+ParallelPlatesResistiveWall = ParallelHorizontalPlatesResistiveWall
 
 
 class WakeField(Element):
@@ -247,22 +225,24 @@ class WakeField(Element):
 
     def __init__(self, slicer, *wake_sources):
         self.slicer = slicer
-        self.wake_kicks = []
-        for source in wake_sources:
-            self.wake_kicks.extend(source.get_wake_kicks(self.slicer))
-        n_turns_wake_max = max(source.n_turns_wake for source in wake_sources)
-        self.n_turns_wake_max = n_turns_wake_max
-        self.slice_set_deque = deque([], maxlen=n_turns_wake_max)
-        self.slice_set_age_deque = deque([], maxlen=n_turns_wake_max)
+        self.wake_kicks = [kick for src in wake_sources for kick in src.get_wake_kicks(slicer)]
+        self.n_turns_wake_max = max(src.n_turns_wake for src in wake_sources)
+        # the remembered slicings, newest first, and their ages in seconds
+        self.slice_set_deque = deque(maxlen=self.n_turns_wake_max)
+        self.slice_set_age_deque = deque(maxlen=self.n_turns_wake_max)
         self._history = None
+
+    def _remember(self, slice_set, turn_time):
+        older = [age + turn_time for age in self.slice_set_age_deque]
+        self.slice_set_age_deque.clear()
+        self.slice_set_age_deque.extend(older)
+        self.slice_set_age_deque.appendleft(0.)
+        self.slice_set_deque.appendleft(slice_set)
 
     def track(self, bunch):
         age_increment = bunch.circumference / (bunch.beta * c)
-        for i in range(len(self.slice_set_age_deque)):
-            self.slice_set_age_deque[i] += age_increment
         slice_set = bunch.get_slices(self.slicer, statistics=['mean_x', 'mean_y'])
-        self.slice_set_deque.appendleft(slice_set)
-        self.slice_set_age_deque.appendleft(0.)
+        self._remember(slice_set, age_increment)
         if not self.wake_kicks:
             return
         if self._history is None:
