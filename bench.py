@@ -29,6 +29,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_MP_PASS = 96
+EPSN = 2.5e-6          # normalised emittance, both planes [m]
+SIGMA_Z = 0.23         # bunch length [m]
+INTENSITY = 1.5e11
+SEED = 12345
 
 
 def parse():
@@ -38,7 +42,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=1000000)
     ap.add_argument("--slices", type=int, default=500)
-    ap.add_argument("--cpu-turns", type=int, default=40)
+    ap.add_argument("--cpu-turns", type=int, default=150)
     ap.add_argument("--cpu-n", type=int, default=1000000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-turns", type=int, default=100)
@@ -74,7 +78,7 @@ def cpu_baseline(n, turns):
         old = None
     try:
         from oracle import workloads as W
-        ob = W.nonlinear_bunch(n, seed=12345)
+        ob = W.nonlinear_bunch(n, seed=SEED)
         m = W.nonlinear_map_oracle()
         for el in m:          # one untimed turn (imports, caches)
             el.track(ob)
@@ -110,12 +114,15 @@ def pass_traffic(n):
     return read + written
 
 
-def build_machine(p0_ref):
+def p0_of_gamma(gamma):
+    from scipy.constants import c, m_p
     import numpy as np
+    return np.sqrt(gamma**2 - 1) * m_p * c
+
+
+def build_machine(p0_ref):
     from scipy.constants import e, m_p
     from pyheadtail_amd.machines.synchrotron import Synchrotron
-    from pyheadtail_amd.particles.slicing import UniformBinSlicer
-    from pyheadtail_amd.impedances.wakes import WakeField, CircularResonator
     mach = Synchrotron(optics_mode='smooth', charge=e, mass=m_p, p0=p0_ref,
                        circumference=6911.5, n_segments=1, beta_x=54.6, beta_y=54.6,
                        D_x=0., D_y=0., accQ_x=20.13, accQ_y=20.18, Qp_x=5., Qp_y=5.,
@@ -140,18 +147,17 @@ def main():
     import numpy as np
     import torch
     torch.cuda.set_device(0)
-    from scipy.constants import c
     from pyheadtail_amd.engine import FusedOneTurnMap
     from pyheadtail_amd.particles.slicing import UniformBinSlicer
     from pyheadtail_amd.impedances.wakes import WakeField, CircularResonator
-    from oracle import workloads as W
-    from tests.machines import to_product
 
-    p0 = W.P0
-    mach = build_machine(p0)
+    mach = build_machine(p0_of_gamma(27.7))
     wake = WakeField(UniformBinSlicer(args.slices, n_sigma_z=3), CircularResonator(1e6, 1e9, 1))
     one_turn = list(mach.one_turn_map) + [wake]
-    bunch = to_product(W.nonlinear_bunch(args.n, seed=12345))
+    # the Gaussian bunch cut at the RF separatrix, drawn with NumPy's legacy
+    # global RNG as PyHEADTAIL does (machines/synchrotron.py:309-360)
+    np.random.seed(SEED)
+    bunch = mach.generate_6D_Gaussian_bunch(args.n, INTENSITY, EPSN, EPSN, SIGMA_Z)
     eng = FusedOneTurnMap(one_turn, graph_turns=args.graph_turns)
 
     # warm-up (graph capture happens here)

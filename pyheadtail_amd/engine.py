@@ -1,4 +1,4 @@
-"""Fused one-turn map: many turns of [local elements..., WakeField] with three
+"""Fused one-turn map: many turns of [local elements..., WakeField] with four
 kernel launches per turn and no host synchronisation.
 
 FusedOneTurnMap(one_turn_map) accepts the element list that a user would

@@ -6,7 +6,7 @@ Usage, from the repository root:
 
     make -C oracle                       # builds the reference's Cython
                                          # statistics module into oracle/_ref
-    python tests/golden/make_golden.py [REFERENCE_ROOT]
+    python tests/golden/make_golden.py [REFERENCE_ROOT [FIXTURE ...]]
 
 REFERENCE_ROOT defaults to /root/reference (the directory holding the
 PyHEADTAIL package).  The script imports the reference package from there
@@ -17,19 +17,19 @@ Fixtures (all float64 unless noted; seeds fixed; 'gamma' is the bunch's):
   stats.npz      cov / std / emittance of cobra_functions.stats on seeded
                  arrays (n = 1000, 2, 1), bunch-level and per-slice
                  statistics of a SliceSet (UniformBinSlicer(10, n_sigma_z=2))
-  linear.npz     config-1 shape: ParticleGenerator bunch (N = 1000,
+  linear.npz     linear-map ring: ParticleGenerator bunch (N = 1000,
                  sigma_z = 0.23, sigma_dp = 1e-3 as stated), Synchrotron
                  one-turn map with LinearMap (Q_s = 0.017), 10 turns; per
                  turn UniformBinSlicer(50, n_sigma_z=3) counts / mean_x /
                  mean_y / (tail, head, width), coordinates after turns
                  1, 2 and 10
-  nonlinear.npz  config-2 shape: Synchrotron.generate_6D_Gaussian_bunch
+  nonlinear.npz  RF + resonator ring: Synchrotron.generate_6D_Gaussian_bunch
                  (N = 1000, separatrix cut) of the non-linear RF machine
                  with Q' = 5 and amplitude detuning, plus
                  WakeField(UniformBinSlicer(50, n_sigma_z=3),
                  CircularResonator(1e6, 1e9, 1)), 5 turns: bunch statistics
                  and the wake slicing per turn, coordinates after turns 1, 5
-  elements.npz   one turn of the config-2 machine element by element (after
+  elements.npz   one turn of the RF + resonator ring element by element (after
                  RFSystems, after the transverse segment, after the
                  WakeField), RFBucket Q_s, beta_z, sigma_dp and the
                  separatrix acceptance (margin 0.05) on an 81 x 81 grid
@@ -45,7 +45,13 @@ Fixtures (all float64 unless noted; seeds fixed; 'gamma' is the bunch's):
                  two-harmonic RFSystems (phase lock, dispersion, p_increment
                  with transverse shrinking); 500 particles, coordinates and
                  (gamma, p0) after each of 5 turns
-  multiturn.npz  config-3 shape at small size: linear longitudinal map,
+  growth.npz     3000 turns of the broadband-resonator instability ring of
+                 tests/simulation_tests/test_bb_resonator.py (two RF
+                 harmonics, 200 slices over +-6 sigma_z) from a seeded,
+                 separatrix-cut Gaussian bunch of 1e5 macroparticles:
+                 checksums and the first 64 particles of the initial bunch,
+                 bunch statistics after every turn, fitted growth rates
+  multiturn.npz  multi-turn wake memory at small size: linear longitudinal map,
                  WakeField(UniformBinSlicer(20, n_sigma_z=3),
                  CircularResonator(1e6, 1e9, 1, n_turns_wake=4),
                  CircularResistiveWall(0.02, C, 5.9e7, 1e-12, n_turns_wake=4)),
@@ -209,7 +215,7 @@ def gen_nonlinear():
 
 
 def gen_elements():
-    """Element by element through one turn of the config-2 machine, and the
+    """Element by element through one turn of the RF + resonator ring, and the
     quantities that fix its bunch: RFBucket Q_s, sigma_dp, the separatrix
     acceptance on a grid."""
     from scipy.constants import c, m_p
@@ -365,14 +371,79 @@ def gen_variants():
     return out
 
 
+# the broadband-resonator instability of tests/simulation_tests/
+# test_bb_resonator.py:13-67 (expected growth 59e-4 / 25e-4 per turn there,
+# with a matched bunch)
+GROWTH = dict(n=100000, turns=3000, fit_from=1500, seed=42, intensity=2.5e11, epsn=2e-6,
+              sigma_z=0.23, circumference=6911.5038378975451, p0_eV=26e9,
+              beta_x=54.644808743169399, beta_y=54.509415262636274, alpha=0.0030864197530864196,
+              R_shunt=10.2e6, frequency=1.3e9, Q=1., n_slices=200, n_sigma_z=6.)
+
+
+def growth_rate(signal, fit_from):
+    """Slope of log|analytic signal| over turns fit_from.. (the fit of
+    test_bb_resonator.py:97-104)."""
+    from scipy.signal import hilbert
+    from scipy.stats import linregress
+    turns = np.arange(len(signal))
+    return linregress(turns[fit_from:], np.log(np.abs(hilbert(signal))[fit_from:])).slope
+
+
+def gen_growth():
+    """3000 turns of the broadband-resonator ring (two RF harmonics, 200
+    slices over +-6 sigma_z) from a separatrix-cut Gaussian bunch of 1e5
+    macroparticles: bunch statistics after every turn and the fitted
+    horizontal and vertical growth rates."""
+    from scipy.constants import c, e, m_p
+    from PyHEADTAIL.machines.synchrotron import Synchrotron
+    from PyHEADTAIL.particles.slicing import UniformBinSlicer
+    from PyHEADTAIL.impedances.wakes import WakeField, CircularResonator
+    G = GROWTH
+    mach = Synchrotron(optics_mode='smooth', circumference=G['circumference'], n_segments=1,
+                       beta_x=G['beta_x'], beta_y=G['beta_y'], D_x=0., D_y=0., accQ_x=QX,
+                       accQ_y=QY, Qp_x=0., Qp_y=0., alpha_mom_compaction=G['alpha'],
+                       longitudinal_mode='non-linear', h_RF=[4620, 4 * 4620],
+                       V_RF=[4.5e6, 0.45e6], dphi_RF=[0., np.pi], p_increment=0.,
+                       p0=G['p0_eV'] * e / c, charge=e, mass=m_p)
+    wake = WakeField(UniformBinSlicer(G['n_slices'], n_sigma_z=G['n_sigma_z']),
+                     CircularResonator(G['R_shunt'], G['frequency'], G['Q']))
+    one_turn = list(mach.one_turn_map) + [wake]
+    np.random.seed(G['seed'])
+    beam = mach.generate_6D_Gaussian_bunch(G['n'], G['intensity'], G['epsn'], G['epsn'],
+                                           G['sigma_z'])
+    init = snapshot(beam)
+    out = dict(order=np.array([type(el).__name__ for el in one_turn]),
+               gamma=np.array([beam.gamma]), initial_sum=init.sum(axis=1),
+               initial_sumsq=(init * init).sum(axis=1), initial_head=init[:, :64],
+               params=np.array(sorted(G.items()), dtype=object).astype(str))
+    names = ('mean_x', 'mean_y', 'mean_z', 'sigma_z', 'epsn_x', 'epsn_y', 'epsn_z')
+    hist = []
+    for turn in range(G['turns']):
+        for el in one_turn:
+            el.track(beam)
+        hist.append([getattr(beam, k)() for k in names])
+    hist = np.array(hist)
+    out.update(stat_names=np.array(names), history=hist, final_head=snapshot(beam)[:, :64],
+               growth=np.array([growth_rate(hist[:, 0], G['fit_from']),
+                                growth_rate(hist[:, 1], G['fit_from'])]))
+    print("growth rates x, y: %.3e %.3e per turn" % tuple(out['growth']))
+    return out
+
+
+GENERATORS = ('stats', 'linear', 'nonlinear', 'multiturn', 'elements', 'extra_wakes',
+              'variants', 'growth')
+
+
 def main():
+    """make_golden.py [reference root] [fixture names...] (default: all)."""
     ref = sys.argv[1] if len(sys.argv) > 1 else '/root/reference'
+    only = sys.argv[2:] or GENERATORS
     cp = load_reference(ref)
-    for name, fn in (('stats', lambda: gen_stats(cp)), ('linear', gen_linear),
-                     ('nonlinear', gen_nonlinear), ('multiturn', gen_multiturn),
-                     ('elements', gen_elements), ('extra_wakes', gen_extra_wakes),
-                     ('variants', gen_variants)):
-        out = fn()
+    fns = dict(stats=lambda: gen_stats(cp), linear=gen_linear, nonlinear=gen_nonlinear,
+               multiturn=gen_multiturn, elements=gen_elements, extra_wakes=gen_extra_wakes,
+               variants=gen_variants, growth=gen_growth)
+    for name in only:
+        out = fns[name]()
         path = os.path.join(HERE, name + '.npz')
         np.savez_compressed(path, **out)
         print("wrote %s (%d bytes)" % (path, os.path.getsize(path)))

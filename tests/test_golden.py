@@ -148,8 +148,8 @@ def test_multiturn_dot_product_wake():
             np.testing.assert_array_equal(snapshot(b), g['coords'][keep.index(turn)])
 
 
-def test_config2_elements_and_bucket():
-    """One turn of the config-2 machine element by element, the bucket's
+def test_rf_machine_elements_and_bucket():
+    """One turn of the RF + resonator machine element by element, the bucket's
     Q_s and the separatrix acceptance (margin 0.05) on a grid."""
     g = gold('elements')
     assert W.q_s_single_harmonic(W.GAMMA, W.ALPHA_C, W.H_RF, W.V_RF) == g['Q_s'][0]
