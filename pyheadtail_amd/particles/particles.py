@@ -9,7 +9,8 @@ Same contract as PyHEADTAIL particles/particles.py:16-390:
     reference's;
   - get_slices(slicer, statistics=...) caches one SliceSet per slicer
     configuration and re-slices whenever statistics are requested (:118-147);
-  - mean_* / sigma_* / epsn_* statistics (:311-366).
+  - mean_* / sigma_* / epsn_* statistics (:311-366), dispersion_* and the
+    statistical Twiss parameters alpha/beta/gamma_Twiss_* (:368-390).
 
 Coordinates are torch float64 tensors on the current HIP device; numpy
 arrays given to the constructor or update() are copied there.  Elements of
@@ -95,7 +96,12 @@ def _coordinate(name):
     def getter(self):
         if self._pending is not None:
             self._run_pending()
-        return self._coords[name]
+        try:
+            return self._coords[name]
+        except KeyError:
+            # a bunch without this plane (e.g. transverse only), as a missing
+            # attribute of the reference's Particles
+            raise AttributeError("this Particles object has no coordinate %r" % name)
 
     def setter(self, value):
         if self._pending is not None:
@@ -310,6 +316,54 @@ class Particles(Kinematics, Printing):
 
     def epsn_z(self, **kwargs):
         return self.effective_normalized_emittance_z(**kwargs)
+
+    # -- dispersion and statistical Twiss parameters (stats.pyx:57-179) ------
+    def _dispersion(self, u):
+        """<u dp> / <dp^2>, or 0 when <dp^2> is 0."""
+        dp = self.dp
+        mean_u_dp, mean_dp2 = self._mean(u * dp), self._mean(dp * dp)
+        return mean_u_dp / mean_dp2 if mean_dp2 > 0 else 0
+
+    def dispersion_x(self):
+        return self._dispersion(self.x)
+
+    def dispersion_y(self):
+        return self._dispersion(self.y)
+
+    def _twiss(self, u, up, dp, entry):
+        """Entry (0, 0) beta, (0, 1) alpha, (1, 1) gamma of the beam matrix
+        with the dispersion terms removed, over the emittance.  Without dp the
+        dispersion terms are 0 (cov_dp2 stands at 1); with dp, cov_dp2 is
+        used as it is (stats.pyx:get_alpha / get_beta / get_gamma)."""
+        arrays = [u, up] + ([dp] if dp is not None else [])
+        k = len(arrays)
+        s = self._sums(arrays)
+        cov = lambda i, j: self._cov_from_sums(s, k, i, j)   # noqa: E731
+        if dp is None:
+            cov_dp2, disp = 1., (0., 0.)
+        else:
+            cov_dp2, disp = cov(2, 2), (cov(0, 2), cov(1, 2))
+        i, j = entry
+        sigma = cov(i, j) - disp[i] * disp[j] / cov_dp2
+        return (-sigma if entry == (0, 1) else sigma) / self._emittance(u, up, dp)
+
+    def alpha_Twiss_x(self):
+        return self._twiss(self.x, self.xp, getattr(self, 'dp', None), (0, 1))
+
+    def alpha_Twiss_y(self):
+        return self._twiss(self.y, self.yp, getattr(self, 'dp', None), (0, 1))
+
+    def beta_Twiss_x(self):
+        return self._twiss(self.x, self.xp, getattr(self, 'dp', None), (0, 0))
+
+    def beta_Twiss_y(self):
+        return self._twiss(self.y, self.yp, getattr(self, 'dp', None), (0, 0))
+
+    def gamma_Twiss_x(self):
+        return self._twiss(self.x, self.xp, getattr(self, 'dp', None), (1, 1))
+
+    def gamma_Twiss_y(self):
+        return self._twiss(self.y, self.yp, getattr(self, 'dp', None), (1, 1))
 
 
 def _add_moment_methods(cls):

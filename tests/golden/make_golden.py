@@ -15,8 +15,10 @@ oracle/_ref, with bytecode writing off (the reference tree is read-only).
 
 Fixtures (all float64 unless noted; seeds fixed; 'gamma' is the bunch's):
   stats.npz      cov / std / emittance of cobra_functions.stats on seeded
-                 arrays (n = 1000, 2, 1), bunch-level and per-slice
-                 statistics of a SliceSet (UniformBinSlicer(10, n_sigma_z=2))
+                 arrays (n = 1000, 2, 1), bunch-level statistics (with the
+                 statistical Twiss parameters and dispersions) and per-slice
+                 statistics of a SliceSet (UniformBinSlicer(10, n_sigma_z=2)),
+                 UniformBinSlicer's slicing from z_sample_points and z_cuts
   linear.npz     linear-map ring: ParticleGenerator bunch (N = 1000,
                  sigma_z = 0.23, sigma_dp = 1e-3 as stated), Synchrotron
                  one-turn map with LinearMap (Q_s = 0.017), 10 turns; per
@@ -51,6 +53,9 @@ Fixtures (all float64 unless noted; seeds fixed; 'gamma' is the bunch's):
                  separatrix-cut Gaussian bunch of 1e5 macroparticles:
                  checksums and the first 64 particles of the initial bunch,
                  bunch statistics after every turn, fitted growth rates
+  machines.npz   Synchrotron element order and Q_s for linear (given Q_s
+                 or from the RF bucket) and non-linear longitudinal modes,
+                 1 / 4 / 5 segments, RF_at 'middle' / 'end_of_transverse'
   multiturn.npz  multi-turn wake memory at small size: linear longitudinal map,
                  WakeField(UniformBinSlicer(20, n_sigma_z=3),
                  CircularResonator(1e6, 1e9, 1, n_turns_wake=4),
@@ -133,6 +138,21 @@ def gen_stats(cp):
                                   beam.sigma_x(), beam.sigma_y(), beam.sigma_z(),
                                   beam.sigma_dp(), beam.epsn_x(), beam.epsn_y(),
                                   beam.epsn_z()])
+    # UniformBinSlicer(z_sample_points=, z_cuts=): n_slices and cuts
+    zsp = []
+    for n_pts, dz, a, cuts in ((10, 0.1, -0.45, None), (10, 0.1, -0.45, (-1.0, 1.0)),
+                               (20, 0.05, -0.5, (-0.2, 0.3)), (7, 0.3, 0.0, (-0.7, 0.05)),
+                               (15, 0.02, 0.1, (0.3, 2.0)), (5, 1.0, -2.0, (-2.5, 2.5))):
+        pts = a + dz * np.arange(n_pts)
+        sl = UniformBinSlicer(0, z_sample_points=pts, z_cuts=cuts)
+        zsp.append([n_pts, dz, a, np.nan if cuts is None else cuts[0],
+                    np.nan if cuts is None else cuts[1], sl.n_slices, sl.z_cuts[0],
+                    sl.z_cuts[1]])
+    out['z_sample_points'] = np.array(zsp)
+    out['beam_twiss'] = np.array([beam.alpha_Twiss_x(), beam.beta_Twiss_x(),
+                                  beam.gamma_Twiss_x(), beam.alpha_Twiss_y(),
+                                  beam.beta_Twiss_y(), beam.gamma_Twiss_y(),
+                                  beam.dispersion_x(), beam.dispersion_y()])
     ss = beam.get_slices(UniformBinSlicer(10, n_sigma_z=2),
                          statistics=['mean_x', 'mean_y', 'sigma_x', 'sigma_dp',
                                      'epsn_x', 'epsn_z', 'eff_epsn_x'])
@@ -430,8 +450,32 @@ def gen_growth():
     return out
 
 
+MACHINE_CASES = [(mode, n_seg, at) for mode in ('linear Q_s', 'non-linear', 'linear bucket')
+                 for n_seg in (1, 4, 5) for at in ('middle', 'end_of_transverse')]
+
+
+def gen_machines():
+    """Synchrotron one-turn-map element order and Q_s for each longitudinal
+    mode, segment count and RF_at placement."""
+    from scipy.constants import c, e, m_p
+    from PyHEADTAIL.machines.synchrotron import Synchrotron
+    rf = dict(h_RF=[4620, 4 * 4620], V_RF=[4.5e6, 0.45e6], dphi_RF=[0., np.pi], p_increment=0.)
+    orders, qs = [], []
+    for mode, n_seg, at in MACHINE_CASES:
+        kw = {'linear Q_s': dict(Q_s=0.017), 'non-linear': rf,
+              'linear bucket': dict(h_RF=4620, V_RF=4.5e6, dphi_RF=0., p_increment=0.)}[mode]
+        m = Synchrotron('smooth', charge=e, mass=m_p, p0=26e9 * e / c, circumference=6911.5,
+                        n_segments=n_seg, beta_x=54., beta_y=54., D_x=0., D_y=0.,
+                        accQ_x=20.13, accQ_y=20.18, alpha_mom_compaction=3e-3,
+                        longitudinal_mode=mode.split()[0], RF_at=at, **kw)
+        orders.append(','.join(type(el).__name__ for el in m.one_turn_map))
+        qs.append(m.Q_s)
+    return dict(cases=np.array(['%s|%d|%s' % cse for cse in MACHINE_CASES]),
+                orders=np.array(orders), Q_s=np.array(qs))
+
+
 GENERATORS = ('stats', 'linear', 'nonlinear', 'multiturn', 'elements', 'extra_wakes',
-              'variants', 'growth')
+              'variants', 'growth', 'machines')
 
 
 def main():
@@ -441,7 +485,7 @@ def main():
     cp = load_reference(ref)
     fns = dict(stats=lambda: gen_stats(cp), linear=gen_linear, nonlinear=gen_nonlinear,
                multiturn=gen_multiturn, elements=gen_elements, extra_wakes=gen_extra_wakes,
-               variants=gen_variants, growth=gen_growth)
+               variants=gen_variants, growth=gen_growth, machines=gen_machines)
     for name in only:
         out = fns[name]()
         path = os.path.join(HERE, name + '.npz')

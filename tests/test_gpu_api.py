@@ -1,0 +1,270 @@
+"""The drop-in API on the device, checked the way the reference's own unit
+tests check it (tests/test_particles.py, test_slicing.py,
+test_generators.py, test_longitudinal_tracking.py, test_transverse_tracking.py
+of the reference), plus the SliceSet data the wake path does not use
+(particle_indices_by_slice, slice_positions, convert_to_particles), pinned
+by brute force on the host, and the statistical Twiss parameters and
+dispersions, pinned by the values the reference computed
+(tests/golden/stats.npz)."""
+import os
+
+import numpy as np
+import pytest
+import torch
+from scipy.constants import c, e, m_p
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), 'golden')
+COORDS = ('x', 'xp', 'y', 'yp', 'z', 'dp')
+
+
+def _case(name):
+    print("\n[case] %s" % name, flush=True)
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def uniform_bunch(n=2048, transverse_only=False, seed=3):
+    from pyheadtail_amd.particles.particles import Particles
+    rng = np.random.RandomState(seed)
+    coords = {'x': rng.uniform(-1, 1, n), 'y': rng.uniform(-1, 1, n),
+              'xp': rng.uniform(-.5, .5, n), 'yp': rng.uniform(-.5, .5, n)}
+    if not transverse_only:
+        coords.update(z=rng.uniform(-1, 1, n), dp=rng.uniform(-.5, .5, n))
+    return Particles(n, 1.234e9 / n, e, m_p, 111., 20.1, coords)
+
+
+def test_particles_contract():
+    _case("Particles: init, copies, update/add errors, setters, id, sort, merge")
+    b = uniform_bunch()
+    assert (b.macroparticlenumber, b.gamma, b.circumference) == (2048, 20.1, 111.)
+    assert np.isclose(b.intensity, 1.234e9)
+    d = b.get_coords_n_momenta_dict()
+    assert d is not b.coords_n_momenta
+    bad = dict(d, x=np.zeros(2049))
+    with pytest.raises(Exception):
+        b.update(bad)
+    b.add({'a': np.zeros(2048), 'ap': np.ones(2048)})
+    assert len(b.coords_n_momenta) == 8
+    with pytest.raises(Exception):
+        b.add({'b': np.zeros(2048), 'y': np.ones(2048)})
+    # update copies: changing the source array leaves the bunch alone
+    src = np.linspace(0, 1, 2048)
+    b.update({'x': src})
+    src[:] = 7.
+    assert host(b.x)[-1] == 1.
+    # kinematic setters re-derive each other
+    for prop in ('gamma', 'beta', 'betagamma', 'p0', 'intensity', 'charge_per_mp'):
+        new = 0.9 * getattr(b, prop)
+        setattr(b, prop, new)
+        assert np.isclose(getattr(b, prop), new), prop
+    assert np.array_equal(host(b.id), np.arange(1, 2049))
+    old = {k: host(getattr(b, k)).copy() for k in ('id',) + COORDS}
+    b.sort_for('z')
+    perm = host(b.id) - 1
+    for k, arr in old.items():
+        assert np.array_equal(arr[perm], host(getattr(b, k))), k
+    assert np.all(np.diff(host(b.z)) >= 0)
+    b2 = uniform_bunch(100)
+    merged = b2 + uniform_bunch(50, seed=4)
+    assert merged.macroparticlenumber == 150 and host(merged.x).shape == (150,)
+    assert np.array_equal(host((0 + b2).x), host(b2.x))
+
+
+def test_get_slices_cache_and_clean():
+    _case("get_slices caching, statistics re-slicing, clean_slices")
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    b = uniform_bunch()
+    sl = UniformBinSlicer(2, 0.1)
+    ss = b.get_slices(sl)
+    assert b.get_slices(sl) is ss
+    assert b.get_slices(UniformBinSlicer(2, 0.1)) is ss
+    assert b.get_slices(sl, statistics=['mean_x']) is not ss
+    b.clean_slices()
+    assert len(b._slice_sets) == 0
+
+
+def test_statistics_against_numpy_and_twiss_against_reference():
+    _case("bunch statistics vs numpy; Twiss parameters / dispersion vs reference")
+    from pyheadtail_amd.particles.particles import Particles
+    b = uniform_bunch()
+    assert abs(b.mean_xp() - np.mean(host(b.xp))) <= 1e-15
+    z = host(b.z)
+    assert abs(b.sigma_z() - np.std(z, ddof=1)) <= 1e-14 * np.std(z)
+    t = uniform_bunch(transverse_only=True)
+    for f in ('alpha_Twiss_x', 'alpha_Twiss_y', 'beta_Twiss_x', 'beta_Twiss_y',
+              'gamma_Twiss_x', 'gamma_Twiss_y', 'epsn_x', 'epsn_y'):
+        assert np.isfinite(getattr(t, f)())
+    with pytest.raises(AttributeError):
+        t.dispersion_x()
+    with pytest.raises(AttributeError):
+        t.dp
+    assert abs(t.epsn_x() - t.effective_normalized_emittance_x()) <= 1e-12 * t.epsn_x()
+    g = np.load(os.path.join(GOLD, 'stats.npz'))
+    beam = Particles(400, 1e11 / 400, e, m_p, 6911.5, g['gamma'][0],
+                     {k: g['beam'][i] for i, k in enumerate(COORDS)})
+    got = np.array([beam.alpha_Twiss_x(), beam.beta_Twiss_x(), beam.gamma_Twiss_x(),
+                    beam.alpha_Twiss_y(), beam.beta_Twiss_y(), beam.gamma_Twiss_y(),
+                    beam.dispersion_x(), beam.dispersion_y()])
+    err = np.max(np.abs(got - g['beam_twiss']) / np.abs(g['beam_twiss']))
+    print("[err] Twiss / dispersion %.3e relative" % err, flush=True)
+    assert err <= 1e-12
+
+
+def test_sliceset_derived_data():
+    _case("SliceSet: indices by slice, positions, convert_to_particles, centres, widths")
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    b = uniform_bunch(5000)
+    ss = b.get_slices(UniformBinSlicer(7, z_cuts=(-0.8, 0.6)))
+    idx = host(ss.slice_index_of_particle)
+    S = ss.n_slices
+    assert S == 7
+    inside = np.where((idx >= 0) & (idx < S))[0]
+    assert np.array_equal(host(ss.particles_within_cuts), inside)
+    assert np.array_equal(host(ss.particles_outside_cuts),
+                          np.where(~((idx >= 0) & (idx < S)))[0])
+    counts = host(ss.n_macroparticles_per_slice)
+    assert counts.sum() + len(host(ss.particles_outside_cuts)) == 5000
+    pos = host(ss.slice_positions)
+    assert pos[0] == 0 and np.array_equal(np.diff(pos), counts)
+    by_slice = host(ss.particle_indices_by_slice)
+    expect = np.concatenate([np.where(idx == s)[0] for s in range(S)])
+    assert np.array_equal(by_slice, expect)
+    for s in (0, 3, S - 1):
+        assert np.array_equal(host(ss.particle_indices_of_slice(s)), np.where(idx == s)[0])
+    vals = np.arange(S, dtype=np.float64) + 0.5
+    per_particle = host(ss.convert_to_particles(vals))
+    ref = np.zeros(5000)
+    ref[inside] = vals[idx[inside]]
+    assert np.array_equal(per_particle, ref)
+    zb = host(ss.z_bins)
+    assert np.array_equal(host(ss.z_centers), zb[:-1] + .5 * np.diff(zb))
+    assert np.array_equal(host(ss.slice_widths), np.diff(zb))
+    np.testing.assert_allclose(host(ss.times), host(ss.z_centers) / (b.beta * c), rtol=1e-15)
+    np.testing.assert_allclose(host(ss.charge_per_slice), counts * b.charge_per_mp, rtol=1e-15)
+    assert ss.z_cut_tail == -0.8 and ss.z_cut_head == zb[-1]
+
+
+def test_slicer_statistics_emittance_without_dispersion():
+    """tests/test_slicing.py:112-139 of the reference: with no dispersion the
+    per-slice emittance equals the effective one; all statistics exist."""
+    _case("UniformBinSlicer add_statistics, epsn == eff_epsn without dispersion")
+    from pyheadtail_amd.particles.generators import generate_Gaussian6DTwiss
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.trackers.longitudinal_tracking import LinearMap
+    np.random.seed(0)
+    lm = LinearMap([0.00308], 6911., 0.017)
+    gamma = 20.
+    p0 = np.sqrt(gamma**2 - 1) * m_p * c
+    beta_z = np.abs(lm.eta(dp=0, gamma=gamma) * lm.circumference / (2 * np.pi * lm.Q_s))
+    epsn_z = 4 * np.pi * 0.23**2 * p0 / (beta_z * e)
+    bunch = generate_Gaussian6DTwiss(10000, 1.05e11, e, m_p, lm.circumference, gamma, 0., 0.,
+                                     42., 42., beta_z, 3.75e-6, 3.75e-6, epsn_z)
+    sl = UniformBinSlicer(5, z_cuts=(-20., 30.))
+    ss = sl.slice(bunch)
+    sl.add_statistics(ss, bunch, True)
+    # without dispersion the correction is the sample correlation of x and
+    # dp, of order 1/n per slice (the reference checks 4 decimal places of
+    # values ~4e-6)
+    np.testing.assert_allclose(host(ss.epsn_x), host(ss.eff_epsn_x), rtol=1e-2)
+    for name in ('mean_x', 'sigma_y', 'epsn_z', 'eff_epsn_y'):
+        assert host(getattr(ss, name)).shape == (5,)
+
+
+def test_generators_contract():
+    """tests/test_generators.py of the reference."""
+    _case("ParticleGenerator: lengths, missing planes, update, distributions, bucket cut")
+    from pyheadtail_amd.general.printers import SilentPrinter
+    from pyheadtail_amd.particles import generators as gf
+    from pyheadtail_amd.trackers.longitudinal_tracking import RFSystems
+    np.random.seed(0)
+    gen = gf.ParticleGenerator(1000, 1e11, e, m_p, 99, 27.1, distribution_x=gf.gaussian2D(0.5),
+                               alpha_x=-0.7, beta_x=4, D_x=0,
+                               distribution_z=gf.gaussian2D(3.0), printer=SilentPrinter())
+    beam = gen.generate()
+    assert beam.x.numel() == 1000
+    with pytest.raises(AttributeError):
+        beam.yp
+    gen.update(beam)
+    assert beam.x.numel() == 1000
+    x_before = host(beam.x).copy()
+    gf.ParticleGenerator(1000, 1e11, e, m_p, 99, 27.1,
+                         distribution_y=gf.gaussian2D(3.0)).update(beam)
+    assert beam.yp.numel() == 1000 and np.array_equal(host(beam.x), x_before)
+    for dist in (gf.gaussian2D(0.1), gf.uniform2D(-2., 3.)):
+        u, up = dist(100)
+        assert u.size == 100 and up.size == 100
+    coords = [np.linspace(-2, 2, 5), np.linspace(-3, 3, 5)]
+    b5 = gf.ParticleGenerator(5, 1e11, e, m_p, 100, 10,
+                              distribution_y=gf.import_distribution2D(coords)).generate()
+    assert b5.y.numel() == 5 and b5.yp.numel() == 5
+    with pytest.raises(AssertionError):
+        gf.ParticleGenerator(10, 1e11, e, m_p, 100, 10, distribution_y=gf.import_distribution2D(
+            [np.linspace(-2, 2, 11), np.linspace(-3, 3, 11)])).generate()
+    rf = RFSystems(99, [4620, 4 * 4620], [10e6, 1e6], [0, 0], [0.00308], 27.1, 0, charge=e,
+                   mass=m_p)
+    accept = rf.get_bucket(gamma=27.1).make_is_accepted(margin=0.)
+    cut = gf.ParticleGenerator(100, 11, e, m_p, 99, 27.1, distribution_z=gf.cut_distribution(
+        is_accepted=accept, distribution=gf.gaussian2D(0.01))).generate()
+    assert cut.z.numel() == 100 and np.sum(accept(host(cut.z), host(cut.dp))) == 100
+
+
+def all_ones(n=5):
+    from pyheadtail_amd.particles.particles import Particles
+    return Particles(n, 1, e, 1, 1, 18., {k: np.ones(n) for k in COORDS})
+
+
+def test_longitudinal_elements_touch_only_z_dp_and_clean_slices():
+    """tests/test_longitudinal_tracking.py:30-110 of the reference."""
+    _case("Drift / LinearMap change only z, dp and drop the slice sets")
+    from pyheadtail_amd.general.printers import AccumulatorPrinter
+    from pyheadtail_amd.trackers.longitudinal_tracking import Drift, LinearMap
+    for el in (Drift([10], 0.1, warningprinter=AccumulatorPrinter()),
+               LinearMap([10], 1., 0.012, printer=AccumulatorPrinter())):
+        beam, ref = all_ones(), all_ones()
+        beam._slice_sets = {'mock': 42}
+        el.track(beam)
+        assert not beam._slice_sets
+        for k in ('x', 'xp', 'y', 'yp'):
+            assert torch.equal(getattr(beam, k), getattr(ref, k)), k
+        if isinstance(el, Drift):
+            assert torch.equal(beam.dp, ref.dp)
+        assert not torch.equal(beam.z, ref.z)
+
+
+def test_transverse_map_with_and_without_detuners_runs():
+    """tests/test_transverse_tracking.py of the reference (10 segments with
+    alpha != 0 and dispersion); in addition each segment preserves the
+    Courant-Snyder action without detuning when D = 0."""
+    _case("TransverseMap over 10 segments, with and without amplitude detuning")
+    from pyheadtail_amd.general.printers import SilentPrinter
+    from pyheadtail_amd.particles.particles import Particles
+    from pyheadtail_amd.trackers.detuners import AmplitudeDetuning
+    from pyheadtail_amd.trackers.transverse_tracking import TransverseMap
+    s = np.linspace(0., 17.1, 11)
+    a = np.linspace(-1.51, 1.52, 10)
+    bt = np.linspace(0.1, 1., 10)
+    np.random.seed(0)
+    n = 1000
+    x, y = np.random.uniform(-.1, .1, n), np.random.uniform(-.1, .1, n)
+    xp, yp = np.random.uniform(0., 1., n), np.random.uniform(0., 1., n)
+    for D, det in ((100 * np.ones(10), []), (100 * np.ones(10), [AmplitudeDetuning(1e-2, 5e-2,
+                                                                                    1e-3)]),
+                   (np.zeros(10), [])):
+        tm = TransverseMap(s, a, bt, D, a.copy(), bt.copy(), D.copy(), 17.89, 19.11, det,
+                           printer=SilentPrinter())
+        beam = Particles(n, 1, e, 1, 1, 18., dict(x=x.copy(), xp=xp.copy(), y=y.copy(),
+                                                  yp=yp.copy(), z=x.copy(), dp=x.copy()))
+        for seg in tm:
+            seg.track(beam)
+        X, XP = host(beam.x), host(beam.xp)
+        assert np.all(np.isfinite(X)) and np.all(np.isfinite(XP))
+        if not np.any(D) and not det:
+            # one turn closes on the optics of boundary 0
+            g0 = (1 + a[0]**2) / bt[0]
+            J0 = g0 * x**2 + 2 * a[0] * x * xp + bt[0] * xp**2
+            J1 = g0 * X**2 + 2 * a[0] * X * XP + bt[0] * XP**2
+            assert np.max(np.abs(J1 - J0) / J0) <= 1e-12
