@@ -46,6 +46,10 @@ enum {
 #define PHX_MAX_TERMS 8          /* per-slice accumulated terms in one pass */
 #define PHX_MAX_PROGRAM 16       /* local elements fused into one particle pass */
 #define PHX_MAX_STATS 16         /* per-turn bunch statistics recorded by a pass */
+#define PHX_STANDALONE_GRID 768  /* blocks of the reducing particle passes of the
+                                    standalone calls (phx_bunch_sums,
+                                    phx_slice_cuts): their fold order */
+#define PHX_MAX_GRID 2048        /* largest engine pass grid */
 
 typedef void *phx_stream_t;
 
@@ -374,6 +378,21 @@ enum {
     PHX_STAT_EPSN_Z, PHX_STAT_COUNT
 };
 
+/* Element orders of the turn programs that have a pass compiled for them
+   (any other order: PHX_SHAPE_GENERIC, which runs every program).  RF1 / RF2
+   is a longitudinal sequence [drift, 1 or 2 kicks, drift] (RFSystems with
+   one or two harmonics), LIN a linear map, T one transverse segment. */
+enum {
+    PHX_SHAPE_GENERIC = 0,
+    PHX_SHAPE_RF1_T = 1, PHX_SHAPE_RF2_T = 2, PHX_SHAPE_T_RF1 = 3, PHX_SHAPE_T_RF2 = 4,
+    PHX_SHAPE_LIN_T = 5, PHX_SHAPE_T_LIN = 6, PHX_SHAPE_T = 7,
+    PHX_SHAPE_COUNT = 8
+};
+/* Wake kick sets with a compiled kick step (other sets: PHX_KICKS_GENERIC):
+   DXDY is [dipole x (xp += k), dipole y (yp += k)], as a round chamber's
+   resonator or resistive wall gives. */
+enum { PHX_KICKS_GENERIC = 0, PHX_KICKS_DXDY = 1, PHX_KICKS_COUNT = 2 };
+
 /* Everything the fused engine needs; all pointers are device memory owned
    by the caller.  The engine handles one-turn maps of the form
    [local elements..., WakeField] (or local elements only: n_slices = 0). */
@@ -384,6 +403,16 @@ typedef struct {
     int32_t n_kicks;                    /* wake kicks of the wake field */
     int32_t hist_capacity;              /* deepest n_turns_wake (ring slots) */
     int32_t record_stats;               /* 1: fill stats[turn][33] */
+    int32_t program_shape;              /* PHX_SHAPE_*: every program's element
+                                           order (GENERIC is always valid) */
+    int32_t kick_shape;                 /* PHX_KICKS_*: the wake field's kicks */
+    int32_t pass_grid;                  /* blocks of the particle pass; 0: as
+                                           many as stay resident on the device.
+                                           The bunch sums are folded per block,
+                                           so the last bits of the cuts depend
+                                           on it: PHX_STANDALONE_GRID gives the
+                                           cuts of phx_slice_cuts bit for bit */
+    int32_t reserved;                   /* 0 */
     const phx_turn_program *programs;   /* [n_programs] */
     const int32_t *turn_program;        /* [n_turns_total] -> program index */
     double *stats;                      /* raw moment sums per turn, see below */

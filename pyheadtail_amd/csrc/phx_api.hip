@@ -8,6 +8,7 @@
 namespace {
 thread_local char g_err[512] = "";
 size_t g_max_lds = 65536;
+int g_num_cus = 256;
 }  // namespace
 
 namespace phx {
@@ -30,6 +31,7 @@ int fail(int code, const char *fmt, ...) {
 }  // namespace phx
 
 size_t phx_max_lds_bytes() { return g_max_lds; }
+int phx_num_cus() { return g_num_cus; }
 
 extern "C" const char *phx_last_error(void) { return g_err; }
 
@@ -47,6 +49,10 @@ extern "C" int phx_init(int device) {
   PHX_CHECK_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device),
                 "phx_init");
   if (lds > 0) g_max_lds = (size_t)lds;
+  int cus = 0;
+  PHX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device),
+                "phx_init");
+  if (cus > 0) g_num_cus = cus;
   return PHX_OK;
 }
 

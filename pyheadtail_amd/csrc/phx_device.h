@@ -8,23 +8,29 @@
 #include <stdint.h>
 
 #include "../../include/phx.h"
+#include "phx_math.h"
 
 #define PHX_BLOCK 256
 #define PHX_WAVE 64
 #define PHX_PPT 2
 #define PHX_CHUNK (PHX_BLOCK * PHX_PPT)
-// Reducing particle passes run a persistent grid of at most PHX_GRID blocks:
-// block b handles the particle groups b, b + G, b + 2G, ... of PHX_CHUNK
-// particles (thread t: particles g*PHX_CHUNK + k*PHX_BLOCK + t, k < PHX_PPT),
-// so block b's partial covers a fixed set of particles in a fixed order.
-#define PHX_GRID 512
+// Reducing particle passes run a persistent grid of G blocks: block b
+// handles the particle groups b, b + G, b + 2G, ... of PHX_CHUNK particles
+// (thread t: particles g*PHX_CHUNK + k*PHX_BLOCK + t, k < PHX_PPT), so block
+// b's partial covers a fixed set of particles in a fixed order for a given G.
+// The standalone reductions use G = PHX_STANDALONE_GRID (three blocks on
+// each of MI355X's 256 CUs); the engine's pass uses the grid that stays
+// resident for its kernel, at most PHX_MAX_GRID.
 
 __host__ __device__ inline int64_t phx_n_groups(int64_t n) {
   return n <= 0 ? 0 : (n + PHX_CHUNK - 1) / PHX_CHUNK;
 }
-__host__ __device__ inline int phx_pass_grid(int64_t n) {
+__host__ __device__ inline int phx_grid_for(int64_t n, int64_t grid) {
   const int64_t g = phx_n_groups(n);
-  return (int)(g < PHX_GRID ? g : PHX_GRID);
+  return (int)(g < grid ? g : grid);
+}
+__host__ __device__ inline int phx_pass_grid(int64_t n) {
+  return phx_grid_for(n, PHX_STANDALONE_GRID);
 }
 // NumPy's 2.*np.pi as a double
 #define PHX_TWO_PI 6.283185307179586
@@ -137,8 +143,8 @@ __device__ __forceinline__ void apply_tseg_n(P6 (&p)[N], const phx_tseg &t) {
       const double ax = dphi_x[k] * PHX_TWO_PI;
       const double ay = dphi_y[k] * PHX_TWO_PI;
       double sx, cx, sy, cy;
-      sincos(ax, &sx, &cx);
-      sincos(ay, &sy, &cy);
+      phx_sincos(ax, &sx, &cx);
+      phx_sincos(ay, &sy, &cy);
       M00[k] = I00 * cx + J00 * sx;
       M01[k] = I01 * cx + J01 * sx;
       M10[k] = I10 * cx + J10 * sx;
@@ -174,38 +180,45 @@ __device__ __forceinline__ void apply_tseg_n(P6 (&p)[N], const phx_tseg &t) {
 // Longitudinal maps (trackers/longitudinal_tracking.py).
 // ---------------------------------------------------------------------------
 template <int N>
-__device__ __forceinline__ void apply_long_seq_n(P6 (&p)[N], const phx_long_seq &s) {
-  for (int i = 0; i < s.n_ops; ++i) {
-    const phx_long_op &o = s.op[i];
-    if (o.type == PHX_LONG_DRIFT) {
-      // Drift.track: z = beta_ratio*z - eta(dp)*dp*length   (:107-111)
-      const double br = o.beta_ratio, eta0 = o.eta0, len = o.length;
+__device__ __forceinline__ void drift_n(P6 (&p)[N], const phx_long_op &o) {
+  // Drift.track: z = beta_ratio*z - eta(dp)*dp*length   (:107-111)
+  const double br = o.beta_ratio, eta0 = o.eta0, len = o.length;
 #pragma unroll
-      for (int k = 0; k < N; ++k) p[k].z = br * p[k].z - (eta0 * p[k].dp) * len;
-    } else {
-      // Kick.track_with_dispersion / track_without_dispersion (:185-203)
-      const bool disp = o.with_dispersion != 0;
-      const double Dx = o.D_x, Dy = o.D_y, h = o.harmonic, tp = o.two_pi, C = o.circumference;
-      const double off = o.phi_offset, lock = o.phi_lock, p0b = o.p0_before, p0a = o.p0_after;
-      const double amp = o.amplitude, pinc = o.p_increment;
+  for (int k = 0; k < N; ++k) p[k].z = br * p[k].z - (eta0 * p[k].dp) * len;
+}
+
+template <int N>
+__device__ __forceinline__ void rf_kick_n(P6 (&p)[N], const phx_long_op &o) {
+  // Kick.track_with_dispersion / track_without_dispersion (:185-203)
+  const bool disp = o.with_dispersion != 0;
+  const double Dx = o.D_x, Dy = o.D_y, h = o.harmonic, tp = o.two_pi, C = o.circumference;
+  const double off = o.phi_offset, lock = o.phi_lock, p0b = o.p0_before, p0a = o.p0_after;
+  const double amp = o.amplitude, pinc = o.p_increment;
 #pragma unroll
-      for (int k = 0; k < N; ++k) {
-        P6 &q = p[k];
-        if (disp) {
-          q.x = q.x - Dx * q.dp;
-          q.y = q.y - Dy * q.dp;
-        }
-        const double phi = (h * ((tp * q.z) / C) + off) + lock;
-        double delta_p = q.dp * p0b;
-        delta_p = delta_p + (amp * sin(phi) - pinc);
-        q.dp = delta_p / p0a;
-        if (disp) {
-          q.x = q.x + Dx * q.dp;
-          q.y = q.y + Dy * q.dp;
-        }
-      }
+  for (int k = 0; k < N; ++k) {
+    P6 &q = p[k];
+    if (disp) {
+      q.x = q.x - Dx * q.dp;
+      q.y = q.y - Dy * q.dp;
+    }
+    const double phi = (h * ((tp * q.z) / C) + off) + lock;
+    double delta_p = q.dp * p0b;
+    delta_p = delta_p + (amp * phx_sin(phi) - pinc);
+    q.dp = delta_p / p0a;
+    if (disp) {
+      q.x = q.x + Dx * q.dp;
+      q.y = q.y + Dy * q.dp;
     }
   }
+}
+
+// A sequence is RFSystems' [drift, kicks..., drift] (:362-369) or a single
+// Drift or Kick; the leading and trailing drifts are applied outside the
+// loop over the inner operations, which keeps the kick's registers apart
+// from the drift's.
+template <int N>
+__device__ __forceinline__ void shrink_n(P6 (&p)[N], const phx_long_seq &s) {
+  // RFSystems._shrink_transverse_emittance (:556-565)
   if (s.shrink) {
     const double f = s.shrink_factor;
 #pragma unroll
@@ -216,6 +229,32 @@ __device__ __forceinline__ void apply_long_seq_n(P6 (&p)[N], const phx_long_seq 
       p[k].yp = p[k].yp * f;
     }
   }
+}
+
+template <int N>
+__device__ __forceinline__ void apply_long_seq_n(P6 (&p)[N], const phx_long_seq &s) {
+  const int n_ops = s.n_ops;
+  const bool lead = n_ops > 0 && s.op[0].type == PHX_LONG_DRIFT;
+  const bool trail = n_ops > 1 && s.op[n_ops - 1].type == PHX_LONG_DRIFT;
+  if (lead) drift_n<N>(p, s.op[0]);
+  const int i1 = trail ? n_ops - 1 : n_ops;
+  for (int i = lead ? 1 : 0; i < i1; ++i) {
+    if (s.op[i].type == PHX_LONG_DRIFT) drift_n<N>(p, s.op[i]);
+    else rf_kick_n<N>(p, s.op[i]);
+  }
+  if (trail) drift_n<N>(p, s.op[n_ops - 1]);
+  shrink_n<N>(p, s);
+}
+
+// RFSystems' sequence with NH harmonics: op[0] drift, op[1..NH] kicks,
+// op[NH+1] drift.
+template <int NH, int N>
+__device__ __forceinline__ void apply_rf_seq_n(P6 (&p)[N], const phx_long_seq &s) {
+  drift_n<N>(p, s.op[0]);
+#pragma unroll
+  for (int i = 1; i <= NH; ++i) rf_kick_n<N>(p, s.op[i]);
+  drift_n<N>(p, s.op[NH + 1]);
+  shrink_n<N>(p, s);
 }
 
 template <int N>
@@ -247,6 +286,36 @@ __device__ __forceinline__ void apply_program_n(P6 (&p)[N], const phx_turn_progr
   }
 }
 
+// A program of a known element order (PHX_SHAPE_*): the elements are
+// applied in straight-line code, which the compiler schedules and allocates
+// registers for far better than the interpreter's loop over element kinds.
+template <int SHAPE, int N>
+__device__ __forceinline__ void apply_shape_n(P6 (&p)[N], const phx_turn_program &g) {
+  if constexpr (SHAPE == PHX_SHAPE_RF1_T) {
+    apply_rf_seq_n<1, N>(p, g.lseq[0]);
+    apply_tseg_n<N>(p, g.tseg[0]);
+  } else if constexpr (SHAPE == PHX_SHAPE_RF2_T) {
+    apply_rf_seq_n<2, N>(p, g.lseq[0]);
+    apply_tseg_n<N>(p, g.tseg[0]);
+  } else if constexpr (SHAPE == PHX_SHAPE_T_RF1) {
+    apply_tseg_n<N>(p, g.tseg[0]);
+    apply_rf_seq_n<1, N>(p, g.lseq[0]);
+  } else if constexpr (SHAPE == PHX_SHAPE_T_RF2) {
+    apply_tseg_n<N>(p, g.tseg[0]);
+    apply_rf_seq_n<2, N>(p, g.lseq[0]);
+  } else if constexpr (SHAPE == PHX_SHAPE_LIN_T) {
+    apply_linmap_n<N>(p, g.lin[0]);
+    apply_tseg_n<N>(p, g.tseg[0]);
+  } else if constexpr (SHAPE == PHX_SHAPE_T_LIN) {
+    apply_tseg_n<N>(p, g.tseg[0]);
+    apply_linmap_n<N>(p, g.lin[0]);
+  } else if constexpr (SHAPE == PHX_SHAPE_T) {
+    apply_tseg_n<N>(p, g.tseg[0]);
+  } else {
+    apply_program_n<N>(p, g);
+  }
+}
+
 __device__ __forceinline__ P6 (&one(P6 &p))[1] { return *reinterpret_cast<P6(*)[1]>(&p); }
 __device__ __forceinline__ void apply_tseg(P6 &p, const phx_tseg &t) { apply_tseg_n<1>(one(p), t); }
 __device__ __forceinline__ void apply_long_seq(P6 &p, const phx_long_seq &s) {
@@ -262,32 +331,34 @@ __device__ __forceinline__ void apply_program(P6 &p, const phx_turn_program &g) 
 // ---------------------------------------------------------------------------
 // Wake kick application (impedances/wake_kicks.py:160-361), kicks in order.
 // ---------------------------------------------------------------------------
-// Kick k of slice idx is kicks[k * ks + idx * ss] (ks = n_slices, ss = 1 for
-// the [kick][slice] layout of the C ABI; ks = 1, ss = n_kicks for the
-// slice-major copy the engine keeps in LDS).
+// Kick k of slice idx is kicks[k * n_slices + idx] (the [kick][slice]
+// layout of the C ABI); the table is small and stays in the caches.
 __device__ __forceinline__ void apply_wake_kicks(P6 &p, int32_t idx, int n_slices,
                                                  const phx_wake_kick *kd, int n_kicks,
-                                                 const double *__restrict__ kicks, int ks,
-                                                 int ss) {
+                                                 const double *__restrict__ kicks) {
   if (idx < 0 || idx >= n_slices) return;
-  // kick values gathered four at a time (loads in flight together, few
-  // registers held), then applied in list order
-  for (int k0 = 0; k0 < n_kicks; k0 += 4) {
-    double kv[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-      if (k0 + q < n_kicks) kv[q] = kicks[(size_t)(k0 + q) * ks + (size_t)idx * ss];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (k0 + q >= n_kicks) break;
-      const phx_wake_kick &d = kd[k0 + q];
-      double v = kv[q];
-      if (d.multiply == 0) v = v * p.x;
-      else if (d.multiply == 2) v = v * p.y;
-      if (d.plane == PHX_PLANE_XP) p.xp = p.xp + v;
-      else if (d.plane == PHX_PLANE_YP) p.yp = p.yp + v;
-      else p.dp = p.dp + v;
-    }
+  for (int k = 0; k < n_kicks; ++k) {
+    const phx_wake_kick &d = kd[k];
+    double v = kicks[(size_t)k * n_slices + idx];
+    if (d.multiply == 0) v = v * p.x;
+    else if (d.multiply == 2) v = v * p.y;
+    if (d.plane == PHX_PLANE_XP) p.xp = p.xp + v;
+    else if (d.plane == PHX_PLANE_YP) p.yp = p.yp + v;
+    else p.dp = p.dp + v;
+  }
+}
+
+// The kick step for a known kick set (PHX_KICKS_*).
+template <int KS>
+__device__ __forceinline__ void apply_kick_set(P6 &p, int32_t idx, int n_slices,
+                                               const phx_wake_kick *kd, int n_kicks,
+                                               const double *__restrict__ kicks) {
+  if constexpr (KS == PHX_KICKS_DXDY) {
+    if (idx < 0 || idx >= n_slices) return;
+    p.xp = p.xp + kicks[idx];
+    p.yp = p.yp + kicks[(size_t)n_slices + idx];
+  } else {
+    apply_wake_kicks(p, idx, n_slices, kd, n_kicks, kicks);
   }
 }
 

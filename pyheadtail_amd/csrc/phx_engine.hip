@@ -47,10 +47,12 @@ __host__ __device__ inline int hist_blocks(int64_t n) {
   return (int)(nc < EN_HIST_BLOCKS ? nc : EN_HIST_BLOCKS);
 }
 
+// The partials region holds the largest pass grid, so the layout does not
+// depend on the grid chosen.
 __host__ __device__ inline EngineWs engine_ws(void *ws, int64_t n, int S) {
   EngineWs w;
   char *p = reinterpret_cast<char *>(ws);
-  const int64_t nc = phx_pass_grid(n);
+  const int64_t nc = phx_grid_for(n, PHX_MAX_GRID);
   const int Se = S > 0 ? S : 1;
   w.partials = reinterpret_cast<double *>(p);
   p += ((size_t)nc * EN_NV * 8 + 255) / 256 * 256;
@@ -62,7 +64,7 @@ __host__ __device__ inline EngineWs engine_ws(void *ws, int64_t n, int S) {
 }
 
 size_t engine_ws_bytes(int64_t n, int S, int n_kicks, int T) {
-  const int64_t nc = phx_pass_grid(n);
+  const int64_t nc = phx_grid_for(n, PHX_MAX_GRID);
   const int Se = S > 0 ? S : 1;
   size_t b = round256((size_t)nc * EN_NV * 8) + acc_parts_bytes(hist_blocks(n), Se, 2);
   b += (size_t)(n_kicks > 0 ? n_kicks : 1) * T * Se * 8;
@@ -73,16 +75,8 @@ struct EngineArgs {
   phx_engine e;
   int n_kicks;
   int T;
-  int kick_lds;  // the pass stages the kick table in LDS (it fits)
+  int grid;  // blocks of the particle pass (its partials: [slot][grid])
 };
-
-// Largest kick table the pass stages in LDS, per wave (n_kicks * n_slices
-// doubles; four copies per block).
-#define EN_KICK_LDS_MAX (8 * 1024)
-
-inline size_t kick_lds_bytes(const phx_engine &e, int n_kicks) {
-  return (size_t)n_kicks * (e.n_slices > 0 ? e.n_slices : 0) * sizeof(double);
-}
 
 __device__ __forceinline__ P6 load_p(const phx_engine &e, int64_t j) {
   P6 p;
@@ -105,11 +99,12 @@ __device__ __forceinline__ void store_p(const phx_engine &e, int64_t j, const P6
 }
 
 // nk = min(program kicks, engine kicks): the kick table holds the latter.
+template <int KS>
 __device__ __forceinline__ void kick_pending(P6 &p, const phx_turn_program &g, int nk,
                                              double tail, double width, int S,
-                                             const double *kicks, int ks, int ss) {
+                                             const double *kicks) {
   const double f = slice_floor(p.z, tail, width);
-  if (f >= 0.0 && f < (double)S) apply_wake_kicks(p, (int32_t)f, S, g.kick, nk, kicks, ks, ss);
+  if (f >= 0.0 && f < (double)S) apply_kick_set<KS>(p, (int32_t)f, S, g.kick, nk, kicks);
 }
 
 // stats slots: [0..6) sum u, [6..12) sum (u-u0), [12..33) sum (u-u0)(v-v0)
@@ -141,16 +136,16 @@ struct PassLayout {
 };
 
 // MODE 0: a turn; MODE 1: flush (pending kick + final statistics only).
-// Persistent grid (phx_pass_grid blocks): each block walks its particle
+// Persistent grid (a.grid blocks): each block walks its particle
 // groups with the next group's coordinates loaded while the current one is
 // computed, so the particle traffic overlaps the FP64 work.
 // No workgroup barrier or fence precedes the last read of the turn program
 // and the coordinate arrays are declared non-aliasing (__restrict__
 // arguments): otherwise the compiler must assume the program may have been
 // written and reloads it with vector loads that wait on the particle loads.
-// So the kick table is staged per wave (a wave needs no barrier to read what
-// its own lanes wrote to LDS) and particle 0 is computed by every lane.  Dynamic LDS: max(per-wave kick tables, block reduction).
-template <int MODE, bool REC, bool WAKE>
+// So particle 0 is computed by every lane, and the kick table is read from
+// global memory (it stays in the caches).  Dynamic LDS: the block reduction.
+template <int MODE, bool REC, bool WAKE, int SHAPE, int KS>
 __global__ void __launch_bounds__(PHX_BLOCK)
     k_engine_pass(EngineArgs a, double *__restrict__ x, double *__restrict__ xp,
                   double *__restrict__ y, double *__restrict__ yp, double *__restrict__ z,
@@ -167,19 +162,7 @@ __global__ void __launch_bounds__(PHX_BLOCK)
   const int pend = *e.pending;
   const double tail = e.state->tail, width = e.state->width;
   const int S = e.n_slices;
-  // This wave's copy of the pending kick table, slice-major: each
-  // particle's kicks are adjacent and the gathers stay on-chip.
   const int nk = g.n_kicks < a.n_kicks ? g.n_kicks : a.n_kicks;
-  const bool klds = pend && a.kick_lds;
-  const int lane = threadIdx.x % PHX_WAVE;
-  double *ktab = pass_lds + (size_t)(threadIdx.x / PHX_WAVE) * nk * S;
-  if (klds) {
-    const int tot = nk * S;
-    for (int i = lane; i < tot; i += PHX_WAVE) {
-      const int k = i / S, s = i - k * S;
-      ktab[s * nk + k] = kicks[i];
-    }
-  }
   const int64_t ng = phx_n_groups(n);
   int64_t gi = blockIdx.x;
   P6 cur[PHX_PPT], nxt[PHX_PPT];
@@ -196,9 +179,9 @@ __global__ void __launch_bounds__(PHX_BLOCK)
   double src[7];
   {
     P6 q = P6{x[0], xp[0], y[0], yp[0], z[0], dp[0]};
-    if (pend) kick_pending(q, g, nk, tail, width, S, kicks, S, 1);
+    if (pend) kick_pending<KS>(q, g, nk, tail, width, S, kicks);
     src[0] = q.x; src[1] = q.xp; src[2] = q.y; src[3] = q.yp; src[4] = q.z; src[5] = q.dp;
-    if (MODE == 0) apply_program(q, g);
+    if (MODE == 0) apply_shape_n<SHAPE, 1>(one(q), g);
     src[6] = q.z;
   }
   double v[L::NV];
@@ -221,8 +204,7 @@ __global__ void __launch_bounds__(PHX_BLOCK)
     if (pend) {
 #pragma unroll
       for (int k = 0; k < PHX_PPT; ++k) {
-        if (klds) kick_pending(cur[k], g, nk, tail, width, S, ktab, 1, nk);
-        else kick_pending(cur[k], g, nk, tail, width, S, kicks, S, 1);
+        kick_pending<KS>(cur[k], g, nk, tail, width, S, kicks);
       }
     }
     if (REC) {
@@ -230,7 +212,7 @@ __global__ void __launch_bounds__(PHX_BLOCK)
       for (int k = 0; k < PHX_PPT; ++k)
         if (base + (int64_t)k * PHX_BLOCK < n) stats_accumulate(&v[L::SOFF], cur[k], shs);
     }
-    if (MODE == 0) apply_program_n<PHX_PPT>(cur, g);
+    if (MODE == 0) apply_shape_n<SHAPE, PHX_PPT>(cur, g);
     if (WAKE) {
 #pragma unroll
       for (int k = 0; k < PHX_PPT; ++k) {
@@ -257,7 +239,6 @@ __global__ void __launch_bounds__(PHX_BLOCK)
 #pragma unroll
     for (int k = 0; k < PHX_PPT; ++k) cur[k] = nxt[k];
   }
-  lds_sync();  // the reduction reuses the kick tables' LDS
   const double r = block_reduce_slots<L::NV, L::MINM, L::MAXM>(v, pass_lds);
   if (threadIdx.x < L::NV) w.partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = r;
 }
@@ -297,8 +278,7 @@ __global__ void __launch_bounds__(PHX_BLOCK) k_engine_fin(EngineArgs a) {
   __shared__ double res[L::NV];
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
   const int turn = *e.turn_counter;
-  const int64_t nc = phx_pass_grid(e.n);
-  fold_slots<0, L::NV, L::MINM, L::MAXM>(w.partials, nc, res, lds);
+  fold_slots<0, L::NV, L::MINM, L::MAXM>(w.partials, a.grid, res, lds);
   if (REC && threadIdx.x < EN_STAT_NV)
     e.stats[(size_t)turn * EN_STAT_NV + threadIdx.x] = res[L::SOFF + threadIdx.x];
   if (threadIdx.x == 0) {
@@ -326,7 +306,7 @@ __global__ void __launch_bounds__(EN_HIST_THREADS) k_engine_hist(EngineArgs a) {
   const int S = e.n_slices;
   const int turn = *e.turn_counter;
   const phx_turn_program &g = e.programs[e.turn_program[turn]];
-  const int64_t ncp = phx_pass_grid(e.n);
+  const int64_t ncp = a.grid;
   // every block needs the z block; only block 0 writes the statistics row
   fold_slots<0, ZB_NV, L::MINM, L::MAXM>(w.partials, ncp, res, lds);
   if constexpr (REC) {
@@ -482,6 +462,16 @@ int check_engine(const phx_engine *e, int *n_kicks, int *T) {
   if (e->n_slices > 0 && (e->n_kicks < 1 || e->hist_capacity < 1))
     return phx::fail(PHX_E_INVALID, "phx_engine: wake field needs kicks and history");
   if (e->record_stats && !e->stats) return phx::fail(PHX_E_INVALID, "phx_engine: stats null");
+  if (e->program_shape < 0 || e->program_shape >= PHX_SHAPE_COUNT)
+    return phx::fail(PHX_E_INVALID, "phx_engine: program_shape %d", e->program_shape);
+  if (e->kick_shape < 0 || e->kick_shape >= PHX_KICKS_COUNT)
+    return phx::fail(PHX_E_INVALID, "phx_engine: kick_shape %d", e->kick_shape);
+  if (e->pass_grid < 0 || e->pass_grid > PHX_MAX_GRID)
+    return phx::fail(PHX_E_INVALID, "phx_engine: pass_grid %d not in 0..%d", e->pass_grid,
+                     PHX_MAX_GRID);
+  if (e->kick_shape == PHX_KICKS_DXDY && e->n_kicks != 2)
+    return phx::fail(PHX_E_INVALID, "phx_engine: kick_shape DXDY needs 2 kicks, not %d",
+                     e->n_kicks);
   if (e->n_slices > 0 && (!e->hist || !e->z_bins || !e->kicks || !e->counts || !e->slice_means))
     return phx::fail(PHX_E_INVALID, "phx_engine: wake buffers null");
   *n_kicks = e->n_kicks;
@@ -489,41 +479,109 @@ int check_engine(const phx_engine *e, int *n_kicks, int *T) {
   return PHX_OK;
 }
 
+template <bool REC, bool WAKE>
+size_t pass_lds_bytes() {
+  constexpr int NV = PassLayout<REC, WAKE>::NV;
+  return BR_LDS(NV) * sizeof(double);
+}
+
+template <int MODE, bool R, bool W, int SH, int KS>
+void launch_pass_kernel(const EngineArgs &a, hipStream_t s) {
+  const phx_engine &e = a.e;
+  double *const *c = e.coords;
+  k_engine_pass<MODE, R, W, SH, KS><<<dim3((unsigned)a.grid), PHX_BLOCK, pass_lds_bytes<R, W>(), s>>>(
+      a, c[0], c[1], c[2], c[3], c[4], c[5], e.programs, e.kicks);
+}
+
+// Blocks of this pass that stay resident on one CU (queried once).
+template <int MODE, bool R, bool W, int SH, int KS>
+int pass_blocks_per_cu() {
+  static int cached = 0;
+  if (cached == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_engine_pass<MODE, R, W, SH, KS>,
+                                                     PHX_BLOCK, pass_lds_bytes<R, W>()) !=
+            hipSuccess ||
+        nb < 1)
+      nb = 1;
+    cached = nb;
+  }
+  return cached;
+}
+
+struct PassKernel {
+  void (*launch)(const EngineArgs &, hipStream_t);
+  int (*blocks_per_cu)();
+};
+
+template <int MODE, bool R, bool W, int SH, int KS>
+PassKernel pass_kernel() {
+  return PassKernel{launch_pass_kernel<MODE, R, W, SH, KS>, pass_blocks_per_cu<MODE, R, W, SH, KS>};
+}
+
+// The pass compiled for (program shape, kick set); a flush applies no
+// program and runs the generic one.
+template <int MODE, bool R, bool W, int SH>
+PassKernel pick_kicks(int ks) {
+  if constexpr (W && MODE == 0)
+    if (ks == PHX_KICKS_DXDY) return pass_kernel<MODE, R, W, SH, PHX_KICKS_DXDY>();
+  return pass_kernel<MODE, R, W, SH, PHX_KICKS_GENERIC>();
+}
+
+template <int MODE, bool R, bool W>
+PassKernel pick_shape(int shape, int ks) {
+  if constexpr (MODE == 0) {
+    switch (shape) {
+      case PHX_SHAPE_RF1_T: return pick_kicks<MODE, R, W, PHX_SHAPE_RF1_T>(ks);
+      case PHX_SHAPE_RF2_T: return pick_kicks<MODE, R, W, PHX_SHAPE_RF2_T>(ks);
+      case PHX_SHAPE_T_RF1: return pick_kicks<MODE, R, W, PHX_SHAPE_T_RF1>(ks);
+      case PHX_SHAPE_T_RF2: return pick_kicks<MODE, R, W, PHX_SHAPE_T_RF2>(ks);
+      case PHX_SHAPE_LIN_T: return pick_kicks<MODE, R, W, PHX_SHAPE_LIN_T>(ks);
+      case PHX_SHAPE_T_LIN: return pick_kicks<MODE, R, W, PHX_SHAPE_T_LIN>(ks);
+      case PHX_SHAPE_T: return pick_kicks<MODE, R, W, PHX_SHAPE_T>(ks);
+      default: break;
+    }
+  }
+  return pick_kicks<MODE, R, W, PHX_SHAPE_GENERIC>(ks);
+}
+
+// MODE 0 with a wake field: the histogram kernel folds the pass's partials;
+// otherwise a one-block finalize does.
+template <int MODE>
+PassKernel pick_pass(const phx_engine &e, bool wake_pass) {
+  const bool rec = e.record_stats != 0;
+  const int shape = e.program_shape, ks = e.kick_shape;
+  if (MODE == 0 && wake_pass)
+    return rec ? pick_shape<MODE, true, true>(shape, ks) : pick_shape<MODE, false, true>(shape, ks);
+  return rec ? pick_shape<MODE, true, false>(shape, ks) : pick_shape<MODE, false, false>(shape, ks);
+}
+
+// The pass grid: the caller's, or as many blocks as stay resident.
+int pass_grid(const phx_engine &e, const PassKernel &k) {
+  int64_t g = e.pass_grid > 0 ? (int64_t)e.pass_grid
+                              : (int64_t)k.blocks_per_cu() * (int64_t)phx_num_cus();
+  if (g > PHX_MAX_GRID) g = PHX_MAX_GRID;
+  if (g < 1) g = 1;
+  return phx_grid_for(e.n, g);
+}
+
 EngineArgs make_args(const phx_engine *e, int nk, int T) {
-  EngineArgs a{*e, nk, T, 0};
-  a.kick_lds = nk > 0 && e->n_slices > 0 && kick_lds_bytes(*e, nk) <= EN_KICK_LDS_MAX;
+  const bool has_wake = e->n_slices > 0 && nk > 0;
+  EngineArgs a{*e, nk, T, 1};
+  a.grid = pass_grid(*e, pick_pass<0>(*e, has_wake));
   return a;
 }
 
-template <bool REC, bool WAKE>
-size_t pass_lds_bytes(const EngineArgs &a) {
-  const size_t tables = a.kick_lds ? (PHX_BLOCK / PHX_WAVE) * kick_lds_bytes(a.e, a.n_kicks) : 0;
-  constexpr int NV = PassLayout<REC, WAKE>::NV;
-  const size_t red = BR_LDS(NV) * sizeof(double);
-  return tables > red ? tables : red;
-}
-
 template <int MODE>
-int launch_pass(const EngineArgs &a, bool has_wake, hipStream_t s) {
-  const phx_engine &e = a.e;
-  const dim3 grid((unsigned)phx_pass_grid(e.n));
-  const bool rec = e.record_stats != 0;
-  double *const *c = e.coords;
-#define PHX_PASS(R, W)                                                                   \
-  k_engine_pass<MODE, R, W><<<grid, PHX_BLOCK, pass_lds_bytes<R, W>(a), s>>>(            \
-      a, c[0], c[1], c[2], c[3], c[4], c[5], e.programs, e.kicks)
-  if (MODE == 0 && has_wake) {
-    // the histogram kernel folds the partials of this pass
-    if (rec) PHX_PASS(true, true);
-    else PHX_PASS(false, true);
-    PHX_CHECK_LAUNCH("phx_engine: pass");
-    return PHX_OK;
-  }
-  if (rec) PHX_PASS(true, false);
-  else PHX_PASS(false, false);
-#undef PHX_PASS
+int launch_pass(const EngineArgs &args, bool has_wake, hipStream_t s) {
+  const bool wake_pass = MODE == 0 && has_wake;
+  const PassKernel k = pick_pass<MODE>(args.e, wake_pass);
+  EngineArgs a = args;
+  if (MODE != 0) a.grid = pass_grid(a.e, k);
+  k.launch(a, s);
   PHX_CHECK_LAUNCH("phx_engine: pass");
-  if (rec) k_engine_fin<MODE, true><<<1, PHX_BLOCK, 0, s>>>(a);
+  if (wake_pass) return PHX_OK;
+  if (a.e.record_stats) k_engine_fin<MODE, true><<<1, PHX_BLOCK, 0, s>>>(a);
   else k_engine_fin<MODE, false><<<1, PHX_BLOCK, 0, s>>>(a);
   PHX_CHECK_LAUNCH("phx_engine: finalize");
   return PHX_OK;

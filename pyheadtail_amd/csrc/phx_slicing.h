@@ -6,6 +6,7 @@
 #include "phx_device.h"
 
 size_t phx_max_lds_bytes();
+int phx_num_cus();
 
 struct Coords6 {
   const double *c[6];

@@ -47,7 +47,7 @@ __global__ void __launch_bounds__(PHX_BLOCK)
     p.yp = yp ? yp[j] : 0.0;
     p.dp = dp ? dp[j] : 0.0;
     p.z = 0.0;
-    apply_wake_kicks(p, s, S, kd.k, n_kicks, kicks, S, 1);
+    apply_wake_kicks(p, s, S, kd.k, n_kicks, kicks);
     if (xp) xp[j] = p.xp;
     if (yp) yp[j] = p.yp;
     if (dp) dp[j] = p.dp;

@@ -73,7 +73,8 @@ class FusedOneTurnMap(Element):
 
     LOCAL = (TransverseSegmentMap, LinearMap, RFSystems, Drift, Kick)
 
-    def __init__(self, one_turn_map, record_stats=False, graph_turns=8, *args, **kwargs):
+    def __init__(self, one_turn_map, record_stats=False, graph_turns=8, specialise=True,
+                 pass_grid=0, *args, **kwargs):
         elements = list(one_turn_map)
         self.wake = None
         if elements and isinstance(elements[-1], WakeField):
@@ -100,6 +101,13 @@ class FusedOneTurnMap(Element):
         self._graphs = {}
         self._engine = None
         self.last_stats = None
+        self._shape = (_phx.SHAPE_GENERIC, _phx.KICKS_GENERIC)
+        # False: always the generic pass (the same arithmetic, for checks)
+        self.specialise = bool(specialise)
+        # blocks of the particle pass (0: as many as stay resident); the cuts
+        # equal those of the element-by-element slicing bit for bit at
+        # _phx.STANDALONE_GRID
+        self.pass_grid = int(pass_grid)
 
     # -- per-turn programs -------------------------------------------------------
     def _program(self, state):
@@ -139,6 +147,37 @@ class FusedOneTurnMap(Element):
         return prog
 
     @staticmethod
+    def _shapes(prog, wake):
+        """(PHX_SHAPE_*, PHX_KICKS_*) of a turn program: its element order,
+        when the library has a pass compiled for it, and its kick set.  All
+        programs of a map share them (the same elements in every turn)."""
+        kinds = []
+        for i in range(prog.n_el):
+            t, k = prog.el_type[i], prog.el_index[i]
+            if t == _phx.EL_TSEG:
+                kinds.append('T')
+            elif t == _phx.EL_LINMAP:
+                kinds.append('LIN')
+            else:
+                seq = prog.lseq[k]
+                ops = [seq.op[j].type for j in range(seq.n_ops)]
+                nh = len(ops) - 2
+                rf = (nh in (1, 2) and ops[0] == _phx.LONG_DRIFT and ops[-1] == _phx.LONG_DRIFT
+                      and all(o == _phx.LONG_KICK for o in ops[1:-1]))
+                kinds.append('RF%d' % nh if rf else '?')
+        shape = {('RF1', 'T'): _phx.SHAPE_RF1_T, ('RF2', 'T'): _phx.SHAPE_RF2_T,
+                 ('T', 'RF1'): _phx.SHAPE_T_RF1, ('T', 'RF2'): _phx.SHAPE_T_RF2,
+                 ('LIN', 'T'): _phx.SHAPE_LIN_T, ('T', 'LIN'): _phx.SHAPE_T_LIN,
+                 ('T',): _phx.SHAPE_T}.get(tuple(kinds), _phx.SHAPE_GENERIC)
+        kicks = _phx.KICKS_GENERIC
+        if wake is not None and prog.n_kicks == 2:
+            k0, k1 = prog.kick[0], prog.kick[1]
+            if ((k0.plane, k0.multiply, k1.plane, k1.multiply) ==
+                    (_phx.PLANE_XP, -1, _phx.PLANE_YP, -1)):
+                kicks = _phx.KICKS_DXDY
+        return shape, kicks
+
+    @staticmethod
     def _state_key(state):
         return (state._gamma, state._beta, state._betagamma, state._p0)
 
@@ -153,6 +192,8 @@ class FusedOneTurnMap(Element):
         while t < n_turns:
             before = self._state_key(state)
             p = self._program(state)
+            if t == 0 and self.specialise:
+                self._shape = self._shapes(p, self.wake)
             key = bytes(memoryview(p).cast("B"))
             if key not in index:
                 index[key] = len(progs)
@@ -214,6 +255,8 @@ class FusedOneTurnMap(Element):
         eng.n_kicks = b['n_kicks']
         eng.hist_capacity = b['T']
         eng.record_stats = 1 if self.record_stats else 0
+        eng.program_shape, eng.kick_shape = self._shape
+        eng.pass_grid = self.pass_grid
         eng.programs = b['programs'].data_ptr()
         eng.turn_program = b['table'].data_ptr()
         eng.stats = b['stats'].data_ptr()

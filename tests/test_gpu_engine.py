@@ -23,6 +23,13 @@ def _fused(machine, **kw):
     return FusedOneTurnMap(machine, **kw)
 
 
+def _standalone_grid():
+    """The pass grid whose fold order is that of the standalone slicing, so
+    that the engine's cuts equal the element path's bit for bit."""
+    from pyheadtail_amd import _phx
+    return _phx.STANDALONE_GRID
+
+
 def test_engine_equals_element_path():
     _print_case("engine vs elements, N=1e5, 500 slices, 12 turns")
     ob = W.nonlinear_bunch(100000, seed=11)
@@ -31,7 +38,7 @@ def test_engine_equals_element_path():
     for _ in range(12):
         for el in m_el:
             el.track(a)
-    eng = _fused(nonlinear_machine(), graph_turns=0)
+    eng = _fused(nonlinear_machine(), graph_turns=0, pass_grid=_standalone_grid())
     eng.track_turns(b, 12)
     ga, gb = to_numpy(a), to_numpy(b)
     for k in COORDS:
@@ -128,9 +135,94 @@ def test_engine_multiturn_wake_equals_elements():
     for _ in range(12):
         for el in m_el:
             el.track(a)
-    eng = _fused(nonlinear_machine(n_slices=100, n_turns_wake=8), graph_turns=4)
+    eng = _fused(nonlinear_machine(n_slices=100, n_turns_wake=8), graph_turns=4,
+                 pass_grid=_standalone_grid())
     eng.track_turns(b, 12)
     ga, gb = to_numpy(a), to_numpy(b)
     for k in COORDS:
         np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)
 
+
+def test_engine_automatic_grid_close_to_standalone_grid():
+    """The automatic pass grid changes only the summation order of the
+    bunch sums behind the cuts: coordinates agree to rounding."""
+    _print_case("engine automatic grid vs standalone grid, 10 turns")
+    ob = W.nonlinear_bunch(100000, seed=16)
+    outs = []
+    for grid in (0, _standalone_grid(), 300):
+        pb = to_product(ob)
+        eng = _fused(nonlinear_machine(), graph_turns=0, pass_grid=grid)
+        eng.track_turns(pb, 10)
+        outs.append(to_numpy(pb))
+    for o in outs[1:]:
+        for k in COORDS:
+            ref = outs[0][k]
+            assert np.max(np.abs(o[k] - ref)) <= 1e-12 * np.sqrt(np.mean(ref * ref)), k
+
+
+def _shape_case_machines():
+    """One map per compiled element order, with round-chamber (dipole x, y)
+    and parallel-plates (four kinds) wakes."""
+    from pyheadtail_amd.trackers.longitudinal_tracking import LinearMap, RFSystems
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.impedances.wakes import (WakeField, CircularResonator,
+                                                 ParallelHorizontalPlatesResonator)
+    from scipy.constants import e, m_p
+
+    def rf(nh):
+        h = [W.H_RF, 4 * W.H_RF][:nh]
+        v = [W.V_RF, 0.1 * W.V_RF][:nh]
+        return RFSystems(W.CIRCUMFERENCE, h, v, [0., np.pi][:nh], [W.ALPHA_C], W.GAMMA,
+                         p_increment=0., charge=e, mass=m_p)
+
+    def tseg():
+        from tests.machines import smooth_transverse
+        return smooth_transverse(5., 1e-10 * W.P0, -5e-11 * W.P0)[0]
+
+    def lin():
+        return LinearMap(np.atleast_1d(W.ALPHA_C), W.CIRCUMFERENCE, 0.017)
+
+    def wake(plates):
+        src = ParallelHorizontalPlatesResonator(1e6, 1e9, 1) if plates else \
+            CircularResonator(1e6, 1e9, 1)
+        return WakeField(UniformBinSlicer(64, n_sigma_z=3), src)
+
+    return {
+        'RF1_T': lambda: [rf(1), tseg(), wake(False)],
+        'RF2_T': lambda: [rf(2), tseg(), wake(True)],
+        'T_RF1': lambda: [tseg(), rf(1), wake(True)],
+        'T_RF2': lambda: [tseg(), rf(2), wake(False)],
+        'LIN_T': lambda: [lin(), tseg(), wake(False)],
+        'T_LIN': lambda: [tseg(), lin(), wake(True)],
+        'T': lambda: [tseg(), wake(False)],
+        'RF1_T_nowake': lambda: [rf(1), tseg()],
+    }
+
+
+@pytest.mark.parametrize("case", sorted(_shape_case_machines()))
+@pytest.mark.parametrize("rec", [False, True])
+def test_specialised_pass_equals_generic(case, rec):
+    """The passes compiled for an element order and kick set run the same
+    arithmetic as the generic interpreter: bit-identical coordinates and
+    statistics."""
+    _print_case("specialised vs generic pass: %s, record_stats=%s" % (case, rec))
+    from pyheadtail_amd import _phx
+    make = _shape_case_machines()[case]
+    ob = W.nonlinear_bunch(20000, seed=21)
+    outs, stats, shapes = [], [], []
+    for spec in (True, False):
+        pb = to_product(ob)
+        eng = _fused(make(), graph_turns=0, specialise=spec, record_stats=rec,
+                     pass_grid=_standalone_grid())
+        eng.track_turns(pb, 6)
+        outs.append(to_numpy(pb))
+        shapes.append(eng._shape)
+        if rec:
+            stats.append(eng.last_stats['raw'])
+    assert shapes[1] == (_phx.SHAPE_GENERIC, _phx.KICKS_GENERIC)
+    expect = getattr(_phx, 'SHAPE_' + case.replace('_nowake', ''))
+    assert shapes[0][0] == expect
+    for k in COORDS:
+        np.testing.assert_array_equal(outs[0][k], outs[1][k], err_msg=k)
+    if rec:
+        np.testing.assert_array_equal(stats[0], stats[1])
