@@ -270,6 +270,15 @@ typedef struct {
 int phx_linear_map(double *x, double *y, double *z, double *dp, int64_t n,
                    const phx_linmap *p, phx_stream_t stream);
 
+/* The FP64 sine and cosine the maps evaluate on the device (the detuned
+   betatron phase, the RF phase): s[i] = sin(x[i]), c[i] = cos(x[i]) for
+   device arrays of n values; either output may be null.  No reference
+   counterpart: it exposes the device arithmetic behind
+   TransverseSegmentMap.track and Kick.track (np.sin / np.cos,
+   general/pmath.py:178-179, 239) for accuracy checks. */
+int phx_math_sincos(const double *x, int64_t n, double *s, double *c,
+                    phx_stream_t stream);
+
 /* ------------------------------------------------------------------------ */
 /* Wake fields (impedances/wakes.py:77-145, 379-678;                        */
 /* impedances/wake_kicks.py:19-361)                                         */

@@ -174,6 +174,7 @@ SIGNATURES = {
     "phx_transverse_segment": (C.c_int, [P, P, P, P, P, I64, P, P]),
     "phx_longitudinal": (C.c_int, [P, P, P, P, P, P, I64, P, P]),
     "phx_linear_map": (C.c_int, [P, P, P, P, I64, P, P]),
+    "phx_math_sincos": (C.c_int, [P, I64, P, P, P]),
     "phx_wake_hist_push": (C.c_int, [P, P, P, P, P, D, D, P]),
     "phx_wake_workspace_bytes": (I64, [I32, I32, I32]),
     "phx_wake_convolve": (C.c_int, [P, P, I32, I32, I32, P, P, P]),

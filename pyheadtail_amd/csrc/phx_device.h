@@ -316,6 +316,27 @@ __device__ __forceinline__ void apply_shape_n(P6 (&p)[N], const phx_turn_program
   }
 }
 
+// z and dp after a program: only the longitudinal elements change them
+// (transverse segments act on x, xp, y, yp), so applying those alone gives
+// the same z and dp bit for bit.
+template <int SHAPE>
+__device__ __forceinline__ void apply_shape_longitudinal(P6 &q, const phx_turn_program &g) {
+  P6(&p)[1] = *reinterpret_cast<P6(*)[1]>(&q);
+  if constexpr (SHAPE == PHX_SHAPE_RF1_T || SHAPE == PHX_SHAPE_T_RF1) {
+    apply_rf_seq_n<1, 1>(p, g.lseq[0]);
+  } else if constexpr (SHAPE == PHX_SHAPE_RF2_T || SHAPE == PHX_SHAPE_T_RF2) {
+    apply_rf_seq_n<2, 1>(p, g.lseq[0]);
+  } else if constexpr (SHAPE == PHX_SHAPE_LIN_T || SHAPE == PHX_SHAPE_T_LIN) {
+    apply_linmap_n<1>(p, g.lin[0]);
+  } else if constexpr (SHAPE == PHX_SHAPE_GENERIC) {
+    for (int e = 0; e < g.n_el; ++e) {
+      const int k = g.el_index[e];
+      if (g.el_type[e] == PHX_EL_LONG) apply_long_seq_n<1>(p, g.lseq[k]);
+      else if (g.el_type[e] == PHX_EL_LINMAP) apply_linmap_n<1>(p, g.lin[k]);
+    }
+  }
+}
+
 __device__ __forceinline__ P6 (&one(P6 &p))[1] { return *reinterpret_cast<P6(*)[1]>(&p); }
 __device__ __forceinline__ void apply_tseg(P6 &p, const phx_tseg &t) { apply_tseg_n<1>(one(p), t); }
 __device__ __forceinline__ void apply_long_seq(P6 &p, const phx_long_seq &s) {
@@ -441,19 +462,42 @@ __device__ __forceinline__ double block_reduce_slots(const double (&v)[NV], doub
 
 // Reduce per-chunk partials part[c*NV + i] over c < n_chunks into res[i]
 // (__shared__, valid on all threads after return).  Called by one block.
-template <int NV, unsigned long long MINM, unsigned long long MAXM>
-__device__ __forceinline__ void reduce_partials(const double *part, int64_t n_chunks,
-                                                double *res, double *lds) {
-  double acc[NV];
+// Fold of the per-block partials of a reducing pass, stored slot-major:
+// slot i of block c is part[i * nc + c].  One wave folds one slot: lane l
+// combines the partials l, l + 64, l + 128, ... in increasing order, then
+// the 64 lane values combine in a fixed butterfly (exchange distance 32,
+// 16, ..., 1; the operations are commutative, so every lane ends with the
+// same value).  The order depends on nc only, not on the block size, so
+// every fold of the same partials gives the same bits.  The waves of the
+// block take slots i0 + w, i0 + w + W, ...; res[i] receives slot i.  Ends
+// with a barrier.
+template <unsigned long long MINM, unsigned long long MAXM>
+__device__ __forceinline__ void fold_partials(const double *part, int64_t nc, int i0, int i1,
+                                              double *res) {
+  const int lane = threadIdx.x % PHX_WAVE, wv = threadIdx.x / PHX_WAVE;
+  const int nw = blockDim.x / PHX_WAVE;
+  for (int i = i0 + wv; i < i1; i += nw) {
+    const double *row = part + (size_t)i * nc;
+    const double id = red_ident<MINM, MAXM>(i);
+    double acc = id;
+    // loads in batches, so that they are in flight together; padding with
+    // the identity leaves the combination order unchanged
+    constexpr int B = 8;
+    for (int64_t c0 = lane; c0 < nc; c0 += (int64_t)B * PHX_WAVE) {
+      double v[B];
 #pragma unroll
-  for (int i = 0; i < NV; ++i) acc[i] = red_ident<MINM, MAXM>(i);
-  for (int64_t c = threadIdx.x; c < n_chunks; c += PHX_BLOCK) {
-    const double *pc = part + c * NV;
+      for (int k = 0; k < B; ++k) {
+        const int64_t c = c0 + (int64_t)k * PHX_WAVE;
+        v[k] = c < nc ? row[c] : id;
+      }
 #pragma unroll
-    for (int i = 0; i < NV; ++i) acc[i] = red_slot<MINM, MAXM>(i, acc[i], pc[i]);
+      for (int k = 0; k < B; ++k) acc = red_slot<MINM, MAXM>(i, acc, v[k]);
+    }
+#pragma unroll
+    for (int off = PHX_WAVE / 2; off > 0; off >>= 1)
+      acc = red_slot<MINM, MAXM>(i, acc, __shfl_xor(acc, off));
+    if (lane == 0) res[i] = acc;
   }
-  const double r = block_reduce_slots<NV, MINM, MAXM>(acc, lds);
-  if (threadIdx.x < NV) res[threadIdx.x] = r;
   lds_sync();
 }
 

@@ -174,14 +174,20 @@ __global__ void __launch_bounds__(PHX_BLOCK)
     nxt[k] = P6{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   }
   // Particle 0 sets the shifts of the shifted sums (stats.pyx:cov): its
-  // state after the pending kick, and its z after this turn's elements.
-  // Every lane computes it (uniform values; once per block).
+  // state after the pending kick (statistics), and its z after this turn's
+  // elements (cuts), which needs the kick only where a kick changes dp and
+  // only the longitudinal elements.  Every lane computes it (uniform values;
+  // once per block).
   double src[7];
   {
     P6 q = P6{x[0], xp[0], y[0], yp[0], z[0], dp[0]};
-    if (pend) kick_pending<KS>(q, g, nk, tail, width, S, kicks);
+    bool kick_dp = false;
+    if constexpr (KS == PHX_KICKS_GENERIC) {
+      for (int k = 0; k < nk; ++k) kick_dp = kick_dp || g.kick[k].plane == PHX_PLANE_DP;
+    }
+    if (pend && (REC || kick_dp)) kick_pending<KS>(q, g, nk, tail, width, S, kicks);
     src[0] = q.x; src[1] = q.xp; src[2] = q.y; src[3] = q.yp; src[4] = q.z; src[5] = q.dp;
-    if (MODE == 0) apply_shape_n<SHAPE, 1>(one(q), g);
+    if (MODE == 0) apply_shape_longitudinal<SHAPE>(q, g);
     src[6] = q.z;
   }
   double v[L::NV];
@@ -243,42 +249,16 @@ __global__ void __launch_bounds__(PHX_BLOCK)
   if (threadIdx.x < L::NV) w.partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = r;
 }
 
-// Fold of slots [I0, I1) of the chunk partials (slot-major, so the loads
-// are coalesced) into res[I0..I1), in the order of the standalone fold
-// (reduce_partials): the first PHX_BLOCK threads stride over the chunks,
-// then the block reduction; so the cuts match the standalone slicing bit for
-// bit whatever the block size.  Slots are independent: folding a subset
-// gives the same values.
-template <int I0, int I1, unsigned long long MINM, unsigned long long MAXM>
-__device__ __forceinline__ void fold_slots(const double *part, int64_t nc, double *res,
-                                           double *lds) {
-  constexpr int NV = I1 - I0;
-  double acc[NV];
-#pragma unroll
-  for (int i = 0; i < NV; ++i) acc[i] = red_ident<MINM, MAXM>(I0 + i);
-  if (threadIdx.x < PHX_BLOCK) {
-    for (int64_t c = threadIdx.x; c < nc; c += PHX_BLOCK) {
-#pragma unroll
-      for (int i = 0; i < NV; ++i)
-        acc[i] = red_slot<MINM, MAXM>(I0 + i, acc[i], part[(size_t)(I0 + i) * nc + c]);
-    }
-  }
-  const double r = block_reduce_slots<NV, MINM, MAXM, I0>(acc, lds);
-  if (threadIdx.x < NV) res[I0 + threadIdx.x] = r;
-  lds_sync();
-}
-
 // One block: fold the chunk partials, then the per-turn bookkeeping (for
 // maps without a wake field, and for the flush).
 template <int MODE, bool REC>
 __global__ void __launch_bounds__(PHX_BLOCK) k_engine_fin(EngineArgs a) {
   using L = PassLayout<REC, false>;
   const phx_engine &e = a.e;
-  __shared__ double lds[BR_LDS(L::NV)];
   __shared__ double res[L::NV];
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
   const int turn = *e.turn_counter;
-  fold_slots<0, L::NV, L::MINM, L::MAXM>(w.partials, a.grid, res, lds);
+  fold_partials<L::MINM, L::MAXM>(w.partials, a.grid, 0, L::NV, res);
   if (REC && threadIdx.x < EN_STAT_NV)
     e.stats[(size_t)turn * EN_STAT_NV + threadIdx.x] = res[L::SOFF + threadIdx.x];
   if (threadIdx.x == 0) {
@@ -297,7 +277,6 @@ template <bool REC>
 __global__ void __launch_bounds__(EN_HIST_THREADS) k_engine_hist(EngineArgs a) {
   using L = PassLayout<REC, true>;
   extern __shared__ unsigned long long acc_lds[];
-  __shared__ double lds[BR_LDS(L::NV)];
   __shared__ double res[L::NV];
   __shared__ double sh_tail, sh_width;
   __shared__ int sh_ex, sh_ey;
@@ -307,10 +286,35 @@ __global__ void __launch_bounds__(EN_HIST_THREADS) k_engine_hist(EngineArgs a) {
   const int turn = *e.turn_counter;
   const phx_turn_program &g = e.programs[e.turn_program[turn]];
   const int64_t ncp = a.grid;
+  constexpr int CH = EN_HIST_THREADS * PHX_PPT;
+  const int64_t nc = (e.n + CH - 1) / CH;
+  const int64_t n = e.n;
+  // The block's first particles load while the cuts are folded (the fold's
+  // barriers are LDS-only, so the loads stay in flight), and each chunk
+  // loads the next one before it is binned.
+  double zz[PHX_PPT], xx[PHX_PPT], yy[PHX_PPT];
+  double zn[PHX_PPT], xn[PHX_PPT], yn[PHX_PPT];
+  auto load_chunk = [&](int64_t c, double (&z)[PHX_PPT], double (&x)[PHX_PPT],
+                        double (&y)[PHX_PPT]) {
+#pragma unroll
+    for (int k = 0; k < PHX_PPT; ++k) {
+      const int64_t j = c * CH + threadIdx.x + (int64_t)k * EN_HIST_THREADS;
+      if (c < nc && j < n) {
+        z[k] = e.coords[4][j];
+        x[k] = e.coords[0][j];
+        y[k] = e.coords[2][j];
+      } else {
+        z[k] = 0.0;
+        x[k] = 0.0;
+        y[k] = 0.0;
+      }
+    }
+  };
+  load_chunk(blockIdx.x, zz, xx, yy);
   // every block needs the z block; only block 0 writes the statistics row
-  fold_slots<0, ZB_NV, L::MINM, L::MAXM>(w.partials, ncp, res, lds);
+  fold_partials<L::MINM, L::MAXM>(w.partials, ncp, 0, ZB_NV, res);
   if constexpr (REC) {
-    if (blockIdx.x == 0) fold_slots<ZB_NV, L::NV, L::MINM, L::MAXM>(w.partials, ncp, res, lds);
+    if (blockIdx.x == 0) fold_partials<L::MINM, L::MAXM>(w.partials, ncp, ZB_NV, L::NV, res);
   }
   if (threadIdx.x == 0) {
     phx_slice_state st;
@@ -343,24 +347,13 @@ __global__ void __launch_bounds__(EN_HIST_THREADS) k_engine_hist(EngineArgs a) {
   acc_lds_zero(A, S, 2);  // ends with a barrier: the shared cuts are visible
   const double tail = sh_tail, width = sh_width;
   const int ex = sh_ex, ey = sh_ey;
-  constexpr int CH = EN_HIST_THREADS * PHX_PPT;
-  const int64_t nc = (e.n + CH - 1) / CH;
   for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
+    load_chunk(c + gridDim.x, zn, xn, yn);
     const int64_t base = c * CH + threadIdx.x;
-    double zz[PHX_PPT], xx[PHX_PPT], yy[PHX_PPT];
 #pragma unroll
     for (int k = 0; k < PHX_PPT; ++k) {
       const int64_t j = base + (int64_t)k * EN_HIST_THREADS;
-      if (j < e.n) {
-        zz[k] = e.coords[4][j];
-        xx[k] = e.coords[0][j];
-        yy[k] = e.coords[2][j];
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < PHX_PPT; ++k) {
-      const int64_t j = base + (int64_t)k * EN_HIST_THREADS;
-      if (j < e.n) {
+      if (j < n) {
         const double f = slice_floor(zz[k], tail, width);
         if (f >= 0.0 && f < (double)S) {
           const int s = (int)f;
@@ -369,6 +362,12 @@ __global__ void __launch_bounds__(EN_HIST_THREADS) k_engine_hist(EngineArgs a) {
           acc_add_value(A, S, 1, s, yy[k], ey, w.acc.fallback);
         }
       }
+    }
+#pragma unroll
+    for (int k = 0; k < PHX_PPT; ++k) {
+      zz[k] = zn[k];
+      xx[k] = xn[k];
+      yy[k] = yn[k];
     }
   }
   acc_lds_store(A, S, 2, w.acc, blockIdx.x);

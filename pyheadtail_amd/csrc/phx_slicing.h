@@ -186,13 +186,14 @@ __device__ __forceinline__ void acc_lds_store(const AccLds &L, int S, int T, con
   }
 }
 
-// Merge of the H partial histograms for a tile of RD_SL slices: RD_G thread
-// groups stride over the partials, then the group sums are combined through
-// LDS, RD_V values per round (integer sums: order-free).  Thread g == 0 of
-// slice s gets the totals.
-#define RD_SL 16
-#define RD_G 64
-#define RD_V 5
+// Merge of the H partial histograms for a tile of RD_SL slices, one block of
+// PHX_BLOCK threads: thread t takes slice t % RD_SL and the partials
+// h = t / RD_SL, t / RD_SL + RD_G, ...; the RD_G groups of a slice are then
+// combined by lane exchanges within each wave and across the waves in LDS.
+// The sums are integers, so the order of the additions does not matter.
+// Threads 0 .. RD_SL-1 receive the totals of their slice.
+#define RD_SL 8
+#define RD_G (PHX_BLOCK / RD_SL)
 
 struct SliceTotals {
   unsigned int count;
@@ -202,8 +203,8 @@ struct SliceTotals {
 template <int T>
 __device__ __forceinline__ bool acc_merge(const AccParts &P, int S, SliceTotals &out) {
   constexpr int NVAL = 2 * T + 1;
-  __shared__ unsigned long long red[RD_V][RD_G][RD_SL];
-  __shared__ unsigned long long tot[NVAL][RD_SL];
+  constexpr int NW = PHX_BLOCK / PHX_WAVE;
+  __shared__ unsigned long long wsum[NW][NVAL][RD_SL];
   const int sl = threadIdx.x % RD_SL, g = threadIdx.x / RD_SL;
   const int s = blockIdx.x * RD_SL + sl;
   // v[0] = count, v[1 + 2i] = hi of term i, v[2 + 2i] = lo of term i
@@ -221,28 +222,30 @@ __device__ __forceinline__ bool acc_merge(const AccParts &P, int S, SliceTotals 
       }
     }
   }
+  // lanes of one slice differ in the bits above log2(RD_SL)
 #pragma unroll
-  for (int r0 = 0; r0 < NVAL; r0 += RD_V) {
+  for (int off = RD_SL; off < PHX_WAVE; off <<= 1)
 #pragma unroll
-    for (int i = 0; i < RD_V; ++i)
-      if (r0 + i < NVAL) red[i][g][sl] = v[r0 + i];
-    lds_sync();
-    if (threadIdx.x < RD_V * RD_SL) {
-      const int i = threadIdx.x / RD_SL, c = threadIdx.x % RD_SL;
-      if (r0 + i < NVAL) {
-        unsigned long long acc = 0ULL;
-        for (int q = 0; q < RD_G; ++q) acc += red[i][q][c];
-        tot[r0 + i][c] = acc;
-      }
-    }
-    lds_sync();
+    for (int i = 0; i < NVAL; ++i) v[i] += __shfl_xor(v[i], off);
+  const int lane = threadIdx.x % PHX_WAVE, wv = threadIdx.x / PHX_WAVE;
+  if (lane < RD_SL) {
+#pragma unroll
+    for (int i = 0; i < NVAL; ++i) wsum[wv][i][lane] = v[i];
   }
-  if (g != 0 || s >= S) return false;
-  out.count = (unsigned int)tot[0][sl];
+  lds_sync();
+  if (threadIdx.x >= RD_SL || s >= S) return false;
+  unsigned long long tot[NVAL];
+#pragma unroll
+  for (int i = 0; i < NVAL; ++i) {
+    tot[i] = 0ULL;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) tot[i] += wsum[q][i][sl];
+  }
+  out.count = (unsigned int)tot[0];
 #pragma unroll
   for (int i = 0; i < T; ++i) {
-    out.hi[i] = tot[1 + 2 * i][sl];
-    out.lo[i] = tot[2 + 2 * i][sl];
+    out.hi[i] = tot[1 + 2 * i];
+    out.lo[i] = tot[2 + 2 * i];
   }
   return true;
 }

@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(PHX_BLOCK)
   }
   }
   const double r = block_reduce_slots<L::NV, L::MINM, L::MAXM>(v, lds);
-  if (threadIdx.x < L::NV) partials[(int64_t)blockIdx.x * L::NV + threadIdx.x] = r;
+  if (threadIdx.x < L::NV) partials[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = r;
 }
 
 // One block folds the chunk partials in a fixed order.
@@ -76,9 +76,8 @@ template <int K>
 __global__ void __launch_bounds__(PHX_BLOCK)
     k_bunch_sums_fin(const double *partials, int64_t n_chunks, double *out) {
   using L = BunchLayout<K>;
-  __shared__ double lds[BR_LDS(L::NV)];
   __shared__ double res[L::NV];
-  reduce_partials<L::NV, L::MINM, L::MAXM>(partials, n_chunks, res, lds);
+  fold_partials<L::MINM, L::MAXM>(partials, n_chunks, 0, L::NV, res);
   if (threadIdx.x < L::NV) out[threadIdx.x] = res[threadIdx.x];
 }
 
@@ -151,15 +150,14 @@ __global__ void __launch_bounds__(PHX_BLOCK)
   }
   }
   const double r = block_reduce_slots<ZB_NV, ZB_MINM, ZB_MAXM>(v, lds);
-  if (threadIdx.x < ZB_NV) partials[(int64_t)blockIdx.x * ZB_NV + threadIdx.x] = r;
+  if (threadIdx.x < ZB_NV) partials[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = r;
 }
 
 __global__ void __launch_bounds__(PHX_BLOCK)
     k_slice_cuts_fin(const double *partials, int64_t n_chunks, phx_slicer cfg, int64_t n,
                      phx_slice_state *state, double *z_bins) {
-  __shared__ double lds[BR_LDS(ZB_NV)];
   __shared__ double res[ZB_NV];
-  reduce_partials<ZB_NV, ZB_MINM, ZB_MAXM>(partials, n_chunks, res, lds);
+  fold_partials<ZB_MINM, ZB_MAXM>(partials, n_chunks, 0, ZB_NV, res);
   zblock_finalize(res, cfg, n, state, z_bins, -1);
 }
 

@@ -167,3 +167,25 @@ extern "C" int phx_linear_map(double *x, double *y, double *z, double *dp, int64
   PHX_CHECK_LAUNCH("phx_linear_map");
   return PHX_OK;
 }
+
+__global__ void __launch_bounds__(PHX_BLOCK)
+    k_math_sincos(const double *__restrict__ x, int64_t n, double *__restrict__ s,
+                  double *__restrict__ c) {
+  const int64_t i = (int64_t)blockIdx.x * PHX_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  double sv, cv;
+  phx_sincos(x[i], &sv, &cv);
+  if (s) s[i] = sv;
+  if (c) c[i] = cv;
+}
+
+extern "C" int phx_math_sincos(const double *x, int64_t n, double *s, double *c,
+                               phx_stream_t stream) {
+  if (!x || n < 0) return phx::fail(PHX_E_INVALID, "phx_math_sincos: bad arguments");
+  if (n == 0) return PHX_OK;
+  const int64_t blocks = (n + PHX_BLOCK - 1) / PHX_BLOCK;
+  if (blocks > 0x7fffffffLL) return phx::fail(PHX_E_LIMIT, "phx_math_sincos: n too large");
+  k_math_sincos<<<dim3((unsigned)blocks), PHX_BLOCK, 0, phx::as_stream(stream)>>>(x, n, s, c);
+  PHX_CHECK_LAUNCH("phx_math_sincos");
+  return PHX_OK;
+}

@@ -6,7 +6,7 @@
 
 #include "phx_device.h"
 
-#define WK_TILE 16                   // target slices per block
+#define WK_TILE 32                   // target slices per block
 #define WK_GROUPS (PHX_BLOCK / WK_TILE)
 
 // Evaluation of the wake function with the reference's operation order.
@@ -18,14 +18,18 @@ __device__ __forceinline__ double wake_eval(const phx_wake_kick &w, double dt,
   if (w.fn == PHX_WAKE_RES_TRANSVERSE) {
     const double d = dt > 0.0 ? 0.0 : dt;  // dt.clip(max=0)
     const double ea = w.prefix * exp(w.alpha * d);
-    if (w.Q > 0.5) return ea * sin(w.omegabar * d);
+    if (w.Q > 0.5) return ea * phx_sin(w.omegabar * d);
     if (w.Q == 0.5) return ea * d;
     return ea * sinh(w.omegabar * d);
   }
   if (w.fn == PHX_WAKE_RES_LONGITUDINAL) {
     const double sg = dt > 0.0 ? 1.0 : (dt < 0.0 ? -1.0 : (dt == 0.0 ? 0.0 : dt));
     const double t = ((-(sg - 1.0) * w.prefix) * w.alpha) * exp(w.alpha * dt);
-    if (w.Q > 0.5) return t * (cos(w.omegabar * dt) + w.alpha_over_omegabar * sin(w.omegabar * dt));
+    if (w.Q > 0.5) {
+      double sn, cs;
+      phx_sincos(w.omegabar * dt, &sn, &cs);
+      return t * (cs + w.alpha_over_omegabar * sn);
+    }
     if (w.Q == 0.5) return t * (1.0 + w.alpha * dt);
     return t * (cosh(w.omegabar * dt) + w.alpha_over_omegabar * sinh(w.omegabar * dt));
   }

@@ -197,3 +197,34 @@ def test_wakefield_multiturn_dot_product():
         err = coord_err(to_numpy(pb), ob)
         print("[err] turn %d coordinates %.3e" % (turn, err), flush=True)
         assert err <= 1e-12, (turn, err)
+
+
+def test_device_sincos_accuracy():
+    """The FP64 sin / cos of the maps (Cody-Waite reduction, Taylor
+    polynomials) against long-double references: at most 1.5 ulp for
+    |x| < 2^20 (1.15 measured), the device library beyond; NaN and
+    infinity give NaN."""
+    from pyheadtail_amd import _phx
+    rng = np.random.RandomState(5)
+    xs = [rng.uniform(-r, r, 200000) for r in (1e-6, 0.8, 3.2, 130., 1e4, 2.**20)]
+    xs.append(np.array([0., -0., np.pi / 4, np.pi / 2, np.pi, 1e-300, 2.**20 - 1, 2.**20,
+                        3e6, 1e15]))
+    x = np.concatenate(xs)
+    xd = torch.from_numpy(x).to(_phx.device())
+    s, c = torch.empty_like(xd), torch.empty_like(xd)
+    _phx.call(_phx.lib().phx_math_sincos, xd.data_ptr(), x.size, s.data_ptr(), c.data_ptr(),
+              _phx.stream())
+    s, c = s.cpu().numpy(), c.cpu().numpy()
+    xl = x.astype(np.longdouble)
+    for got, ref in ((s, np.sin(xl)), (c, np.cos(xl))):
+        ulp = np.spacing(np.abs(ref.astype(np.float64)))
+        err = np.abs(got.astype(np.longdouble) - ref) / ulp
+        small = np.abs(x) < 2.**20
+        print("[err] max %.3f ulp (|x| < 2^20)" % float(np.max(err[small])), flush=True)
+        assert float(np.max(err[small])) <= 1.5
+        assert float(np.max(err[~small])) <= 2.0
+    bad = np.array([np.nan, np.inf, -np.inf])
+    bd = torch.from_numpy(bad).to(_phx.device())
+    s2 = torch.empty_like(bd)
+    _phx.call(_phx.lib().phx_math_sincos, bd.data_ptr(), 3, s2.data_ptr(), None, _phx.stream())
+    assert np.all(np.isnan(s2.cpu().numpy()))
