@@ -76,6 +76,7 @@ struct EngineArgs {
   int n_kicks;
   int T;
   int grid;  // blocks of the particle pass (its partials: [slot][grid])
+  int gmax;  // kicks one convolution block serves (LDS permitting)
 };
 
 __device__ __forceinline__ P6 load_p(const phx_engine &e, int64_t j) {
@@ -419,7 +420,7 @@ __global__ void __launch_bounds__(PHX_BLOCK) k_engine_conv(EngineArgs a) {
   const phx_turn_program &g = e.programs[e.turn_program[e.state->turn]];
   const WakeKicks &kk = *reinterpret_cast<const WakeKicks *>(g.kick);
   if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0) *e.pending = 1;
-  wake_conv_block(e.hist, kk, e.n_slices, w.conv_part, a.T, e.kicks);
+  wake_conv_block(e.hist, kk, e.n_slices, w.conv_part, a.T, e.kicks, a.gmax);
 }
 
 __global__ void k_engine_conv_reduce(EngineArgs a) {
@@ -566,8 +567,9 @@ int pass_grid(const phx_engine &e, const PassKernel &k) {
 
 EngineArgs make_args(const phx_engine *e, int nk, int T) {
   const bool has_wake = e->n_slices > 0 && nk > 0;
-  EngineArgs a{*e, nk, T, 1};
+  EngineArgs a{*e, nk, T, 1, 1};
   a.grid = pass_grid(*e, pick_pass<0>(*e, has_wake));
+  if (e->n_slices > 0) a.gmax = wake_conv_gmax(e->n_slices, phx_max_lds_bytes());
   return a;
 }
 
@@ -600,7 +602,8 @@ int enqueue_stage(const EngineArgs &a, int stage, bool has_wake, hipStream_t s) 
     return PHX_OK;
   }
   const int tiles = (e.n_slices + WK_TILE - 1) / WK_TILE;
-  k_engine_conv<<<dim3(tiles, a.n_kicks, a.T), PHX_BLOCK, wake_conv_lds_bytes(e.n_slices), s>>>(a);
+  k_engine_conv<<<dim3(tiles, a.n_kicks, a.T), PHX_BLOCK, wake_conv_lds_bytes(e.n_slices, a.gmax),
+                  s>>>(a);
   PHX_CHECK_LAUNCH("phx_engine: conv");
   if (a.T > 1) {
     const int tot = a.n_kicks * e.n_slices;
@@ -620,7 +623,7 @@ int enqueue_turn(const EngineArgs &a, bool has_wake, hipStream_t s) {
 
 int check_lds(const phx_engine *e) {
   if (e->n_slices > 0 && (acc_lds_bytes(e->n_slices, 2) > phx_max_lds_bytes() ||
-                          wake_conv_lds_bytes(e->n_slices) + WK_STATIC_LDS > phx_max_lds_bytes()))
+                          wake_conv_lds_bytes(e->n_slices, 1) + WK_STATIC_LDS > phx_max_lds_bytes()))
     return phx::fail(PHX_E_LIMIT, "phx_engine: n_slices=%d exceeds LDS", e->n_slices);
   return PHX_OK;
 }

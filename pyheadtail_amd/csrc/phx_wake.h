@@ -9,40 +9,84 @@
 #define WK_TILE 32                   // target slices per block
 #define WK_GROUPS (PHX_BLOCK / WK_TILE)
 
-// Evaluation of the wake function with the reference's operation order.
-// rw_pre[c] is the resistive-wall prefix ((((Y*(c-2))/2)*sqrt(beta))*L/pi)/b^3
-// for sign(dt+|dt_min|) - 1 = c - 2 (precomputed per source turn; the chain
-// only takes three values).
-__device__ __forceinline__ double wake_eval(const phx_wake_kick &w, double dt,
-                                            const double *rw_pre) {
+// Evaluation of the wake function with the reference's operation order, in
+// two parts: the shape, which kicks of the same source share (they differ
+// only in the Yokoya factor or the prefix that carries it), and the value of
+// one kick from the shape.  wake_value(w, wake_shape(w, dt)) is the
+// function's value bit for bit.
+//   resonator, transverse:   W = (prefix * e^{a d}) * sin(wbar d)   (d = min(dt, 0))
+//   resonator, longitudinal: W = (((-(sgn(dt)-1) * R) * a) * e^{a dt})
+//                                * (cos(wbar dt) + a/wbar sin(wbar dt))
+//   resistive wall:          W = (pre[c] * sqrt(k_sigma / dtc)) * sqrt(Z0 c)
+// with rw_pre[c] the resistive-wall prefix ((((Y*(c-2))/2)*sqrt(beta))*L/pi)/b^3
+// for sign(dt+|dt_min|) - 1 = c - 2 (per source turn; the chain only takes
+// three values) and dtc = min(dt, -|dt_min|).
+struct WakeShape {
+  double f;  // e^{a d}; longitudinal: -(sgn(dt) - 1); resistive wall: class c, or NaN
+  double g;  // sin / d / sinh; longitudinal: e^{a dt}; resistive wall: sqrt(k_sigma/dtc)
+  double h;  // longitudinal: cos + a/wbar sin (cosh + a/wbar sinh, 1 + a dt)
+};
+
+__device__ __forceinline__ WakeShape wake_shape(const phx_wake_kick &w, double dt) {
+  WakeShape b;
+  b.h = 0.0;
   if (w.fn == PHX_WAKE_RES_TRANSVERSE) {
     const double d = dt > 0.0 ? 0.0 : dt;  // dt.clip(max=0)
-    const double ea = w.prefix * exp(w.alpha * d);
-    if (w.Q > 0.5) return ea * phx_sin(w.omegabar * d);
-    if (w.Q == 0.5) return ea * d;
-    return ea * sinh(w.omegabar * d);
-  }
-  if (w.fn == PHX_WAKE_RES_LONGITUDINAL) {
+    b.f = exp(w.alpha * d);
+    if (w.Q > 0.5) b.g = phx_sin(w.omegabar * d);
+    else if (w.Q == 0.5) b.g = d;
+    else b.g = sinh(w.omegabar * d);
+  } else if (w.fn == PHX_WAKE_RES_LONGITUDINAL) {
     const double sg = dt > 0.0 ? 1.0 : (dt < 0.0 ? -1.0 : (dt == 0.0 ? 0.0 : dt));
-    const double t = ((-(sg - 1.0) * w.prefix) * w.alpha) * exp(w.alpha * dt);
+    b.f = -(sg - 1.0);
+    b.g = exp(w.alpha * dt);
     if (w.Q > 0.5) {
       double sn, cs;
       phx_sincos(w.omegabar * dt, &sn, &cs);
-      return t * (cs + w.alpha_over_omegabar * sn);
+      b.h = cs + w.alpha_over_omegabar * sn;
+    } else if (w.Q == 0.5) {
+      b.h = 1.0 + w.alpha * dt;
+    } else {
+      b.h = cosh(w.omegabar * dt) + w.alpha_over_omegabar * sinh(w.omegabar * dt);
     }
-    if (w.Q == 0.5) return t * (1.0 + w.alpha * dt);
-    return t * (cosh(w.omegabar * dt) + w.alpha_over_omegabar * sinh(w.omegabar * dt));
+  } else {
+    const double a = dt + w.dt_min_abs;
+    if (a > 0.0) b.f = 2.0;
+    else if (a < 0.0) b.f = 0.0;
+    else if (a == 0.0) b.f = 1.0;
+    else b.f = a;  // NaN propagates like np.sign(nan)
+    const double mdt = -w.dt_min_abs;
+    const double dtc = dt > mdt ? mdt : dt;  // dt.clip(max=-|dt_min|)
+    b.g = sqrt(w.k_sigma / dtc);
   }
-  // resistive wall
-  const double a = dt + w.dt_min_abs;
-  double pre;
-  if (a > 0.0) pre = rw_pre[2];
-  else if (a < 0.0) pre = rw_pre[0];
-  else if (a == 0.0) pre = rw_pre[1];
-  else pre = a;  // NaN propagates like np.sign(nan)
-  const double mdt = -w.dt_min_abs;
-  const double dtc = dt > mdt ? mdt : dt;  // dt.clip(max=-|dt_min|)
-  return (pre * sqrt(w.k_sigma / dtc)) * w.sqrt_z0c;
+  return b;
+}
+
+__device__ __forceinline__ double wake_value(const phx_wake_kick &w, const WakeShape &b,
+                                             const double *rw_pre) {
+  if (w.fn == PHX_WAKE_RES_TRANSVERSE) return (w.prefix * b.f) * b.g;
+  if (w.fn == PHX_WAKE_RES_LONGITUDINAL) return (((b.f * w.prefix) * w.alpha) * b.g) * b.h;
+  const double pre = b.f == 0.0 ? rw_pre[0] : (b.f == 1.0 ? rw_pre[1] : (b.f == 2.0 ? rw_pre[2] : b.f));
+  return (pre * b.g) * w.sqrt_z0c;
+}
+
+__device__ __forceinline__ double wake_eval(const phx_wake_kick &w, double dt,
+                                            const double *rw_pre) {
+  return wake_value(w, wake_shape(w, dt), rw_pre);
+}
+
+// Kicks whose functions share the shape: same function, same parameters
+// apart from the Yokoya factor, same memory and convolution form.
+__device__ __forceinline__ bool same_shape(const phx_wake_kick &a, const phx_wake_kick &b) {
+  if (a.fn != b.fn || a.n_turns_wake != b.n_turns_wake || a.dot_product != b.dot_product)
+    return false;
+  if (a.fn == PHX_WAKE_RES_TRANSVERSE)
+    return a.Q == b.Q && a.alpha == b.alpha && a.omegabar == b.omegabar;
+  if (a.fn == PHX_WAKE_RES_LONGITUDINAL)
+    return a.Q == b.Q && a.alpha == b.alpha && a.omegabar == b.omegabar &&
+           a.alpha_over_omegabar == b.alpha_over_omegabar;
+  return a.L == b.L && a.b3 == b.b3 && a.dt_min_abs == b.dt_min_abs && a.k_sigma == b.k_sigma &&
+         a.sqrt_z0c == b.sqrt_z0c;
 }
 
 __device__ __forceinline__ void rw_prefixes(const phx_wake_kick &w, double beta_src,
@@ -105,22 +149,43 @@ __device__ __forceinline__ void hist_push_device(phx_wake_hist *h, const double 
 // convolution of its turn into part[(k*T + t)*S + j] and wake_conv_reduce
 // sums the turns in order (WakeKick._accumulate_source_signal,
 // wake_kicks.py:118-139).
+//
+// In the dot-product form (multi-turn memory), where every (source, target)
+// pair needs its own wake value, the kicks that share the function's shape
+// (same_shape, in index order) are served by one block, up to gmax of them:
+// the block of the member at position 0, gmax, 2 gmax, ... evaluates the
+// shape once per pair and the value of each member from it; the other
+// members' blocks return at once.  Each kick's value and sum are those of a
+// block of its own, bit for bit.
 struct WakeKicks {
   phx_wake_kick k[PHX_MAX_KICKS];
 };
 
+#define WK_GMAX 4  // kicks one block serves
+
 __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const WakeKicks &kk,
-                                                int S, double *part, int T, double *out) {
-  // LDS: [WK_TILE + S - 1] wake values (convolution form) or [S] source
-  // times (dot-product form), then [S] source moments
+                                                int S, double *part, int T, double *out,
+                                                int gmax) {
+  // LDS, convolution form: [WK_TILE + S - 1] wake values, then [S] source
+  // moments; dot-product form: [S] source times, then [gmax][S] moments.
   extern __shared__ double vbuf[];
-  __shared__ double red[PHX_BLOCK];
-  __shared__ double rwp[3];
-  const int tile = blockIdx.x, ki = blockIdx.y, t = blockIdx.z;
+  __shared__ double red[WK_GMAX][PHX_BLOCK];
+  __shared__ double rwp[WK_GMAX][3];
+  const int tile = blockIdx.x, ki = blockIdx.y, t = blockIdx.z, nk = gridDim.y;
   const phx_wake_kick &w = kk.k[ki];
   const int nf = h->n_filled, head = h->head, cap = h->capacity;
   const int n_turns = nf < w.n_turns_wake ? nf : w.n_turns_wake;
   if (t >= n_turns) return;
+  int mem[WK_GMAX];
+  int nm = 1;
+  mem[0] = ki;
+  if (w.dot_product) {
+    int pos = 0;
+    for (int k = 0; k < ki; ++k) pos += same_shape(kk.k[k], w) ? 1 : 0;
+    if (pos % gmax != 0) return;
+    for (int k = ki + 1; k < nk && nm < gmax; ++k)
+      if (same_shape(kk.k[k], w)) mem[nm++] = k;
+  }
   const int j0 = tile * WK_TILE;
   const int jj = threadIdx.x % WK_TILE, g = threadIdx.x / WK_TILE;
   const int j = j0 + jj;
@@ -128,42 +193,89 @@ __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const Wa
   const double age = h->age[slot];
   const double *tsrc = h->times + (size_t)slot * S;
   const double *ttgt = h->times + (size_t)head * S;
-  const double *a = h->mom + ((size_t)slot * 3 + w.moment) * S;
-  double *abuf = vbuf + (WK_TILE + S - 1);
-  for (int i = threadIdx.x; i < S; i += blockDim.x) abuf[i] = a[i];
-  if (threadIdx.x == 0 && w.fn == PHX_WAKE_RESISTIVE_WALL) rw_prefixes(w, h->beta[slot], rwp);
-  lds_sync();
-  double acc = 0.0;
+  double acc[WK_GMAX];
+#pragma unroll
+  for (int m = 0; m < WK_GMAX; ++m) acc[m] = 0.0;
   if (w.dot_product) {
     // dt[i, j] = t_target[j] - (t_source[i] + age)     (wake_kicks.py:81-85)
-    for (int i = threadIdx.x; i < S; i += blockDim.x) vbuf[i] = tsrc[i] + age;
+    double *tb = vbuf, *ab = vbuf + S;
+    for (int i = threadIdx.x; i < S; i += blockDim.x) tb[i] = tsrc[i] + age;
+    for (int m = 0; m < nm; ++m) {
+      const double *a = h->mom + ((size_t)slot * 3 + kk.k[mem[m]].moment) * S;
+      for (int i = threadIdx.x; i < S; i += blockDim.x) ab[(size_t)m * S + i] = a[i];
+      if (threadIdx.x == 0 && w.fn == PHX_WAKE_RESISTIVE_WALL)
+        rw_prefixes(kk.k[mem[m]], h->beta[slot], rwp[m]);
+    }
     lds_sync();
     if (j < S) {
       const double tj = ttgt[j];
-      for (int i = g; i < S; i += WK_GROUPS) acc = acc + abuf[i] * wake_eval(w, tj - vbuf[i], rwp);
+      if (w.fn == PHX_WAKE_RESISTIVE_WALL) {
+        // the resistive wall inline, the members' constants in registers
+        double p0[WK_GMAX], p1[WK_GMAX], p2[WK_GMAX], zc[WK_GMAX];
+#pragma unroll
+        for (int m = 0; m < WK_GMAX; ++m) {
+          const int km = m < nm ? mem[m] : mem[0];
+          p0[m] = rwp[m][0];
+          p1[m] = rwp[m][1];
+          p2[m] = rwp[m][2];
+          zc[m] = kk.k[km].sqrt_z0c;
+        }
+        const double dmin = w.dt_min_abs, mdt = -w.dt_min_abs, ks = w.k_sigma;
+#pragma unroll 2
+        for (int i = g; i < S; i += WK_GROUPS) {
+          const double dt = tj - tb[i];
+          const double a = dt + dmin;
+          const double dtc = dt > mdt ? mdt : dt;
+          const double root = sqrt(ks / dtc);
+#pragma unroll
+          for (int m = 0; m < WK_GMAX; ++m) {
+            if (m < nm) {
+              const double pre = a > 0.0 ? p2[m] : (a < 0.0 ? p0[m] : (a == 0.0 ? p1[m] : a));
+              acc[m] = acc[m] + ab[(size_t)m * S + i] * ((pre * root) * zc[m]);
+            }
+          }
+        }
+      } else {
+        for (int i = g; i < S; i += WK_GROUPS) {
+          const WakeShape b = wake_shape(w, tj - tb[i]);
+#pragma unroll
+          for (int m = 0; m < WK_GMAX; ++m)
+            if (m < nm)
+              acc[m] = acc[m] + ab[(size_t)m * S + i] * wake_value(kk.k[mem[m]], b, rwp[m]);
+        }
+      }
     }
   } else {
     // dt = concat(t_tgt - s_src[-1], (t_tgt - s_src[0])[1:]) and
     // np.convolve(moments, W, 'valid')[j] = sum_m a[m] W[j + S - 1 - m]
     // (wake_kicks.py:109-116)
+    double *abuf = vbuf + (WK_TILE + S - 1);
+    const double *a = h->mom + ((size_t)slot * 3 + w.moment) * S;
+    for (int i = threadIdx.x; i < S; i += blockDim.x) abuf[i] = a[i];
+    if (threadIdx.x == 0 && w.fn == PHX_WAKE_RESISTIVE_WALL) rw_prefixes(w, h->beta[slot], rwp[0]);
+    lds_sync();
     const double s_last = tsrc[S - 1] + age, s_first = tsrc[0] + age;
     const int p0 = j0, p1 = min(j0 + WK_TILE - 1 + S - 1, 2 * S - 2);
     for (int p = p0 + threadIdx.x; p <= p1; p += blockDim.x) {
       const double dt = p <= S - 1 ? ttgt[p] - s_last : ttgt[p - S + 1] - s_first;
-      vbuf[p - p0] = wake_eval(w, dt, rwp);
+      vbuf[p - p0] = wake_eval(w, dt, rwp[0]);
     }
     lds_sync();
     if (j < S) {
-      for (int m = g; m < S; m += WK_GROUPS) acc = acc + abuf[m] * vbuf[j + S - 1 - m - p0];
+      for (int m = g; m < S; m += WK_GROUPS) acc[0] = acc[0] + abuf[m] * vbuf[j + S - 1 - m - p0];
     }
   }
-  red[threadIdx.x] = acc;
+#pragma unroll
+  for (int m = 0; m < WK_GMAX; ++m) red[m][threadIdx.x] = acc[m];
   lds_sync();
   if (g == 0 && j < S) {
-    double r = red[jj];
-    for (int q = 1; q < WK_GROUPS; ++q) r = r + red[q * WK_TILE + jj];
-    if (T == 1) out[(size_t)ki * S + j] = w.wake_factor * (0.0 + r);
-    else part[((size_t)ki * T + t) * S + j] = r;
+    for (int m = 0; m < nm; ++m) {
+      double r = red[m][jj];
+      for (int q = 1; q < WK_GROUPS; ++q) r = r + red[m][q * WK_TILE + jj];
+      const int km = mem[m];
+      if (T == 1) out[(size_t)km * S + j] = kk.k[km].wake_factor * (0.0 + r);
+      else part[((size_t)km * T + t) * S + j] = r;
+    }
   }
 }
 
@@ -182,6 +294,16 @@ __device__ __forceinline__ void wake_conv_reduce(const phx_wake_hist *h, const W
   out[(size_t)ki * S + j] = w.wake_factor * acc;
 }
 
-// Dynamic LDS of wake_conv_block; the static part is PHX_BLOCK + 3 doubles.
-inline size_t wake_conv_lds_bytes(int S) { return (size_t)(WK_TILE + 2 * S - 1) * sizeof(double); }
-#define WK_STATIC_LDS ((PHX_BLOCK + 3) * sizeof(double))
+// Static LDS of wake_conv_block; its dynamic LDS serves up to gmax kicks per
+// block in the dot-product form.
+#define WK_STATIC_LDS ((WK_GMAX * PHX_BLOCK + 3 * WK_GMAX) * sizeof(double))
+inline size_t wake_conv_lds_bytes(int S, int gmax) {
+  const size_t conv = (size_t)(WK_TILE + 2 * S - 1), dot = (size_t)(1 + gmax) * S;
+  return (conv > dot ? conv : dot) * sizeof(double);
+}
+// The most kicks per block (<= WK_GMAX) whose moments fit the LDS, at least 1.
+inline int wake_conv_gmax(int S, size_t max_lds) {
+  for (int gm = WK_GMAX; gm > 1; --gm)
+    if (wake_conv_lds_bytes(S, gm) + WK_STATIC_LDS <= max_lds) return gm;
+  return 1;
+}

@@ -13,8 +13,9 @@ __global__ void k_hist_push(phx_wake_hist *h, const double *times, const int32_t
 }
 
 __global__ void __launch_bounds__(PHX_BLOCK)
-    k_wake_conv(const phx_wake_hist *h, WakeKicks kk, int S, double *part, int T, double *out) {
-  wake_conv_block(h, kk, S, part, T, out);
+    k_wake_conv(const phx_wake_hist *h, WakeKicks kk, int S, double *part, int T, double *out,
+                int gmax) {
+  wake_conv_block(h, kk, S, part, T, out, gmax);
 }
 
 __global__ void k_wake_conv_reduce(const phx_wake_hist *h, WakeKicks kk, int n_kicks, int S,
@@ -99,11 +100,11 @@ int phx_wake_convolve_ws(const phx_wake_hist *hist, const phx_wake_kick *kicks, 
     const int tk = kicks[i].n_turns_wake < max_turns ? kicks[i].n_turns_wake : max_turns;
     T = tk > T ? tk : T;
   }
-  const size_t lds = wake_conv_lds_bytes(n_slices);
-  if (lds + WK_STATIC_LDS > phx_max_lds_bytes())
+  if (wake_conv_lds_bytes(n_slices, 1) + WK_STATIC_LDS > phx_max_lds_bytes())
     return phx::fail(PHX_E_LIMIT, "phx_wake_convolve: n_slices=%d too large for LDS", n_slices);
-  k_wake_conv<<<dim3(tiles, n_kicks, T), PHX_BLOCK, lds, s>>>(hist, kk, n_slices, part, T,
-                                                             kicks_out);
+  const int gmax = wake_conv_gmax(n_slices, phx_max_lds_bytes());
+  k_wake_conv<<<dim3(tiles, n_kicks, T), PHX_BLOCK, wake_conv_lds_bytes(n_slices, gmax), s>>>(
+      hist, kk, n_slices, part, T, kicks_out, gmax);
   PHX_CHECK_LAUNCH("phx_wake_convolve");
   if (T > 1) {
     const int tot = n_kicks * n_slices;
