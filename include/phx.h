@@ -401,6 +401,9 @@ enum {
    DXDY is [dipole x (xp += k), dipole y (yp += k)], as a round chamber's
    resonator or resistive wall gives. */
 enum { PHX_KICKS_GENERIC = 0, PHX_KICKS_DXDY = 1, PHX_KICKS_COUNT = 2 };
+/* Wake convolution kernels: resistive-wall kicks in the dot-product form
+   (multi-turn memory) have their own; every other kick is GENERAL. */
+enum { PHX_CONV_GENERAL = 1, PHX_CONV_RW_DOT = 2 };
 
 /* Everything the fused engine needs; all pointers are device memory owned
    by the caller.  The engine handles one-turn maps of the form
@@ -421,7 +424,9 @@ typedef struct {
                                            so the last bits of the cuts depend
                                            on it: PHX_STANDALONE_GRID gives the
                                            cuts of phx_slice_cuts bit for bit */
-    int32_t reserved;                   /* 0 */
+    int32_t conv_kinds;                 /* PHX_CONV_* bits: the convolution
+                                           kernels the programs' kicks need;
+                                           0: both (always valid) */
     const phx_turn_program *programs;   /* [n_programs] */
     const int32_t *turn_program;        /* [n_turns_total] -> program index */
     double *stats;                      /* raw moment sums per turn, see below */

@@ -30,12 +30,18 @@ CUTS_FIXED, CUTS_NSIGMA, CUTS_MINMAX = 0, 1, 2
 DET_CHROMA, DET_AMPLITUDE = 1, 2
 LONG_DRIFT, LONG_KICK = 1, 2
 WAKE_RES_TRANSVERSE, WAKE_RES_LONGITUDINAL, WAKE_RESISTIVE_WALL = 1, 2, 3
+
+
+def conv_kind(kick):
+    """The convolution kernel a phx_wake_kick record needs (CONV_*)."""
+    return CONV_RW_DOT if kick.dot_product and kick.fn == WAKE_RESISTIVE_WALL else CONV_GENERAL
 MOM_N, MOM_X, MOM_Y = 0, 1, 2
 PLANE_XP, PLANE_YP, PLANE_DP = 0, 1, 2
 EL_TSEG, EL_LONG, EL_LINMAP = 1, 2, 3
 (SHAPE_GENERIC, SHAPE_RF1_T, SHAPE_RF2_T, SHAPE_T_RF1, SHAPE_T_RF2, SHAPE_LIN_T, SHAPE_T_LIN,
  SHAPE_T) = range(8)
 KICKS_GENERIC, KICKS_DXDY = 0, 1
+CONV_GENERAL, CONV_RW_DOT = 1, 2
 STAT_NV = 33
 
 
@@ -139,7 +145,7 @@ class Engine(C.Structure):
                 ("n_kicks", C.c_int32), ("hist_capacity", C.c_int32),
                 ("record_stats", C.c_int32), ("program_shape", C.c_int32),
                 ("kick_shape", C.c_int32), ("pass_grid", C.c_int32),
-                ("reserved", C.c_int32), ("programs", C.c_void_p),
+                ("conv_kinds", C.c_int32), ("programs", C.c_void_p),
                 ("turn_program", C.c_void_p), ("stats", C.c_void_p),
                 ("turn_counter", C.c_void_p), ("pending", C.c_void_p),
                 ("state", C.c_void_p), ("hist", C.c_void_p), ("z_bins", C.c_void_p),

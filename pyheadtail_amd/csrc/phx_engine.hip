@@ -423,6 +423,15 @@ __global__ void __launch_bounds__(PHX_BLOCK) k_engine_conv(EngineArgs a) {
   wake_conv_block(e.hist, kk, e.n_slices, w.conv_part, a.T, e.kicks, a.gmax);
 }
 
+__global__ void __launch_bounds__(RW_THREADS) k_engine_conv_rw(EngineArgs a) {
+  const phx_engine &e = a.e;
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
+  const phx_turn_program &g = e.programs[e.turn_program[e.state->turn]];
+  const WakeKicks &kk = *reinterpret_cast<const WakeKicks *>(g.kick);
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *e.pending = 1;
+  wake_rw_block(e.hist, kk, e.n_slices, w.conv_part, a.T, e.kicks);
+}
+
 __global__ void k_engine_conv_reduce(EngineArgs a) {
   const phx_engine &e = a.e;
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
@@ -469,6 +478,8 @@ int check_engine(const phx_engine *e, int *n_kicks, int *T) {
   if (e->pass_grid < 0 || e->pass_grid > PHX_MAX_GRID)
     return phx::fail(PHX_E_INVALID, "phx_engine: pass_grid %d not in 0..%d", e->pass_grid,
                      PHX_MAX_GRID);
+  if (e->conv_kinds < 0 || e->conv_kinds > (PHX_CONV_GENERAL | PHX_CONV_RW_DOT))
+    return phx::fail(PHX_E_INVALID, "phx_engine: conv_kinds %d", e->conv_kinds);
   if (e->kick_shape == PHX_KICKS_DXDY && e->n_kicks != 2)
     return phx::fail(PHX_E_INVALID, "phx_engine: kick_shape DXDY needs 2 kicks, not %d",
                      e->n_kicks);
@@ -601,10 +612,17 @@ int enqueue_stage(const EngineArgs &a, int stage, bool has_wake, hipStream_t s) 
     PHX_CHECK_LAUNCH("phx_engine: merge");
     return PHX_OK;
   }
-  const int tiles = (e.n_slices + WK_TILE - 1) / WK_TILE;
-  k_engine_conv<<<dim3(tiles, a.n_kicks, a.T), PHX_BLOCK, wake_conv_lds_bytes(e.n_slices, a.gmax),
-                  s>>>(a);
-  PHX_CHECK_LAUNCH("phx_engine: conv");
+  if (e.conv_kinds == 0 || (e.conv_kinds & PHX_CONV_GENERAL)) {
+    const int tiles = (e.n_slices + WK_TILE - 1) / WK_TILE;
+    k_engine_conv<<<dim3(tiles, a.n_kicks, a.T), PHX_BLOCK,
+                    wake_conv_lds_bytes(e.n_slices, a.gmax), s>>>(a);
+    PHX_CHECK_LAUNCH("phx_engine: conv");
+  }
+  if (e.conv_kinds == 0 || (e.conv_kinds & PHX_CONV_RW_DOT)) {
+    k_engine_conv_rw<<<dim3((e.n_slices + RW_TILE - 1) / RW_TILE * a.T, a.n_kicks), RW_THREADS,
+                       wake_rw_lds_bytes(e.n_slices), s>>>(a);
+    PHX_CHECK_LAUNCH("phx_engine: conv resistive wall");
+  }
   if (a.T > 1) {
     const int tot = a.n_kicks * e.n_slices;
     k_engine_conv_reduce<<<dim3((tot + 255) / 256), 256, 0, s>>>(a);
@@ -625,6 +643,11 @@ int check_lds(const phx_engine *e) {
   if (e->n_slices > 0 && (acc_lds_bytes(e->n_slices, 2) > phx_max_lds_bytes() ||
                           wake_conv_lds_bytes(e->n_slices, 1) + WK_STATIC_LDS > phx_max_lds_bytes()))
     return phx::fail(PHX_E_LIMIT, "phx_engine: n_slices=%d exceeds LDS", e->n_slices);
+  const bool rw = e->conv_kinds == 0 || (e->conv_kinds & PHX_CONV_RW_DOT);
+  if (e->n_slices > 0 && rw && wake_rw_lds_bytes(e->n_slices) + RW_STATIC_LDS > phx_max_lds_bytes())
+    return phx::fail(PHX_E_LIMIT,
+                     "phx_engine: n_slices=%d exceeds LDS of the resistive-wall convolution",
+                     e->n_slices);
   return PHX_OK;
 }
 

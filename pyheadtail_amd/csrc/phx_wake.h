@@ -161,6 +161,12 @@ struct WakeKicks {
   phx_wake_kick k[PHX_MAX_KICKS];
 };
 
+// Resistive-wall kicks in the dot-product form have a kernel of their own
+// (wake_rw_block); the general block serves every other kick.
+__device__ __host__ __forceinline__ bool is_rw_dot(const phx_wake_kick &k) {
+  return k.dot_product && k.fn == PHX_WAKE_RESISTIVE_WALL;
+}
+
 #define WK_GMAX 4  // kicks one block serves
 
 __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const WakeKicks &kk,
@@ -173,6 +179,7 @@ __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const Wa
   __shared__ double rwp[WK_GMAX][3];
   const int tile = blockIdx.x, ki = blockIdx.y, t = blockIdx.z, nk = gridDim.y;
   const phx_wake_kick &w = kk.k[ki];
+  if (is_rw_dot(w)) return;  // wake_rw_block's
   const int nf = h->n_filled, head = h->head, cap = h->capacity;
   const int n_turns = nf < w.n_turns_wake ? nf : w.n_turns_wake;
   if (t >= n_turns) return;
@@ -209,40 +216,12 @@ __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const Wa
     lds_sync();
     if (j < S) {
       const double tj = ttgt[j];
-      if (w.fn == PHX_WAKE_RESISTIVE_WALL) {
-        // the resistive wall inline, the members' constants in registers
-        double p0[WK_GMAX], p1[WK_GMAX], p2[WK_GMAX], zc[WK_GMAX];
+      for (int i = g; i < S; i += WK_GROUPS) {
+        const WakeShape b = wake_shape(w, tj - tb[i]);
 #pragma unroll
-        for (int m = 0; m < WK_GMAX; ++m) {
-          const int km = m < nm ? mem[m] : mem[0];
-          p0[m] = rwp[m][0];
-          p1[m] = rwp[m][1];
-          p2[m] = rwp[m][2];
-          zc[m] = kk.k[km].sqrt_z0c;
-        }
-        const double dmin = w.dt_min_abs, mdt = -w.dt_min_abs, ks = w.k_sigma;
-#pragma unroll 2
-        for (int i = g; i < S; i += WK_GROUPS) {
-          const double dt = tj - tb[i];
-          const double a = dt + dmin;
-          const double dtc = dt > mdt ? mdt : dt;
-          const double root = sqrt(ks / dtc);
-#pragma unroll
-          for (int m = 0; m < WK_GMAX; ++m) {
-            if (m < nm) {
-              const double pre = a > 0.0 ? p2[m] : (a < 0.0 ? p0[m] : (a == 0.0 ? p1[m] : a));
-              acc[m] = acc[m] + ab[(size_t)m * S + i] * ((pre * root) * zc[m]);
-            }
-          }
-        }
-      } else {
-        for (int i = g; i < S; i += WK_GROUPS) {
-          const WakeShape b = wake_shape(w, tj - tb[i]);
-#pragma unroll
-          for (int m = 0; m < WK_GMAX; ++m)
-            if (m < nm)
-              acc[m] = acc[m] + ab[(size_t)m * S + i] * wake_value(kk.k[mem[m]], b, rwp[m]);
-        }
+        for (int m = 0; m < WK_GMAX; ++m)
+          if (m < nm)
+            acc[m] = acc[m] + ab[(size_t)m * S + i] * wake_value(kk.k[mem[m]], b, rwp[m]);
       }
     }
   } else {
@@ -289,9 +268,210 @@ __device__ __forceinline__ void wake_conv_reduce(const phx_wake_hist *h, const W
   const phx_wake_kick &w = kk.k[ki];
   const int nf = h->n_filled;
   const int n_turns = nf < w.n_turns_wake ? nf : w.n_turns_wake;
+  const double *p = part + (size_t)ki * T * S + j;
   double acc = 0.0;
-  for (int q = 0; q < n_turns; ++q) acc = acc + part[((size_t)ki * T + q) * S + j];
+  int q = 0;
+  for (; q + 8 <= n_turns; q += 8) {
+    double v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = p[(size_t)(q + r) * S];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) acc = acc + v[r];
+  }
+  for (; q < n_turns; ++q) acc = acc + p[(size_t)q * S];
   out[(size_t)ki * S + j] = w.wake_factor * acc;
+}
+
+// ---------------------------------------------------------------------------
+// Resistive wall in the dot-product form (multi-turn memory).  For dt < -|dt_min|
+// the function is W = pre0 * sqrt(k_sigma / dt) * sqrt(Z0 c), pre0 the
+// prefix of class 0 (rw_prefixes), k_sigma = -mu_r/(pi sigma) < 0, so
+//   W = coef * rsqrt(-dt),    coef = pre0 * sqrt(-k_sigma) * sqrt(Z0 c),
+// at dt = -|dt_min| it is half of that and above it zero (pre1 = pre0 / 2
+// exactly, pre2 = 0).  The kicks of one shape (same_shape) differ only in
+// coef and in the source moment (n, n x, n y), so a block sums
+//   acc_u[j] = sum_i m_u[i] * sel(dt_ij) * rsqrt(x_ij)
+// once per distinct moment u and writes coef_k * acc_u(k) for every kick k.
+// One block = (RW_TILE targets, source turn, shape group): the block stages
+// the turn's source times and moments in LDS, each wave runs over its share
+// of the sources (the same source in all lanes: LDS broadcasts) for two
+// targets per lane.  Blocks whose every pair has dt < -|dt_min| (all sources
+// of turns gone by, decided from the extreme times: rounding is monotonic)
+// skip the class selection.  The sums run in a fixed order: each wave in
+// source order, then the waves in order.
+#define RW_WAVES 16
+#define RW_THREADS (RW_WAVES * 64)
+#define RW_JT 4
+#define RW_TILE (RW_JT * 64)
+
+// 1/sqrt(x), x > 0: the hardware estimate y (relative error e0 < 2^-22) and
+// one third-order step y (1 + e/2 + 3e^2/8), e = 1 - x y^2, which leaves
+// O(e0^3) plus rounding: within a few ulp.
+__device__ __forceinline__ double rw_rsqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x, y * y, 1.0);
+  return fma(y, e * fma(0.375, e, 0.5), y);
+}
+
+// The class-0 prefix, rw_prefixes' chain for c = 0.
+__device__ __forceinline__ double rw_prefix0(const phx_wake_kick &w, double beta_src) {
+  double t = w.yokoya * -2.0;
+  t = t / 2.0;
+  t = t * sqrt(beta_src);
+  t = t * w.L;
+  t = t / PHX_PI;
+  return t / w.b3;
+}
+
+template <bool UNIFORM>
+__device__ __forceinline__ double rw_pair(double tja, double ts, double dmin) {
+  const double dt = tja - ts;
+  if (UNIFORM) return rw_rsqrt(-dt);
+  // dt.clip(max=-|dt_min|) and the class of sign(dt + |dt_min|)
+  const double a = dt + dmin;
+  const double x = dt > -dmin ? dmin : -dt;
+  const double sel = a < 0.0 ? 1.0 : (a == 0.0 ? 0.5 : (a > 0.0 ? 0.0 : a));
+  return sel * rw_rsqrt(x);
+}
+
+// Sources [ib, ie) of the LDS arrays, in order, for RW_JT targets.
+template <int NU, bool UNIFORM>
+__device__ __forceinline__ void rw_sum(const double *ts, const double *m0, const double *m1,
+                                       const double *m2, int ib, int ie,
+                                       const double (&tja)[RW_JT], double dmin,
+                                       double (&acc)[3][RW_JT]) {
+#pragma unroll 2
+  for (int i = ib; i < ie; ++i) {
+    const double t = ts[i];
+    const double a0 = m0[i], a1 = NU > 1 ? m1[i] : 0.0, a2 = NU > 2 ? m2[i] : 0.0;
+#pragma unroll
+    for (int q = 0; q < RW_JT; ++q) {
+      const double w = rw_pair<UNIFORM>(tja[q], t, dmin);
+      acc[0][q] = fma(a0, w, acc[0][q]);
+      if (NU > 1) acc[1][q] = fma(a1, w, acc[1][q]);
+      if (NU > 2) acc[2][q] = fma(a2, w, acc[2][q]);
+    }
+  }
+}
+
+__device__ __forceinline__ double wave_min_nan(double v, int &bad) {
+  bad |= v != v;
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+#define RW_STATIC_LDS (RW_WAVES * sizeof(double))
+// Dynamic LDS of wake_rw_block: the sources, then (reused) the wave sums of
+// one moment at a time.
+inline size_t wake_rw_lds_bytes(int S) {
+  const size_t src = (size_t)4 * S, red = (size_t)RW_WAVES * RW_TILE;
+  return (src > red ? src : red) * sizeof(double);
+}
+
+__device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const WakeKicks &kk, int S,
+                                              double *part, int T, double *out) {
+  extern __shared__ double rw_lds[];
+  __shared__ double wmin[RW_WAVES];
+  // blockIdx.x = tile + tiles * turn, so that consecutive blocks (which go to
+  // different XCDs) cover different turns; blockIdx.y = kick
+  const int tiles = (S + RW_TILE - 1) / RW_TILE;
+  const int tile = blockIdx.x % tiles, t = blockIdx.x / tiles, ki = blockIdx.y, nk = gridDim.y;
+  const phx_wake_kick &w = kk.k[ki];
+  if (!is_rw_dot(w)) return;
+  const int nf = h->n_filled, head = h->head, cap = h->capacity;
+  const int n_turns = nf < w.n_turns_wake ? nf : w.n_turns_wake;
+  if (t >= n_turns) return;
+  for (int k = 0; k < ki; ++k)
+    if (same_shape(kk.k[k], w)) return;  // the group's first kick serves it
+  unsigned mask = 0;                     // moments the group reads
+  for (int k = ki; k < nk; ++k)
+    if (same_shape(kk.k[k], w)) mask |= 1u << kk.k[k].moment;
+  const int nu = __popc(mask);
+  const int mo0 = __ffs(mask) - 1;
+  const int mo1 = nu > 1 ? __ffs(mask & ~(1u << mo0)) - 1 : mo0;
+  const int mo2 = nu > 2 ? 2 : mo1;
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int slot = ring_slot(head, t, cap);
+  const double age = h->age[slot];
+  const double *tsrc = h->times + (size_t)slot * S;
+  const double *mom = h->mom + (size_t)slot * 3 * S;
+  double *ts = rw_lds, *m0 = rw_lds + S, *m1 = rw_lds + 2 * S, *m2 = rw_lds + 3 * S;
+  int bad = 0;
+  double mn = tsrc[0];
+  for (int i = threadIdx.x; i < S; i += RW_THREADS) {
+    const double v = tsrc[i];
+    ts[i] = v;
+    bad |= v != v;
+    mn = v < mn ? v : mn;
+    m0[i] = mom[(size_t)mo0 * S + i];
+    if (nu > 1) m1[i] = mom[(size_t)mo1 * S + i];
+    if (nu > 2) m2[i] = mom[(size_t)mo2 * S + i];
+  }
+  // dt = (t_target - age) - t_source: the age leaves the pair loop
+  double tja[RW_JT];
+  double mx = -INFINITY;
+#pragma unroll
+  for (int q = 0; q < RW_JT; ++q) {
+    const int j = tile * RW_TILE + q * 64 + lane;
+    tja[q] = h->times[(size_t)head * S + (j < S ? j : S - 1)] - age;
+    bad |= tja[q] != tja[q];
+    mx = tja[q] > mx ? tja[q] : mx;
+  }
+  mn = wave_min_nan(mn, bad);
+  const double tmax = -wave_min_nan(-mx, bad);
+  if (lane == 0) wmin[wave] = mn;
+  bad = __syncthreads_or(bad);
+  for (int q = 0; q < RW_WAVES; ++q) mn = wmin[q] < mn ? wmin[q] : mn;
+  const bool uniform = !bad && (tmax - mn) + w.dt_min_abs < 0.0;
+
+  const int chunk = (S + RW_WAVES - 1) / RW_WAVES;
+  const int ib = min(wave * chunk, S), ie = min(ib + chunk, S);
+  const double dmin = w.dt_min_abs;
+  double acc[3][RW_JT] = {};
+  if (uniform) {
+    if (nu == 1) rw_sum<1, true>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
+    else if (nu == 2) rw_sum<2, true>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
+    else rw_sum<3, true>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
+  } else {
+    if (nu == 1) rw_sum<1, false>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
+    else if (nu == 2) rw_sum<2, false>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
+    else rw_sum<3, false>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
+  }
+  // the sources' LDS takes the wave sums, one moment at a time; thread
+  // r < RW_TILE finishes target j0 + r
+  double *red = rw_lds;  // [wave][RW_TILE]
+  double tot[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    if (u >= nu) break;
+    lds_sync();
+#pragma unroll
+    for (int q = 0; q < RW_JT; ++q) red[wave * RW_TILE + q * 64 + lane] = acc[u][q];
+    lds_sync();
+    if (threadIdx.x < RW_TILE) {
+      double r = red[threadIdx.x];
+      for (int q = 1; q < RW_WAVES; ++q) r = r + red[q * RW_TILE + threadIdx.x];
+      tot[u] = r;
+    }
+  }
+  if (threadIdx.x >= RW_TILE) return;
+  const int j = tile * RW_TILE + threadIdx.x;
+  if (j >= S) return;
+  const double beta = h->beta[slot], root = sqrt(-w.k_sigma);
+  for (int k = ki; k < nk; ++k) {
+    const phx_wake_kick &kw = kk.k[k];
+    if (!same_shape(kw, w)) continue;
+    const int u = __popc(mask & ((1u << kw.moment) - 1u));
+    const double sum = u == 0 ? tot[0] : (u == 1 ? tot[1] : tot[2]);
+    const double v = ((rw_prefix0(kw, beta) * root) * kw.sqrt_z0c) * sum;
+    if (T == 1) out[(size_t)k * S + j] = kw.wake_factor * (0.0 + v);
+    else part[((size_t)k * T + t) * S + j] = v;
+  }
 }
 
 // Static LDS of wake_conv_block; its dynamic LDS serves up to gmax kicks per

@@ -18,6 +18,11 @@ __global__ void __launch_bounds__(PHX_BLOCK)
   wake_conv_block(h, kk, S, part, T, out, gmax);
 }
 
+__global__ void __launch_bounds__(RW_THREADS)
+    k_wake_rw(const phx_wake_hist *h, WakeKicks kk, int S, double *part, int T, double *out) {
+  wake_rw_block(h, kk, S, part, T, out);
+}
+
 __global__ void k_wake_conv_reduce(const phx_wake_hist *h, WakeKicks kk, int n_kicks, int S,
                                    const double *part, int T, double *out) {
   wake_conv_reduce(h, kk, n_kicks, S, part, T, out);
@@ -103,9 +108,20 @@ int phx_wake_convolve_ws(const phx_wake_hist *hist, const phx_wake_kick *kicks, 
   if (wake_conv_lds_bytes(n_slices, 1) + WK_STATIC_LDS > phx_max_lds_bytes())
     return phx::fail(PHX_E_LIMIT, "phx_wake_convolve: n_slices=%d too large for LDS", n_slices);
   const int gmax = wake_conv_gmax(n_slices, phx_max_lds_bytes());
-  k_wake_conv<<<dim3(tiles, n_kicks, T), PHX_BLOCK, wake_conv_lds_bytes(n_slices, gmax), s>>>(
-      hist, kk, n_slices, part, T, kicks_out, gmax);
-  PHX_CHECK_LAUNCH("phx_wake_convolve");
+  bool general = false, rw = false;
+  for (int i = 0; i < n_kicks; ++i) (is_rw_dot(kicks[i]) ? rw : general) = true;
+  if (rw && wake_rw_lds_bytes(n_slices) + RW_STATIC_LDS > phx_max_lds_bytes())
+    return phx::fail(PHX_E_LIMIT, "phx_wake_convolve: n_slices=%d too large for LDS", n_slices);
+  if (general) {
+    k_wake_conv<<<dim3(tiles, n_kicks, T), PHX_BLOCK, wake_conv_lds_bytes(n_slices, gmax), s>>>(
+        hist, kk, n_slices, part, T, kicks_out, gmax);
+    PHX_CHECK_LAUNCH("phx_wake_convolve");
+  }
+  if (rw) {
+    k_wake_rw<<<dim3((n_slices + RW_TILE - 1) / RW_TILE * T, n_kicks), RW_THREADS,
+                wake_rw_lds_bytes(n_slices), s>>>(hist, kk, n_slices, part, T, kicks_out);
+    PHX_CHECK_LAUNCH("phx_wake_convolve: resistive wall");
+  }
   if (T > 1) {
     const int tot = n_kicks * n_slices;
     k_wake_conv_reduce<<<dim3((tot + 255) / 256), 256, 0, s>>>(hist, kk, n_kicks, n_slices, part,

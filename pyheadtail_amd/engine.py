@@ -102,6 +102,7 @@ class FusedOneTurnMap(Element):
         self._engine = None
         self.last_stats = None
         self._shape = (_phx.SHAPE_GENERIC, _phx.KICKS_GENERIC)
+        self._conv_kinds = 0
         # False: always the generic pass (the same arithmetic, for checks)
         self.specialise = bool(specialise)
         # blocks of the particle pass (0: as many as stay resident); the cuts
@@ -194,6 +195,8 @@ class FusedOneTurnMap(Element):
             p = self._program(state)
             if t == 0 and self.specialise:
                 self._shape = self._shapes(p, self.wake)
+            for k in range(p.n_kicks):
+                self._conv_kinds |= _phx.conv_kind(p.kick[k])
             key = bytes(memoryview(p).cast("B"))
             if key not in index:
                 index[key] = len(progs)
@@ -257,6 +260,7 @@ class FusedOneTurnMap(Element):
         eng.record_stats = 1 if self.record_stats else 0
         eng.program_shape, eng.kick_shape = self._shape
         eng.pass_grid = self.pass_grid
+        eng.conv_kinds = self._conv_kinds
         eng.programs = b['programs'].data_ptr()
         eng.turn_program = b['table'].data_ptr()
         eng.stats = b['stats'].data_ptr()

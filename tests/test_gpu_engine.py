@@ -226,3 +226,36 @@ def test_specialised_pass_equals_generic(case, rec):
         np.testing.assert_array_equal(outs[0][k], outs[1][k], err_msg=k)
     if rec:
         np.testing.assert_array_equal(stats[0], stats[1])
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_engine_resistive_wall_memory_equals_elements(mixed):
+    """Multi-turn resistive wall (its own convolution kernel), alone and next
+    to a multi-turn resonator (the general kernel): the engine equals the
+    element path bit for bit and launches only the kernels the kicks need."""
+    _print_case("engine vs elements, 8-turn resistive wall, mixed=%s" % mixed)
+    from pyheadtail_amd import _phx
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.impedances.wakes import (WakeField, CircularResonator,
+                                                 ParallelHorizontalPlatesResistiveWall)
+
+    def make():
+        srcs = [ParallelHorizontalPlatesResistiveWall(0.02, W.CIRCUMFERENCE, 5.9e7, 1e-12,
+                                                      n_turns_wake=8)]
+        if mixed:
+            srcs.append(CircularResonator(1e6, 1e9, 1, n_turns_wake=4))
+        return nonlinear_machine(wake=False) + [WakeField(UniformBinSlicer(100, n_sigma_z=3),
+                                                          *srcs)]
+    ob = W.nonlinear_bunch(40000, seed=17)
+    a, b = to_product(ob), to_product(ob)
+    m_el = make()
+    for _ in range(12):
+        for el in m_el:
+            el.track(a)
+    eng = _fused(make(), graph_turns=4, pass_grid=_standalone_grid())
+    eng.track_turns(b, 12)
+    want = _phx.CONV_RW_DOT | (_phx.CONV_GENERAL if mixed else 0)
+    assert eng._conv_kinds == want
+    ga, gb = to_numpy(a), to_numpy(b)
+    for k in COORDS:
+        np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)
