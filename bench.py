@@ -17,6 +17,14 @@ launch stream.  The CPU baseline is the oracle's NumPy restatement of the
 reference CPU path on one host core, timed on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 50]
+
+--workload rw64 measures instead the multi-turn resistive-wall workload:
+1e6 macroparticles, LinearMap (alpha_c = 1.9e-3, Q_s = 0.017), one smooth
+transverse segment, 1000 slices over +-3 sigma_z and a parallel-plates
+resistive wall (b = 0.02 m, L = 6911.5 m, sigma_c = 5.9e7 S/m, dt_min =
+1e-12 s, dipolar and quadrupolar x/y kicks) with 64 turns of memory, in FP64
+and with the FP32 convolution, and the FP32 kicks' error against FP64 per
+kick component on the bunch's own 64-turn history.
 """
 import argparse
 import json
@@ -47,6 +55,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-turns", type=int, default=100)
     ap.add_argument("--graph-turns", type=int, default=10)
+    ap.add_argument("--workload", choices=("resonator", "rw64"), default="resonator")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of the pass's algorithmic bytes "
                          "(48 B read + 48 B written per macroparticle), for scaling the "
@@ -132,6 +141,77 @@ def build_machine(p0_ref):
     return mach
 
 
+def rw64(args):
+    """The multi-turn resistive-wall workload in FP64 and FP32 (one JSON line)."""
+    import numpy as np
+    import torch
+    torch.cuda.set_device(0)
+    from scipy.constants import e, m_p
+    from pyheadtail_amd import _phx
+    from pyheadtail_amd.engine import FusedOneTurnMap
+    from pyheadtail_amd.machines.synchrotron import Synchrotron
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.impedances.wakes import WakeField, ParallelHorizontalPlatesResistiveWall
+    n, S, T = args.n, 1000 if args.slices == 500 else args.slices, 64
+    steps = min(args.steps, 500)
+    mach = Synchrotron(optics_mode='smooth', charge=e, mass=m_p, p0=p0_of_gamma(27.7),
+                       circumference=6911.5, n_segments=1, beta_x=54.6, beta_y=54.6,
+                       D_x=0., D_y=0., accQ_x=20.13, accQ_y=20.18,
+                       alpha_mom_compaction=1.9e-3, longitudinal_mode='linear', Q_s=0.017)
+    res, engines = {}, {}
+    for prec in ('fp64', 'fp32'):
+        wake = WakeField(UniformBinSlicer(S, n_sigma_z=3),
+                         ParallelHorizontalPlatesResistiveWall(0.02, 6911.5, 5.9e7, 1e-12,
+                                                               n_turns_wake=T),
+                         precision=prec)
+        np.random.seed(SEED)
+        bunch = mach.generate_6D_Gaussian_bunch(n, INTENSITY, EPSN, EPSN, SIGMA_Z)
+        eng = FusedOneTurnMap(list(mach.one_turn_map) + [wake], graph_turns=args.graph_turns)
+        eng.track_turns(bunch, T + max(args.warmup, args.graph_turns))  # history filled
+        bunch.x
+        torch.cuda.synchronize()
+        stage_ms = eng.timed_turns(bunch, min(args.stage_turns, 50))
+        bunch.x
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.track_turns(bunch, steps)
+        bunch.x
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[prec] = {"value": n * steps / dt, "ms_per_step": dt / steps * 1e3,
+                     "stage_ms": {"pass": stage_ms[0], "hist": stage_ms[1],
+                                  "conv": stage_ms[2]}}
+        engines[prec] = (eng, wake, bunch)
+    # FP32 against FP64 on the FP64 run's own 64-turn history
+    eng, wake, bunch = engines['fp64']
+    ring = eng._bufs['history']
+    ring.n_pushed = T
+    recs = [k.record(bunch) for k in wake.wake_kicks]
+    k64 = ring.convolve(recs).cpu().numpy()
+    for r in recs:
+        r.precision = _phx.PREC_FP32
+    k32 = ring.convolve(recs).cpu().numpy()
+    err = {}
+    for i, k in enumerate(wake.wake_kicks):
+        d = k32[i] - k64[i]
+        err[type(k).__name__] = {
+            "max_rel": float(np.max(np.abs(d)) / np.max(np.abs(k64[i]))),
+            "rms_rel": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(k64[i] ** 2)))}
+    out = {
+        "metric": "macroparticle-turns/s, multi-turn resistive-wall wake",
+        "value": res['fp64']['value'], "unit": "macroparticle-turns/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": res['fp64']['ms_per_step'],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded 6D Gaussian bunch)",
+        "config": {"workload": "multi-turn resistive wall: %d macroparticles, LinearMap, %d "
+                               "slices +-3 sigma_z, parallel-plates resistive wall (4 kicks), "
+                               "%d turns of memory" % (n, S, T),
+                   "macroparticles": n, "n_slices": S, "n_turns_wake": T},
+        "fp64": res['fp64'], "fp32": res['fp32'], "fp32_kick_error_vs_fp64": err,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -141,6 +221,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.n)
+        return
+    if args.workload == "rw64":
+        rw64(args)
         return
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_turns)
 

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PHX_ABI_VERSION 1
+#define PHX_ABI_VERSION 2
 
 enum {
     PHX_OK = 0,
@@ -291,6 +291,12 @@ enum {
 };
 enum { PHX_MOM_N = 0, PHX_MOM_X = 1, PHX_MOM_Y = 2 };
 enum { PHX_PLANE_XP = 0, PHX_PLANE_YP = 1, PHX_PLANE_DP = 2 };
+/* Arithmetic of a multi-turn (dot-product) resistive-wall convolution:
+   FP64 throughout, or FP32 wake values (from the FP64 time differences) and
+   FP32 products summed in FP32 over each run of at most ceil(n_slices/8)
+   sources, those partial sums added in FP64.  Every other convolution is
+   FP64 whatever the field says. */
+enum { PHX_PREC_FP64 = 0, PHX_PREC_FP32 = 1 };
 
 typedef struct {
     int32_t fn;               /* PHX_WAKE_* */
@@ -299,6 +305,8 @@ typedef struct {
     int32_t multiply;         /* -1: += k[idx];  0: += k[idx]*x;  2: += k[idx]*y */
     int32_t n_turns_wake;
     int32_t dot_product;      /* 1: _convolution_dot_product, 0: np.convolve */
+    int32_t precision;        /* PHX_PREC_* */
+    int32_t reserved;         /* 0 */
     /* resonator: branch by Q (>0.5, ==0.5, <0.5); prefix = host-evaluated
        Yokoya*R*omega^2/(Q*omegabar) (transverse) or R (longitudinal) */
     double Q, prefix, alpha, omegabar, alpha_over_omegabar;

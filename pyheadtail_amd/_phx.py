@@ -42,6 +42,7 @@ EL_TSEG, EL_LONG, EL_LINMAP = 1, 2, 3
  SHAPE_T) = range(8)
 KICKS_GENERIC, KICKS_DXDY = 0, 1
 CONV_GENERAL, CONV_RW_DOT = 1, 2
+PREC_FP64, PREC_FP32 = 0, 1
 STAT_NV = 33
 
 
@@ -115,7 +116,8 @@ class LinMap(C.Structure):
 class WakeKick(C.Structure):
     _fields_ = [("fn", C.c_int32), ("moment", C.c_int32), ("plane", C.c_int32),
                 ("multiply", C.c_int32), ("n_turns_wake", C.c_int32),
-                ("dot_product", C.c_int32),
+                ("dot_product", C.c_int32), ("precision", C.c_int32),
+                ("reserved", C.c_int32),
                 ("Q", C.c_double), ("prefix", C.c_double), ("alpha", C.c_double),
                 ("omegabar", C.c_double), ("alpha_over_omegabar", C.c_double),
                 ("yokoya", C.c_double), ("L", C.c_double), ("b3", C.c_double),

@@ -428,8 +428,8 @@ __global__ void __launch_bounds__(RW_THREADS) k_engine_conv_rw(EngineArgs a) {
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
   const phx_turn_program &g = e.programs[e.turn_program[e.state->turn]];
   const WakeKicks &kk = *reinterpret_cast<const WakeKicks *>(g.kick);
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *e.pending = 1;
-  wake_rw_block(e.hist, kk, e.n_slices, w.conv_part, a.T, e.kicks);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *e.pending = 1;
+  wake_rw_block(e.hist, kk, a.n_kicks, e.n_slices, w.conv_part, a.T, e.kicks);
 }
 
 __global__ void k_engine_conv_reduce(EngineArgs a) {
@@ -619,7 +619,7 @@ int enqueue_stage(const EngineArgs &a, int stage, bool has_wake, hipStream_t s) 
     PHX_CHECK_LAUNCH("phx_engine: conv");
   }
   if (e.conv_kinds == 0 || (e.conv_kinds & PHX_CONV_RW_DOT)) {
-    k_engine_conv_rw<<<dim3((e.n_slices + RW_TILE - 1) / RW_TILE * a.T, a.n_kicks), RW_THREADS,
+    k_engine_conv_rw<<<dim3((e.n_slices + RW_TILE - 1) / RW_TILE * a.T), RW_THREADS,
                        wake_rw_lds_bytes(e.n_slices), s>>>(a);
     PHX_CHECK_LAUNCH("phx_engine: conv resistive wall");
   }

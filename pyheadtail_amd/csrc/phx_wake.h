@@ -78,7 +78,8 @@ __device__ __forceinline__ double wake_eval(const phx_wake_kick &w, double dt,
 // Kicks whose functions share the shape: same function, same parameters
 // apart from the Yokoya factor, same memory and convolution form.
 __device__ __forceinline__ bool same_shape(const phx_wake_kick &a, const phx_wake_kick &b) {
-  if (a.fn != b.fn || a.n_turns_wake != b.n_turns_wake || a.dot_product != b.dot_product)
+  if (a.fn != b.fn || a.n_turns_wake != b.n_turns_wake || a.dot_product != b.dot_product ||
+      a.precision != b.precision)
     return false;
   if (a.fn == PHX_WAKE_RES_TRANSVERSE)
     return a.Q == b.Q && a.alpha == b.alpha && a.omegabar == b.omegabar;
@@ -292,16 +293,16 @@ __device__ __forceinline__ void wake_conv_reduce(const phx_wake_hist *h, const W
 // coef and in the source moment (n, n x, n y), so a block sums
 //   acc_u[j] = sum_i m_u[i] * sel(dt_ij) * rsqrt(x_ij)
 // once per distinct moment u and writes coef_k * acc_u(k) for every kick k.
-// One block = (RW_TILE targets, source turn, shape group): the block stages
+// One block = (RW_TILE targets, source turn), for each shape group: the block stages
 // the turn's source times and moments in LDS, each wave runs over its share
 // of the sources (the same source in all lanes: LDS broadcasts) for two
 // targets per lane.  Blocks whose every pair has dt < -|dt_min| (all sources
 // of turns gone by, decided from the extreme times: rounding is monotonic)
 // skip the class selection.  The sums run in a fixed order: each wave in
 // source order, then the waves in order.
-#define RW_WAVES 16
+#define RW_WAVES 8
 #define RW_THREADS (RW_WAVES * 64)
-#define RW_JT 4
+#define RW_JT 2
 #define RW_TILE (RW_JT * 64)
 
 // 1/sqrt(x), x > 0: the hardware estimate y (relative error e0 < 2^-22) and
@@ -335,22 +336,70 @@ __device__ __forceinline__ double rw_pair(double tja, double ts, double dmin) {
 }
 
 // Sources [ib, ie) of the LDS arrays, in order, for RW_JT targets.
-template <int NU, bool UNIFORM>
+template <int NU, bool UNIFORM, bool F32>
 __device__ __forceinline__ void rw_sum(const double *ts, const double *m0, const double *m1,
                                        const double *m2, int ib, int ie,
                                        const double (&tja)[RW_JT], double dmin,
                                        double (&acc)[3][RW_JT]) {
-#pragma unroll 2
+  if (!F32) {
+#pragma unroll 4
+    for (int i = ib; i < ie; ++i) {
+      const double t = ts[i];
+      const double a0 = m0[i], a1 = NU > 1 ? m1[i] : 0.0, a2 = NU > 2 ? m2[i] : 0.0;
+#pragma unroll
+      for (int q = 0; q < RW_JT; ++q) {
+        const double w = rw_pair<UNIFORM>(tja[q], t, dmin);
+        acc[0][q] = fma(a0, w, acc[0][q]);
+        if (NU > 1) acc[1][q] = fma(a1, w, acc[1][q]);
+        if (NU > 2) acc[2][q] = fma(a2, w, acc[2][q]);
+      }
+    }
+    return;
+  }
+  // FP32: the wake value rsqrt(x) from x rounded to FP32 (the difference dt
+  // itself is taken in FP64), the moments rounded to FP32, FP32 sums
+  float fa[3][RW_JT] = {};
+#pragma unroll 4
   for (int i = ib; i < ie; ++i) {
     const double t = ts[i];
-    const double a0 = m0[i], a1 = NU > 1 ? m1[i] : 0.0, a2 = NU > 2 ? m2[i] : 0.0;
+    const float a0 = (float)m0[i], a1 = NU > 1 ? (float)m1[i] : 0.f, a2 = NU > 2 ? (float)m2[i] : 0.f;
 #pragma unroll
     for (int q = 0; q < RW_JT; ++q) {
-      const double w = rw_pair<UNIFORM>(tja[q], t, dmin);
-      acc[0][q] = fma(a0, w, acc[0][q]);
-      if (NU > 1) acc[1][q] = fma(a1, w, acc[1][q]);
-      if (NU > 2) acc[2][q] = fma(a2, w, acc[2][q]);
+      const double dt = tja[q] - t;
+      float w;
+      if (UNIFORM) {
+        w = __builtin_amdgcn_rsqf((float)(-dt));
+      } else {
+        const double a = dt + dmin;
+        const double x = dt > -dmin ? dmin : -dt;
+        const float sel = a < 0.0 ? 1.f : (a == 0.0 ? 0.5f : (a > 0.0 ? 0.f : (float)a));
+        w = sel * __builtin_amdgcn_rsqf((float)x);
+      }
+      fa[0][q] = fmaf(a0, w, fa[0][q]);
+      if (NU > 1) fa[1][q] = fmaf(a1, w, fa[1][q]);
+      if (NU > 2) fa[2][q] = fmaf(a2, w, fa[2][q]);
     }
+  }
+#pragma unroll
+  for (int u = 0; u < 3; ++u)
+#pragma unroll
+    for (int q = 0; q < RW_JT; ++q) acc[u][q] = (double)fa[u][q];
+}
+
+template <bool F32>
+__device__ __forceinline__ void rw_dispatch(int nu, bool uniform, const double *ts,
+                                            const double *m0, const double *m1,
+                                            const double *m2, int ib, int ie,
+                                            const double (&tja)[RW_JT], double dmin,
+                                            double (&acc)[3][RW_JT]) {
+  if (uniform) {
+    if (nu == 1) rw_sum<1, true, F32>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
+    else if (nu == 2) rw_sum<2, true, F32>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
+    else rw_sum<3, true, F32>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
+  } else {
+    if (nu == 1) rw_sum<1, false, F32>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
+    else if (nu == 2) rw_sum<2, false, F32>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
+    else rw_sum<3, false, F32>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
   }
 }
 
@@ -371,21 +420,15 @@ inline size_t wake_rw_lds_bytes(int S) {
   return (src > red ? src : red) * sizeof(double);
 }
 
-__device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const WakeKicks &kk, int S,
-                                              double *part, int T, double *out) {
+// The group of kick ki (its first kick) for targets [tile * RW_TILE, +RW_TILE)
+// and source turn t; called by all threads of the block.
+__device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const WakeKicks &kk, int nk,
+                                              int S, double *part, int T, double *out, int ki,
+                                              int tile, int t) {
   extern __shared__ double rw_lds[];
   __shared__ double wmin[RW_WAVES];
-  // blockIdx.x = tile + tiles * turn, so that consecutive blocks (which go to
-  // different XCDs) cover different turns; blockIdx.y = kick
-  const int tiles = (S + RW_TILE - 1) / RW_TILE;
-  const int tile = blockIdx.x % tiles, t = blockIdx.x / tiles, ki = blockIdx.y, nk = gridDim.y;
   const phx_wake_kick &w = kk.k[ki];
-  if (!is_rw_dot(w)) return;
-  const int nf = h->n_filled, head = h->head, cap = h->capacity;
-  const int n_turns = nf < w.n_turns_wake ? nf : w.n_turns_wake;
-  if (t >= n_turns) return;
-  for (int k = 0; k < ki; ++k)
-    if (same_shape(kk.k[k], w)) return;  // the group's first kick serves it
+  const int head = h->head, cap = h->capacity;
   unsigned mask = 0;                     // moments the group reads
   for (int k = ki; k < nk; ++k)
     if (same_shape(kk.k[k], w)) mask |= 1u << kk.k[k].moment;
@@ -433,15 +476,10 @@ __device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const Wake
   const int ib = min(wave * chunk, S), ie = min(ib + chunk, S);
   const double dmin = w.dt_min_abs;
   double acc[3][RW_JT] = {};
-  if (uniform) {
-    if (nu == 1) rw_sum<1, true>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
-    else if (nu == 2) rw_sum<2, true>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
-    else rw_sum<3, true>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
-  } else {
-    if (nu == 1) rw_sum<1, false>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
-    else if (nu == 2) rw_sum<2, false>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
-    else rw_sum<3, false>(ts, m0, m1, m2, ib, ie, tja, dmin, acc);
-  }
+  if (w.precision == PHX_PREC_FP32)
+    rw_dispatch<true>(nu, uniform, ts, m0, m1, m2, ib, ie, tja, dmin, acc);
+  else
+    rw_dispatch<false>(nu, uniform, ts, m0, m1, m2, ib, ie, tja, dmin, acc);
   // the sources' LDS takes the wave sums, one moment at a time; thread
   // r < RW_TILE finishes target j0 + r
   double *red = rw_lds;  // [wave][RW_TILE]
@@ -459,18 +497,39 @@ __device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const Wake
       tot[u] = r;
     }
   }
-  if (threadIdx.x >= RW_TILE) return;
   const int j = tile * RW_TILE + threadIdx.x;
-  if (j >= S) return;
-  const double beta = h->beta[slot], root = sqrt(-w.k_sigma);
-  for (int k = ki; k < nk; ++k) {
-    const phx_wake_kick &kw = kk.k[k];
-    if (!same_shape(kw, w)) continue;
-    const int u = __popc(mask & ((1u << kw.moment) - 1u));
-    const double sum = u == 0 ? tot[0] : (u == 1 ? tot[1] : tot[2]);
-    const double v = ((rw_prefix0(kw, beta) * root) * kw.sqrt_z0c) * sum;
-    if (T == 1) out[(size_t)k * S + j] = kw.wake_factor * (0.0 + v);
-    else part[((size_t)k * T + t) * S + j] = v;
+  if (threadIdx.x < RW_TILE && j < S) {
+    const double beta = h->beta[slot], root = sqrt(-w.k_sigma);
+    for (int k = ki; k < nk; ++k) {
+      const phx_wake_kick &kw = kk.k[k];
+      if (!same_shape(kw, w)) continue;
+      const int u = __popc(mask & ((1u << kw.moment) - 1u));
+      const double sum = u == 0 ? tot[0] : (u == 1 ? tot[1] : tot[2]);
+      const double v = ((rw_prefix0(kw, beta) * root) * kw.sqrt_z0c) * sum;
+      if (T == 1) out[(size_t)k * S + j] = kw.wake_factor * (0.0 + v);
+      else part[((size_t)k * T + t) * S + j] = v;
+    }
+  }
+  lds_sync();  // the LDS is the next group's
+}
+
+// One block = (RW_TILE targets, source turn): blockIdx.x = tile + tiles *
+// turn, so that consecutive blocks (which go to different XCDs) cover
+// different turns.  The block serves every resistive-wall group (in kick
+// order) whose memory reaches turn t.
+__device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const WakeKicks &kk, int nk,
+                                              int S, double *part, int T, double *out) {
+  const int tiles = (S + RW_TILE - 1) / RW_TILE;
+  const int tile = blockIdx.x % tiles, t = blockIdx.x / tiles;
+  const int nf = h->n_filled;
+  for (int ki = 0; ki < nk; ++ki) {
+    const phx_wake_kick &w = kk.k[ki];
+    if (!is_rw_dot(w)) continue;
+    const int n_turns = nf < w.n_turns_wake ? nf : w.n_turns_wake;
+    if (t >= n_turns) continue;
+    bool first = true;
+    for (int k = 0; k < ki; ++k) first = first && !same_shape(kk.k[k], w);
+    if (first) wake_rw_group(h, kk, nk, S, part, T, out, ki, tile, t);
   }
 }
 

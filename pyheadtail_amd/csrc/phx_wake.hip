@@ -19,8 +19,9 @@ __global__ void __launch_bounds__(PHX_BLOCK)
 }
 
 __global__ void __launch_bounds__(RW_THREADS)
-    k_wake_rw(const phx_wake_hist *h, WakeKicks kk, int S, double *part, int T, double *out) {
-  wake_rw_block(h, kk, S, part, T, out);
+    k_wake_rw(const phx_wake_hist *h, WakeKicks kk, int nk, int S, double *part, int T,
+              double *out) {
+  wake_rw_block(h, kk, nk, S, part, T, out);
 }
 
 __global__ void k_wake_conv_reduce(const phx_wake_hist *h, WakeKicks kk, int n_kicks, int S,
@@ -85,6 +86,8 @@ static int check_kicks(const phx_wake_kick *kicks, int32_t n_kicks, const char *
     if (k.multiply != -1 && k.multiply != 0 && k.multiply != 2)
       return phx::fail(PHX_E_INVALID, "%s: kick %d bad multiply", who, i);
     if (k.n_turns_wake < 1) return phx::fail(PHX_E_INVALID, "%s: kick %d n_turns_wake < 1", who, i);
+    if (k.precision != PHX_PREC_FP64 && k.precision != PHX_PREC_FP32)
+      return phx::fail(PHX_E_INVALID, "%s: kick %d bad precision %d", who, i, k.precision);
   }
   return PHX_OK;
 }
@@ -118,8 +121,8 @@ int phx_wake_convolve_ws(const phx_wake_hist *hist, const phx_wake_kick *kicks, 
     PHX_CHECK_LAUNCH("phx_wake_convolve");
   }
   if (rw) {
-    k_wake_rw<<<dim3((n_slices + RW_TILE - 1) / RW_TILE * T, n_kicks), RW_THREADS,
-                wake_rw_lds_bytes(n_slices), s>>>(hist, kk, n_slices, part, T, kicks_out);
+    k_wake_rw<<<dim3((n_slices + RW_TILE - 1) / RW_TILE * T), RW_THREADS,
+                wake_rw_lds_bytes(n_slices), s>>>(hist, kk, n_kicks, n_slices, part, T, kicks_out);
     PHX_CHECK_LAUNCH("phx_wake_convolve: resistive wall");
   }
   if (T > 1) {

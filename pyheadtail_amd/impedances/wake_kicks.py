@@ -31,6 +31,8 @@ class WakeKick(Printing):
     moment = _phx.MOM_N
     plane = _phx.PLANE_XP
     multiply = -1
+    # arithmetic of a multi-turn resistive-wall convolution (_phx.PREC_*)
+    precision = _phx.PREC_FP64
 
     def __init__(self, wake_function, slicer, n_turns_wake, *args, **kwargs):
         self.wake_function = wake_function
@@ -59,6 +61,7 @@ class WakeKick(Printing):
         k.multiply = self.multiply
         k.n_turns_wake = int(self.n_turns_wake)
         k.dot_product = 1 if self.dot_product else 0
+        k.precision = self.precision
         k.wake_factor = self._wake_factor(bunch)
         return k
 

@@ -223,9 +223,20 @@ ParallelPlatesResistiveWall = ParallelHorizontalPlatesResistiveWall
 class WakeField(Element):
     """Collective kick from the wake of the current and past slicings."""
 
-    def __init__(self, slicer, *wake_sources):
+    PRECISIONS = {'fp64': _phx.PREC_FP64, 'fp32': _phx.PREC_FP32}
+
+    def __init__(self, slicer, *wake_sources, precision='fp64'):
+        """precision (not in PyHEADTAIL): 'fp32' evaluates the multi-turn
+        resistive-wall convolutions with FP32 wake values and products
+        (summed in FP32 over runs of n_slices/8 sources, in FP64 beyond);
+        every other convolution stays FP64."""
+        if precision not in self.PRECISIONS:
+            raise ValueError("precision must be one of %s" % sorted(self.PRECISIONS))
         self.slicer = slicer
+        self.precision = precision
         self.wake_kicks = [kick for src in wake_sources for kick in src.get_wake_kicks(slicer)]
+        for kick in self.wake_kicks:
+            kick.precision = self.PRECISIONS[precision]
         self.n_turns_wake_max = max(src.n_turns_wake for src in wake_sources)
         # the remembered slicings, newest first, and their ages in seconds
         self.slice_set_deque = deque(maxlen=self.n_turns_wake_max)

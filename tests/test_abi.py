@@ -34,7 +34,7 @@ def test_record_layouts_match():
 
 def test_host_only_entry_points():
     lib = _phx.load_library()
-    assert lib.phx_abi_version() == 1
+    assert lib.phx_abi_version() == 2
     assert lib.phx_bunch_sums_len(6) == 51
     assert lib.phx_bunch_sums_len(7) == -1
     assert lib.phx_workspace_bytes(10**6, 500) > 10**6 // 1024 * 64 * 8

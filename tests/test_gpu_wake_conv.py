@@ -145,3 +145,56 @@ def test_rw_kicks_of_one_source_share_sums():
     got = h.convolve([k.record(_Bunch()) for k in kicks]).cpu().numpy()
     qx, qy = got[names.index('QuadrupoleWakeKickX')], got[names.index('QuadrupoleWakeKickY')]
     np.testing.assert_allclose(qx, -qy, rtol=1e-15, atol=0)
+
+
+def test_fp32_resistive_wall_kick_error_per_component():
+    """FP32 wake values and products against the FP64 path, 1000 slices x 64
+    turns of a parallel-plates wall: max and RMS error per kick component,
+    relative to the component's max and RMS; bound 1e-6 (max)."""
+    from pyheadtail_amd import _phx
+    print("\n[case] FP32 vs FP64 resistive wall, S=1000 T=64", flush=True)
+    h, times, moms, ages, betas = _history(1000, 64, 64, seed=64)
+    kicks, _ = _kicks(64, False, 1e-12)
+    b = _Bunch()
+    recs64 = [k.record(b) for k in kicks]
+    recs32 = [k.record(b) for k in kicks]
+    for r in recs32:
+        r.precision = _phx.PREC_FP32
+    k64 = h.convolve(recs64).cpu().numpy()
+    k32 = h.convolve(recs32).cpu().numpy()
+    for i, k in enumerate(kicks):
+        d = k32[i] - k64[i]
+        emax = np.max(np.abs(d)) / np.max(np.abs(k64[i]))
+        erms = np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(k64[i] ** 2))
+        print("[err] %s FP32 max %.3e rms %.3e" % (type(k).__name__, emax, erms), flush=True)
+        assert emax <= 1e-6 and erms <= 1e-6, (type(k).__name__, emax, erms)
+
+
+def test_fp32_resistive_wall_tracking():
+    """500 turns of a 64-turn parallel-plates wall, FP32 convolution against
+    FP64: centroid and emittance histories within 1e-6 of their scale."""
+    print("\n[case] FP32 vs FP64 tracking, 1e5 MP, S=500, T=64, 500 turns", flush=True)
+    from pyheadtail_amd.engine import FusedOneTurnMap
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.impedances.wakes import WakeField, ParallelHorizontalPlatesResistiveWall
+    from tests.machines import linear_machine, to_product
+    ob = W.linear_bunch(100000, seed=5)
+    hist = {}
+    for prec in ('fp64', 'fp32'):
+        wake = WakeField(UniformBinSlicer(500, n_sigma_z=3),
+                         ParallelHorizontalPlatesResistiveWall(0.02, W.CIRCUMFERENCE, 5.9e7, 1e-12,
+                                                               n_turns_wake=64),
+                         precision=prec)
+        pb = to_product(ob)
+        pb.particlenumber_per_mp = 3e11 / pb.macroparticlenumber
+        eng = FusedOneTurnMap(linear_machine() + [wake], record_stats=True, graph_turns=8)
+        eng.track_turns(pb, 500)
+        eng.flush(pb)
+        hist[prec] = eng.last_stats
+    for key in ('mean_x', 'mean_y', 'epsn_x', 'epsn_y'):
+        a, b = hist['fp32'][key], hist['fp64'][key]
+        scale = np.max(np.abs(b)) if key.startswith('epsn') else \
+            np.max(hist['fp64']['sigma_' + key[5:]])
+        err = np.max(np.abs(a - b)) / scale
+        print("[err] %s history FP32 vs FP64 %.3e" % (key, err), flush=True)
+        assert err <= 1e-6, (key, err)
