@@ -117,7 +117,9 @@ typedef struct {
     double width;             /* (head - tail) / n_slices */
     double mean_z;            /* np.mean(z)   (PHX_CUTS_NSIGMA only) */
     double sigma_z;           /* cov(z, z)^.5 (PHX_CUTS_NSIGMA only) */
-    double absmax[6];         /* max |coordinate| of x, xp, y, yp, z, dp */
+    double absmax[6];         /* max |coordinate| of x, xp, y, yp, z, dp over
+                                 the finite values (the fused engine keeps
+                                 those of x and y only; the others -inf) */
     int64_t n;                /* particles in the bunch */
     int32_t n_slices;
     int32_t turn;             /* turn index of the fused engine */

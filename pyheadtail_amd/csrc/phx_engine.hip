@@ -227,7 +227,7 @@ __global__ void __launch_bounds__(PHX_BLOCK)
           double zb[ZB_NV];
 #pragma unroll
           for (int i = 0; i < ZB_NV; ++i) zb[i] = v[i];
-          zblock_accumulate(zb, cur[k], zshift);
+          zblock_accumulate<0x05u>(zb, cur[k], zshift);  // max |x|, |y|: the histogram's
 #pragma unroll
           for (int i = 0; i < ZB_NV; ++i) v[i] = zb[i];
         }

@@ -20,18 +20,23 @@ struct Coords6 {
 #define ZB_MINM (1ULL << 3)
 #define ZB_MAXM (0x7FULL << 4)
 
+// Bit i of AMASK: keep max |u_i| (the others stay -inf).  Comparisons
+// instead of fmin/fmax: the same results here (a NaN z or u never replaces
+// the running value, which is never NaN) without the NaN quieting of the
+// fmin/fmax instructions.
+template <unsigned AMASK = 0x3Fu>
 __device__ __forceinline__ void zblock_accumulate(double (&v)[ZB_NV], const P6 &p,
                                                   double shift) {
   const double dz = p.z - shift;
   v[0] = v[0] + p.z;
   v[1] = v[1] + dz;
   v[2] = v[2] + dz * dz;
-  v[3] = fmin(v[3], p.z);
-  v[4] = fmax(v[4], p.z);
+  v[3] = p.z < v[3] ? p.z : v[3];
+  v[4] = p.z > v[4] ? p.z : v[4];
   const double a[6] = {fabs(p.x), fabs(p.xp), fabs(p.y), fabs(p.yp), fabs(p.z), fabs(p.dp)};
 #pragma unroll
   for (int i = 0; i < 6; ++i)
-    if (isfinite(a[i])) v[5 + i] = fmax(v[5 + i], a[i]);
+    if (((AMASK >> i) & 1u) && a[i] < INFINITY && a[i] > v[5 + i]) v[5 + i] = a[i];
 }
 
 // Called by one block with the reduced z block in res (shared): writes the
