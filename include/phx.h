@@ -272,6 +272,23 @@ typedef struct {
 int phx_linear_map(double *x, double *y, double *z, double *dp, int64_t n,
                    const phx_linmap *p, phx_stream_t stream);
 
+/* Ideal transverse damper (feedback/transverse_damper.py:15-94): with
+   m_u = sums[u] / n the bunch means from phx_bunch_sums over (xp, yp, x, y)
+   (sums: device, first four entries),
+     xp -= coef_xp * m_xp;   yp -= coef_yp * m_yp            (resistive part)
+     xp -= (coef_x * m_x) / beta;   yp -= (coef_y * m_y) / beta  (if beta)
+   in that order, per active plane.  The coefficients are the host products
+   gain * sin(phase) and gain * cos(phase), gain = 2 / damping rate. */
+typedef struct {
+    double coef_xp, coef_yp;  /* gain * sin(phase) per plane */
+    double coef_x, coef_y;    /* gain * cos(phase) per plane */
+    double beta;              /* local_beta_function; 0: no reactive part */
+    int32_t planes;           /* bit 0: horizontal, bit 1: vertical */
+    int32_t pad;
+} phx_damper;
+int phx_transverse_damper(double *xp, double *yp, int64_t n, const double *sums,
+                          const phx_damper *p, phx_stream_t stream);
+
 /* The FP64 sine and cosine the maps evaluate on the device (the detuned
    betatron phase, the RF phase): s[i] = sin(x[i]), c[i] = cos(x[i]) for
    device arrays of n values; either output may be null.  No reference
@@ -289,8 +306,19 @@ int phx_math_sincos(const double *x, int64_t n, double *s, double *c,
 enum {
     PHX_WAKE_RES_TRANSVERSE = 1,  /* Resonator.function_transverse (:450-470) */
     PHX_WAKE_RES_LONGITUDINAL = 2,/* Resonator.function_longitudinal (:472-493) */
-    PHX_WAKE_RESISTIVE_WALL = 3   /* ResistiveWall.function_transverse (:611-624) */
+    PHX_WAKE_RESISTIVE_WALL = 3,  /* ResistiveWall.function_transverse (:611-624) */
+    PHX_WAKE_TABLE = 4            /* WakeTable.function_transverse /
+                                     function_longitudinal (:290-376) */
 };
+/* Wake tables: linear interpolation of (table_t, table_w) at -dt with
+   np.interp's arithmetic (interp1d delegates to it for FP64 data).
+   TRANS_UR: ultrarelativistic transverse, W(-min(dt, 0)); TRANS_LOWBETA:
+   W(-dt); both NaN outside the table (the reference raises ValueError).
+   LONG_BL: longitudinal with table_t[0] == 0, (sign(-dt) + 1)/2 * W(-dt);
+   LONG: W(-dt); both 0 outside the table.  table_w holds the strengths
+   already converted to SI units with the reference's signs. */
+enum { PHX_TABLE_TRANS_UR = 0, PHX_TABLE_TRANS_LOWBETA = 1, PHX_TABLE_LONG_BL = 2,
+       PHX_TABLE_LONG = 3 };
 enum { PHX_MOM_N = 0, PHX_MOM_X = 1, PHX_MOM_Y = 2 };
 enum { PHX_PLANE_XP = 0, PHX_PLANE_YP = 1, PHX_PLANE_DP = 2 };
 /* Arithmetic of a multi-turn (dot-product) resistive-wall convolution:
@@ -315,6 +343,11 @@ typedef struct {
     /* resistive wall */
     double yokoya, L, b3, dt_min_abs, k_sigma, sqrt_z0c;
     double wake_factor;       /* WakeKick._wake_factor of the current bunch */
+    /* wake table (PHX_WAKE_TABLE): device arrays of table_n >= 2 points,
+       table_t strictly increasing */
+    const double *table_t, *table_w;
+    int32_t table_n;
+    int32_t table_kind;       /* PHX_TABLE_* */
 } phx_wake_kick;
 
 /* Multi-turn history of slice data (impedances/wakes.py:116-145): a ring of

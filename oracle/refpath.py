@@ -513,6 +513,41 @@ def resistive_wall_transverse(yokoya, pipe_radius, length, conductivity, dt_min)
     return wake
 
 
+def wake_table_transverse(time_ns, strength):
+    """WakeTable.function_transverse (wakes.py:290-339): ns -> s, V/pC/mm ->
+    V/C/m with the sign flip, scipy's interp1d (the reference's own
+    interpolator, bounds errors included) at -dt, dt clipped at 0 for an
+    ultrarelativistic table."""
+    from scipy.interpolate import interp1d
+    time = 1e-9 * np.asarray(time_ns, dtype=np.float64)
+    strength = -1e15 * np.asarray(strength, dtype=np.float64)
+    f = interp1d(time, strength)
+    if time[0] == 0 and strength[0] == 0:
+        return lambda dt, *a, **k: f(-np.asarray(dt, dtype=np.float64).clip(max=0))
+    if time[0] < 0:
+        return lambda dt, *a, **k: f(-np.asarray(dt, dtype=np.float64))
+    raise ValueError('does not meet requirements.')
+
+
+def wake_table_longitudinal(time_ns, strength):
+    """WakeTable.function_longitudinal (wakes.py:341-376): V/pC -> V/C, zero
+    outside the table, half the value at dt = 0 for a table from 0."""
+    from scipy.interpolate import interp1d
+    time = 1e-9 * np.asarray(time_ns, dtype=np.float64)
+    strength = 1e12 * np.asarray(strength, dtype=np.float64)
+    f = interp1d(time, strength, bounds_error=False, fill_value=0.)
+
+    def wake(dt, *a, **k):
+        dt = np.asarray(dt, dtype=np.float64)
+        w = f(-dt)
+        if time[0] == 0:
+            return (np.sign(-dt) + 1.) / 2. * w
+        if time[0] < 0:
+            return w
+        raise ValueError('Longitudinal wake component does not meet requirements.')
+    return wake
+
+
 # kick kinds: (moment, kicked coordinate, multiplying coordinate or None)
 KICK_KINDS = {
     'ConstantX': ('n', 'xp', None), 'ConstantY': ('n', 'yp', None),

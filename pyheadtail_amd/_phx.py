@@ -29,7 +29,8 @@ MAX_GRID = 2048
 CUTS_FIXED, CUTS_NSIGMA, CUTS_MINMAX = 0, 1, 2
 DET_CHROMA, DET_AMPLITUDE = 1, 2
 LONG_DRIFT, LONG_KICK = 1, 2
-WAKE_RES_TRANSVERSE, WAKE_RES_LONGITUDINAL, WAKE_RESISTIVE_WALL = 1, 2, 3
+WAKE_RES_TRANSVERSE, WAKE_RES_LONGITUDINAL, WAKE_RESISTIVE_WALL, WAKE_TABLE = 1, 2, 3, 4
+TABLE_TRANS_UR, TABLE_TRANS_LOWBETA, TABLE_LONG_BL, TABLE_LONG = 0, 1, 2, 3
 
 
 def conv_kind(kick):
@@ -113,6 +114,12 @@ class LinMap(C.Structure):
                 ("D_x", C.c_double), ("D_y", C.c_double)]
 
 
+class Damper(C.Structure):
+    _fields_ = [("coef_xp", C.c_double), ("coef_yp", C.c_double), ("coef_x", C.c_double),
+                ("coef_y", C.c_double), ("beta", C.c_double), ("planes", C.c_int32),
+                ("pad", C.c_int32)]
+
+
 class WakeKick(C.Structure):
     _fields_ = [("fn", C.c_int32), ("moment", C.c_int32), ("plane", C.c_int32),
                 ("multiply", C.c_int32), ("n_turns_wake", C.c_int32),
@@ -122,7 +129,9 @@ class WakeKick(C.Structure):
                 ("omegabar", C.c_double), ("alpha_over_omegabar", C.c_double),
                 ("yokoya", C.c_double), ("L", C.c_double), ("b3", C.c_double),
                 ("dt_min_abs", C.c_double), ("k_sigma", C.c_double),
-                ("sqrt_z0c", C.c_double), ("wake_factor", C.c_double)]
+                ("sqrt_z0c", C.c_double), ("wake_factor", C.c_double),
+                ("table_t", C.c_void_p), ("table_w", C.c_void_p), ("table_n", C.c_int32),
+                ("table_kind", C.c_int32)]
 
 
 class WakeHist(C.Structure):
@@ -158,7 +167,8 @@ class Engine(C.Structure):
 RECORDS = {
     "phx_slicer": Slicer, "phx_slice_state": SliceState, "phx_term": Term,
     "phx_detuner": Detuner, "phx_tseg": TSeg, "phx_long_op": LongOp,
-    "phx_long_seq": LongSeq, "phx_linmap": LinMap, "phx_wake_kick": WakeKick,
+    "phx_long_seq": LongSeq, "phx_linmap": LinMap, "phx_damper": Damper,
+    "phx_wake_kick": WakeKick,
     "phx_wake_hist": WakeHist, "phx_turn_program": TurnProgram, "phx_engine": Engine,
 }
 
@@ -182,6 +192,7 @@ SIGNATURES = {
     "phx_transverse_segment": (C.c_int, [P, P, P, P, P, I64, P, P]),
     "phx_longitudinal": (C.c_int, [P, P, P, P, P, P, I64, P, P]),
     "phx_linear_map": (C.c_int, [P, P, P, P, I64, P, P]),
+    "phx_transverse_damper": (C.c_int, [P, P, I64, P, P, P]),
     "phx_math_sincos": (C.c_int, [P, I64, P, P, P]),
     "phx_wake_hist_push": (C.c_int, [P, P, P, P, P, D, D, P]),
     "phx_wake_workspace_bytes": (I64, [I32, I32, I32]),

@@ -76,6 +76,7 @@ extern "C" int64_t phx_sizeof(const char *name) {
   PHX_SZ(phx_long_op)
   PHX_SZ(phx_long_seq)
   PHX_SZ(phx_linmap)
+  PHX_SZ(phx_damper)
   PHX_SZ(phx_wake_kick)
   PHX_SZ(phx_wake_hist)
   PHX_SZ(phx_turn_program)

@@ -169,6 +169,41 @@ extern "C" int phx_linear_map(double *x, double *y, double *z, double *dp, int64
 }
 
 __global__ void __launch_bounds__(PHX_BLOCK)
+    k_damper(double *__restrict__ xp, double *__restrict__ yp, int64_t n,
+             const double *__restrict__ sums, phx_damper p) {
+  const double nd = (double)n;
+  const double mxp = sums[0] / nd, myp = sums[1] / nd, mx = sums[2] / nd, my = sums[3] / nd;
+  const double kxp = p.coef_xp * mxp, kyp = p.coef_yp * myp;
+  const double kx = (p.coef_x * mx) / p.beta, ky = (p.coef_y * my) / p.beta;
+  const bool hx = p.planes & 1, hy = p.planes & 2, react = p.beta != 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * PHX_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * PHX_BLOCK) {
+    if (hx) {
+      double v = xp[i] - kxp;
+      if (react) v = v - kx;
+      xp[i] = v;
+    }
+    if (hy) {
+      double v = yp[i] - kyp;
+      if (react) v = v - ky;
+      yp[i] = v;
+    }
+  }
+}
+
+extern "C" int phx_transverse_damper(double *xp, double *yp, int64_t n, const double *sums,
+                                     const phx_damper *p, phx_stream_t stream) {
+  if (!p || !sums) return phx::fail(PHX_E_INVALID, "phx_transverse_damper: null pointer");
+  if (((p->planes & 1) && !xp) || ((p->planes & 2) && !yp))
+    return phx::fail(PHX_E_INVALID, "phx_transverse_damper: null coordinate");
+  if (n < 0) return phx::fail(PHX_E_INVALID, "phx_transverse_damper: n < 0");
+  if (n == 0 || !(p->planes & 3)) return PHX_OK;
+  k_damper<<<dim3(grid_for(n)), PHX_BLOCK, 0, phx::as_stream(stream)>>>(xp, yp, n, sums, *p);
+  PHX_CHECK_LAUNCH("phx_transverse_damper");
+  return PHX_OK;
+}
+
+__global__ void __launch_bounds__(PHX_BLOCK)
     k_math_sincos(const double *__restrict__ x, int64_t n, double *__restrict__ s,
                   double *__restrict__ c) {
   const int64_t i = (int64_t)blockIdx.x * PHX_BLOCK + threadIdx.x;

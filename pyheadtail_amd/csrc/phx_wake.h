@@ -1,5 +1,6 @@
 // Wake-field device code shared by the standalone calls and the engine.
-//   wake functions     impedances/wakes.py:450-493 (resonator), :611-624 (resistive wall)
+//   wake functions     impedances/wakes.py:450-493 (resonator), :611-624 (resistive wall),
+//                      :290-376 (wake tables)
 //   WakeKick engine    impedances/wake_kicks.py:66-139
 //   WakeField history  impedances/wakes.py:116-145
 #pragma once
@@ -27,6 +28,50 @@ struct WakeShape {
   double h;  // longitudinal: cos + a/wbar sin (cosh + a/wbar sinh, 1 + a dt)
 };
 
+// np.interp(x, t, w) for x inside [t[0], t[n-1]] (numpy's arr_interp: the
+// node value where x hits a node, else slope * (x - t[j]) + w[j] with the
+// other end's form if that gives NaN); left / right outside, NaN for NaN.
+__device__ __forceinline__ double table_interp(const double *t, const double *w, int n, double x,
+                                               double left, double right) {
+  if (x != x) return x;
+  if (x < t[0]) return left;
+  if (x > t[n - 1]) return right;
+  if (x == t[n - 1]) return w[n - 1];
+  int lo = 0, hi = n - 1;  // t[lo] <= x < t[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (t[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  if (t[lo] == x) return w[lo];
+  const double slope = (w[lo + 1] - w[lo]) / (t[lo + 1] - t[lo]);
+  double r = slope * (x - t[lo]) + w[lo];
+  if (r != r) {
+    r = slope * (x - t[lo + 1]) + w[lo + 1];
+    if (r != r && w[lo] == w[lo + 1]) r = w[lo];
+  }
+  return r;
+}
+
+// WakeTable's wake functions (impedances/wakes.py:290-376), see PHX_TABLE_*.
+__device__ __forceinline__ double table_wake(const phx_wake_kick &w, double dt) {
+  const double *t = w.table_t, *v = w.table_w;
+  const int n = w.table_n;
+  switch (w.table_kind) {
+    case PHX_TABLE_TRANS_UR: {
+      const double d = dt > 0.0 ? 0.0 : dt;  // dt.clip(max=0)
+      return table_interp(t, v, n, -d, NAN, NAN);
+    }
+    case PHX_TABLE_TRANS_LOWBETA: return table_interp(t, v, n, -dt, NAN, NAN);
+    case PHX_TABLE_LONG_BL: {
+      const double x = -dt;
+      const double sg = x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : (x == 0.0 ? 0.0 : x));
+      return (sg + 1.0) / 2.0 * table_interp(t, v, n, x, 0.0, 0.0);
+    }
+    default: return table_interp(t, v, n, -dt, 0.0, 0.0);
+  }
+}
+
 __device__ __forceinline__ WakeShape wake_shape(const phx_wake_kick &w, double dt) {
   WakeShape b;
   b.h = 0.0;
@@ -49,6 +94,9 @@ __device__ __forceinline__ WakeShape wake_shape(const phx_wake_kick &w, double d
     } else {
       b.h = cosh(w.omegabar * dt) + w.alpha_over_omegabar * sinh(w.omegabar * dt);
     }
+  } else if (w.fn == PHX_WAKE_TABLE) {
+    b.f = table_wake(w, dt);
+    b.g = 0.0;
   } else {
     const double a = dt + w.dt_min_abs;
     if (a > 0.0) b.f = 2.0;
@@ -66,6 +114,7 @@ __device__ __forceinline__ double wake_value(const phx_wake_kick &w, const WakeS
                                              const double *rw_pre) {
   if (w.fn == PHX_WAKE_RES_TRANSVERSE) return (w.prefix * b.f) * b.g;
   if (w.fn == PHX_WAKE_RES_LONGITUDINAL) return (((b.f * w.prefix) * w.alpha) * b.g) * b.h;
+  if (w.fn == PHX_WAKE_TABLE) return b.f;
   const double pre = b.f == 0.0 ? rw_pre[0] : (b.f == 1.0 ? rw_pre[1] : (b.f == 2.0 ? rw_pre[2] : b.f));
   return (pre * b.g) * w.sqrt_z0c;
 }
@@ -86,6 +135,9 @@ __device__ __forceinline__ bool same_shape(const phx_wake_kick &a, const phx_wak
   if (a.fn == PHX_WAKE_RES_LONGITUDINAL)
     return a.Q == b.Q && a.alpha == b.alpha && a.omegabar == b.omegabar &&
            a.alpha_over_omegabar == b.alpha_over_omegabar;
+  if (a.fn == PHX_WAKE_TABLE)
+    return a.table_t == b.table_t && a.table_w == b.table_w && a.table_n == b.table_n &&
+           a.table_kind == b.table_kind;
   return a.L == b.L && a.b3 == b.b3 && a.dt_min_abs == b.dt_min_abs && a.k_sigma == b.k_sigma &&
          a.sqrt_z0c == b.sqrt_z0c;
 }

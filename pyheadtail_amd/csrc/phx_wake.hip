@@ -79,13 +79,17 @@ static int check_kicks(const phx_wake_kick *kicks, int32_t n_kicks, const char *
   if (n_kicks > PHX_MAX_KICKS) return phx::fail(PHX_E_LIMIT, "%s: n_kicks=%d > %d", who, n_kicks, PHX_MAX_KICKS);
   for (int i = 0; i < n_kicks; ++i) {
     const phx_wake_kick &k = kicks[i];
-    if (k.fn < PHX_WAKE_RES_TRANSVERSE || k.fn > PHX_WAKE_RESISTIVE_WALL)
+    if (k.fn < PHX_WAKE_RES_TRANSVERSE || k.fn > PHX_WAKE_TABLE)
       return phx::fail(PHX_E_INVALID, "%s: kick %d bad wake function %d", who, i, k.fn);
     if (k.moment < 0 || k.moment > 2 || k.plane < 0 || k.plane > 2)
       return phx::fail(PHX_E_INVALID, "%s: kick %d bad moment/plane", who, i);
     if (k.multiply != -1 && k.multiply != 0 && k.multiply != 2)
       return phx::fail(PHX_E_INVALID, "%s: kick %d bad multiply", who, i);
     if (k.n_turns_wake < 1) return phx::fail(PHX_E_INVALID, "%s: kick %d n_turns_wake < 1", who, i);
+    if (k.fn == PHX_WAKE_TABLE &&
+        (!k.table_t || !k.table_w || k.table_n < 2 || k.table_kind < PHX_TABLE_TRANS_UR ||
+         k.table_kind > PHX_TABLE_LONG))
+      return phx::fail(PHX_E_INVALID, "%s: kick %d bad wake table", who, i);
     if (k.precision != PHX_PREC_FP64 && k.precision != PHX_PREC_FP32)
       return phx::fail(PHX_E_INVALID, "%s: kick %d bad precision %d", who, i, k.precision);
   }

@@ -21,6 +21,7 @@ The WakeField keeps its SliceSet history on the device (WakeHistory).
 from collections import deque
 
 import numpy as np
+import torch
 from scipy.constants import c
 
 from .. import _phx
@@ -28,6 +29,50 @@ from ..general.element import Element, Printing
 from .wake_kicks import (WakeHistory, apply_kicks, ConstantWakeKickZ, DipoleWakeKickX,
                          DipoleWakeKickY, QuadrupoleWakeKickX, QuadrupoleWakeKickY)
 from .wake_kicks import *  # noqa: F401,F403  (kick classes, as the reference re-exports)
+
+
+class TableWake(object):
+    """Linear interpolation of a wake table at -dt, as interp1d (which hands
+    FP64 data to np.interp) in WakeTable.function_transverse /
+    function_longitudinal (wakes.py:290-376).  time [s] and strength [SI]
+    are the converted table columns; kind is one of _phx.TABLE_*."""
+
+    def __init__(self, time, strength, kind, name):
+        self.time = np.ascontiguousarray(time, dtype=np.float64)
+        self.strength = np.ascontiguousarray(strength, dtype=np.float64)
+        self.kind, self.name = kind, name
+        self._dev = None
+
+    def _interp(self, x, outside):
+        y = np.interp(x, self.time, self.strength)
+        out = (x < self.time[0]) | (x > self.time[-1])
+        if outside is None:
+            if np.any(out):
+                raise ValueError("%s: a value of -dt is outside the wake table's range "
+                                 "[%g, %g] s" % (self.name, self.time[0], self.time[-1]))
+        else:
+            y = np.where(out, outside, y)
+        return y
+
+    def __call__(self, dt, *args, **kwargs):
+        dt = np.asarray(dt, dtype=np.float64)
+        if self.kind == _phx.TABLE_TRANS_UR:
+            return self._interp(-dt.clip(max=0), None)
+        if self.kind == _phx.TABLE_TRANS_LOWBETA:
+            return self._interp(-dt, None)
+        w = self._interp(-dt, 0.)
+        if self.kind == _phx.TABLE_LONG_BL:
+            return (np.sign(-dt) + 1.) / 2. * w
+        return w
+
+    def fill(self, k):
+        if self._dev is None:
+            dev = _phx.device()
+            self._dev = (torch.from_numpy(self.time).to(dev),
+                         torch.from_numpy(self.strength).to(dev))
+        k.fn = _phx.WAKE_TABLE
+        k.table_t, k.table_w = self._dev[0].data_ptr(), self._dev[1].data_ptr()
+        k.table_n, k.table_kind = len(self.time), self.kind
 
 
 class ResonatorTransverseWake(object):
@@ -142,6 +187,75 @@ class WakeSource(Printing):
 
     def _set_yokoya(self, X1, Y1, X2, Y2):
         self.Yokoya_X1, self.Yokoya_Y1, self.Yokoya_X2, self.Yokoya_Y2 = X1, Y1, X2, Y2
+
+
+class WakeTable(WakeSource):
+    """Wake functions from a table (wakes.py:167-376): columns named by
+    wake_file_columns, 'time' [ns] among them; transverse components
+    [V/pC/mm], 'longitudinal' [V/pC]; HEADTAIL signs.  The component names
+    and the order of the kicks are the reference's.  quadrupole_xy gives
+    its kick like every other component (the reference appends it to a
+    list that does not exist)."""
+
+    COMPONENTS = ('constant_x', 'constant_y', 'longitudinal', 'dipole_x', 'dipole_y',
+                  'dipole_xy', 'dipole_yx', 'quadrupole_x', 'quadrupole_y', 'quadrupole_xy',
+                  'quadrupole_yx')
+
+    def __init__(self, wake_file, wake_file_columns, n_turns_wake=1, *args, **kwargs):
+        self.wake_table = {}
+        wake_data = np.loadtxt(wake_file)
+        if len(wake_file_columns) != wake_data.shape[1]:
+            raise ValueError("Length of wake_file_columns list does not correspond to the "
+                             "number of columns in the specified wake_file.")
+        if 'time' not in wake_file_columns:
+            raise ValueError("No wake_file_column with name 'time' has been specified.")
+        for i, name in enumerate(wake_file_columns):
+            self.wake_table[name] = wake_data[:, i]
+        self.n_turns_wake = n_turns_wake
+
+    def _is_provided(self, wake_component):
+        return wake_component in self.wake_table
+
+    def get_wake_kicks(self, slicer):
+        kinds = {'constant_x': ConstantWakeKickX, 'constant_y': ConstantWakeKickY,
+                 'longitudinal': ConstantWakeKickZ, 'dipole_x': DipoleWakeKickX,
+                 'dipole_y': DipoleWakeKickY, 'dipole_xy': DipoleWakeKickXY,
+                 'dipole_yx': DipoleWakeKickYX, 'quadrupole_x': QuadrupoleWakeKickX,
+                 'quadrupole_y': QuadrupoleWakeKickY, 'quadrupole_xy': QuadrupoleWakeKickXY,
+                 'quadrupole_yx': QuadrupoleWakeKickYX}
+        kicks = []
+        for comp in self.COMPONENTS:
+            if not self._is_provided(comp):
+                continue
+            fn = (self.function_longitudinal() if comp == 'longitudinal'
+                  else self.function_transverse(comp))
+            kicks.append(kinds[comp](fn, slicer, self.n_turns_wake))
+        return kicks
+
+    def function_transverse(self, wake_component):
+        """ns -> s, V/pC/mm -> V/C/m with the sign flip; ultrarelativistic
+        (table starts at (0, 0): dt clipped at 0) or low-beta (table starts
+        below 0) wake (wakes.py:290-339)."""
+        time = 1e-9 * self.wake_table['time']
+        strength = -1e15 * self.wake_table[wake_component]
+        if time[0] == 0 and strength[0] == 0:
+            self.prints(wake_component + ' Assuming ultrarelativistic wake.')
+            return TableWake(time, strength, _phx.TABLE_TRANS_UR, wake_component)
+        if time[0] < 0:
+            self.prints(wake_component + ' Found low beta wake.')
+            return TableWake(time, strength, _phx.TABLE_TRANS_LOWBETA, wake_component)
+        raise ValueError(wake_component + ' does not meet requirements.')
+
+    def function_longitudinal(self):
+        """ns -> s, V/pC -> V/C; zero outside the table; half the value at
+        dt = 0 when the table starts at 0 (wakes.py:341-376)."""
+        time = 1e-9 * self.wake_table['time']
+        strength = 1e12 * self.wake_table['longitudinal']
+        if time[0] == 0:
+            return TableWake(time, strength, _phx.TABLE_LONG_BL, 'longitudinal')
+        if time[0] < 0:
+            return TableWake(time, strength, _phx.TABLE_LONG, 'longitudinal')
+        raise ValueError('Longitudinal wake component does not meet requirements.')
 
 
 class Resonator(WakeSource):
@@ -265,4 +379,14 @@ class WakeField(Element):
         for i in range(0, len(records), _phx.MAX_KICKS):
             group = records[i:i + _phx.MAX_KICKS]
             kicks = self._history.convolve(group)
+            self._check_table_range(group, kicks)
             apply_kicks(bunch, slice_set, group, kicks)
+
+    def _check_table_range(self, records, kicks):
+        """Transverse wake tables give NaN outside their range, where the
+        reference's interp1d raises ValueError: raise it here (one host
+        synchronisation, only for such tables)."""
+        rows = [i for i, r in enumerate(records) if r.fn == _phx.WAKE_TABLE and
+                r.table_kind in (_phx.TABLE_TRANS_UR, _phx.TABLE_TRANS_LOWBETA)]
+        if rows and not bool(torch.isfinite(kicks[rows]).all()):
+            raise ValueError("wake table: a value of -dt is outside the table's range")
