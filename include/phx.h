@@ -397,6 +397,48 @@ int phx_wake_apply(double *const coords[6], int64_t n, const int32_t *idx,
                    phx_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* 2.5-D transverse Gaussian space charge (spacecharge/spacecharge.py:      */
+/* 159-417)                                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* The slice data TransverseGaussianSpaceCharge.track reads (device): the
+   counts and bin edges (Nlambda = counts * charge_per_mp / diff(z_bins)),
+   the slice means and standard deviations of x and y. */
+typedef struct {
+    int32_t n_slices;
+    int32_t pad;
+    const int32_t *counts;    /* [n_slices] */
+    const double *z_bins;     /* [n_slices + 1] */
+    const double *mean_x, *mean_y, *sigma_x, *sigma_y;  /* [n_slices] */
+} phx_sc_slices;
+
+typedef struct {
+    double prefactor;         /* charge * length / (p0 * betagamma * gamma * c) */
+    double charge_per_mp;
+    double denom_pre;         /* 2 * epsilon_0 * sqrt(pi) */
+    double round_pre;         /* 2 * pi * epsilon_0 */
+    double rtol, atol;        /* ratio_threshold, absolute_threshold */
+    int32_t sig_check;        /* 1: swap x/y for sigma_x < sigma_y, round beam */
+    int32_t pad;
+} phx_spacecharge;
+
+/* TransverseGaussianSpaceCharge.track on the particles inside the cuts:
+   xp += (E_x/Q * Nlambda) * prefactor, yp likewise, with the Bassetti-
+   Erskine field of the particle's slice (Faddeeva function by Weideman's
+   rational approximation, N = 40).  The slice index comes from idx when not
+   null, else from z and *state. */
+int phx_spacecharge_kick(double *const coords[6], int64_t n, const int32_t *idx,
+                         const phx_slice_state *state, const phx_sc_slices *slices,
+                         const phx_spacecharge *p, phx_stream_t stream);
+
+/* The device Faddeeva function: wr + i wi = w(x + i y) for y >= 0 (device
+   arrays of n values).  No reference counterpart: it exposes the function
+   behind the space-charge field (pmath.wofz -> scipy.special.wofz,
+   general/pmath.py:50-52) for accuracy checks. */
+int phx_math_wofz(const double *x, const double *y, int64_t n, double *wr, double *wi,
+                  phx_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Fused one-turn map                                                       */
 /* ------------------------------------------------------------------------ */
 

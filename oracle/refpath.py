@@ -548,6 +548,72 @@ def wake_table_longitudinal(time_ns, strength):
     return wake
 
 
+# ---------------------------------------------------------------------------
+# 2.5-D transverse Gaussian space charge (spacecharge/spacecharge.py:159-417)
+# ---------------------------------------------------------------------------
+SC_RATIO_THRESHOLD = 1e-3      # :168-172
+SC_ABSOLUTE_THRESHOLD = 1e-10  # :174-178
+
+
+def sc_efieldn_mit(x, y, sig_x, sig_y):
+    """_efieldn_mit (:275-300) with scipy.special.wofz (pmath.wofz)."""
+    from scipy.special import wofz
+    from scipy.constants import epsilon_0
+    sig_sqrt = np.sqrt(2 * (sig_x**2 - sig_y**2))
+    w1 = wofz(x / sig_sqrt + 1j * (y / sig_sqrt))
+    ex = np.exp(-x * x / (2 * sig_x * sig_x) + -y * y / (2 * sig_y * sig_y))
+    w2 = wofz(x * sig_y / (sig_x * sig_sqrt) + 1j * (y * sig_x / (sig_y * sig_sqrt)))
+    denom = 2. * epsilon_0 * np.sqrt(np.pi) * sig_sqrt
+    return (w1.imag - ex * w2.imag) / denom, (w1.real - ex * w2.real) / denom
+
+
+def sc_efieldn_round(x, y, sig_r):
+    """_efieldn_round (:392-399)."""
+    from scipy.constants import epsilon_0
+    r2 = x * x + y * y
+    amplitude = (1 - np.exp(-r2 / (2 * sig_r * sig_r))) / (2 * np.pi * epsilon_0 * r2)
+    return x * amplitude, y * amplitude
+
+
+def sc_efieldn_checked(x, y, sig_x, sig_y):
+    """add_sigma_check (:401-417)."""
+    tol = dict(rtol=SC_RATIO_THRESHOLD, atol=SC_ABSOLUTE_THRESHOLD)
+    if np.allclose(sig_y, sig_x, **tol):
+        if np.allclose(sig_y, 0, **tol):
+            return np.zeros(x.shape), np.zeros(x.shape)
+        return sc_efieldn_round(x, y, sig_x)
+    if np.all(sig_x < sig_y):
+        en_y, en_x = sc_efieldn_mit(y, x, sig_y, sig_x)
+        return en_x, en_y
+    return sc_efieldn_mit(x, y, sig_x, sig_y)
+
+
+def transverse_gaussian_space_charge(b, n_slices, length, n_sigma_z=None, z_cuts=None,
+                                     sig_check=True):
+    """TransverseGaussianSpaceCharge.track (:202-224) with get_efieldn
+    (:226-246): the kick of every particle inside the cuts, slice by slice."""
+    from scipy.constants import c
+    sd = slice_uniform(b, n_slices, n_sigma_z, z_cuts)
+    mx, my = mean_per_slice(sd, b.x), mean_per_slice(sd, b.y)
+    sx, sy = std_per_slice(sd, b.x), std_per_slice(sd, b.y)
+    prefactor = (b.charge * length / (b.p0 * b.betagamma * b.gamma * c))
+    charge_per_mp = b.particlenumber_per_mp * b.charge
+    lam = sd.n_macroparticles_per_slice * charge_per_mp / np.diff(sd.z_bins)
+    idx = sd.slice_index_of_particle
+    efield = sc_efieldn_checked if sig_check else sc_efieldn_mit
+    for s in range(n_slices):
+        p_id = np.where(idx == s)[0]
+        if len(p_id) == 0:
+            continue
+        x = b.x[p_id] - mx[s]
+        y = b.y[p_id] - my[s]
+        en_x, en_y = efield(np.abs(x), np.abs(y), sx[s], sy[s])
+        en_x = np.abs(en_x) * np.sign(x)
+        en_y = np.abs(en_y) * np.sign(y)
+        b.xp[p_id] = b.xp[p_id] + (en_x * lam[s]) * prefactor
+        b.yp[p_id] = b.yp[p_id] + (en_y * lam[s]) * prefactor
+
+
 # kick kinds: (moment, kicked coordinate, multiplying coordinate or None)
 KICK_KINDS = {
     'ConstantX': ('n', 'xp', None), 'ConstantY': ('n', 'yp', None),

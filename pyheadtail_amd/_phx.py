@@ -114,6 +114,18 @@ class LinMap(C.Structure):
                 ("D_x", C.c_double), ("D_y", C.c_double)]
 
 
+class SCSlices(C.Structure):
+    _fields_ = [("n_slices", C.c_int32), ("pad", C.c_int32), ("counts", C.c_void_p),
+                ("z_bins", C.c_void_p), ("mean_x", C.c_void_p), ("mean_y", C.c_void_p),
+                ("sigma_x", C.c_void_p), ("sigma_y", C.c_void_p)]
+
+
+class SpaceCharge(C.Structure):
+    _fields_ = [("prefactor", C.c_double), ("charge_per_mp", C.c_double),
+                ("denom_pre", C.c_double), ("round_pre", C.c_double), ("rtol", C.c_double),
+                ("atol", C.c_double), ("sig_check", C.c_int32), ("pad", C.c_int32)]
+
+
 class Damper(C.Structure):
     _fields_ = [("coef_xp", C.c_double), ("coef_yp", C.c_double), ("coef_x", C.c_double),
                 ("coef_y", C.c_double), ("beta", C.c_double), ("planes", C.c_int32),
@@ -168,6 +180,7 @@ RECORDS = {
     "phx_slicer": Slicer, "phx_slice_state": SliceState, "phx_term": Term,
     "phx_detuner": Detuner, "phx_tseg": TSeg, "phx_long_op": LongOp,
     "phx_long_seq": LongSeq, "phx_linmap": LinMap, "phx_damper": Damper,
+    "phx_sc_slices": SCSlices, "phx_spacecharge": SpaceCharge,
     "phx_wake_kick": WakeKick,
     "phx_wake_hist": WakeHist, "phx_turn_program": TurnProgram, "phx_engine": Engine,
 }
@@ -193,6 +206,8 @@ SIGNATURES = {
     "phx_longitudinal": (C.c_int, [P, P, P, P, P, P, I64, P, P]),
     "phx_linear_map": (C.c_int, [P, P, P, P, I64, P, P]),
     "phx_transverse_damper": (C.c_int, [P, P, I64, P, P, P]),
+    "phx_spacecharge_kick": (C.c_int, [P, I64, P, P, P, P, P]),
+    "phx_math_wofz": (C.c_int, [P, P, I64, P, P, P]),
     "phx_math_sincos": (C.c_int, [P, I64, P, P, P]),
     "phx_wake_hist_push": (C.c_int, [P, P, P, P, P, D, D, P]),
     "phx_wake_workspace_bytes": (I64, [I32, I32, I32]),

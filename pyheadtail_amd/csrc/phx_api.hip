@@ -77,6 +77,8 @@ extern "C" int64_t phx_sizeof(const char *name) {
   PHX_SZ(phx_long_seq)
   PHX_SZ(phx_linmap)
   PHX_SZ(phx_damper)
+  PHX_SZ(phx_sc_slices)
+  PHX_SZ(phx_spacecharge)
   PHX_SZ(phx_wake_kick)
   PHX_SZ(phx_wake_hist)
   PHX_SZ(phx_turn_program)
