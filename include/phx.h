@@ -397,6 +397,28 @@ int phx_wake_apply(double *const coords[6], int64_t n, const int32_t *idx,
                    phx_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* Uniform-charge slicing (particles/slicing.py:618-686)                    */
+/* ------------------------------------------------------------------------ */
+
+/* Workspace of phx_ucs_sort for n particles (zero-filling not needed). */
+int64_t phx_ucs_workspace_bytes(int64_t n);
+/* z sorted ascending (z_sorted) with the permutation perm (z_sorted[r] =
+   z[perm[r]]; stable: equal z keep their index order, where np.argsort's
+   quicksort leaves it unspecified), and below[0] = #(z < tail), below[1] =
+   #(z < head) for the cuts in *state (np.searchsorted on the sorted z).
+   below: device, 2 x uint64. */
+int phx_ucs_sort(const double *z, int64_t n, const phx_slice_state *state, double *z_sorted,
+                 int32_t *perm, uint64_t *below, void *ws, phx_stream_t stream);
+/* Slice indices and bin edges from first (device, n_slices + 1 entries):
+   sorted position r lies in slice i for first[i] <= r < first[i+1], else
+   -1; idx[perm[r]] is written for every r.  z_bins[0] and z_bins[n_slices]
+   are the cuts, z_bins[i] = (z_sorted[first[i]-1] + z_sorted[first[i]])/2
+   (NaN where first[i] is not a particle position). */
+int phx_ucs_assign(const double *z_sorted, const int32_t *perm, int64_t n, int32_t n_slices,
+                   const int32_t *first, const phx_slice_state *state, int32_t *idx,
+                   double *z_bins, phx_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* 2.5-D transverse Gaussian space charge (spacecharge/spacecharge.py:      */
 /* 159-417)                                                                 */
 /* ------------------------------------------------------------------------ */

@@ -676,8 +676,9 @@ class Kick(object):
 class WakeFieldOracle(object):
     """WakeField.track (wakes.py:121-145)."""
 
-    def __init__(self, n_slices, kicks, n_sigma_z=None, z_cuts=None):
+    def __init__(self, n_slices, kicks, n_sigma_z=None, z_cuts=None, uniform_charge=False):
         self.n_slices, self.n_sigma_z, self.z_cuts = n_slices, n_sigma_z, z_cuts
+        self.uniform_charge = uniform_charge
         self.kicks = kicks
         T = max(k.n_turns_wake for k in kicks)
         self.slices = deque([], maxlen=T)
@@ -686,7 +687,10 @@ class WakeFieldOracle(object):
     def track(self, b):
         for i in range(len(self.ages)):
             self.ages[i] += b.circumference / (b.beta * c)
-        sd = slice_uniform(b, self.n_slices, self.n_sigma_z, self.z_cuts)
+        if self.uniform_charge:
+            sd = slice_uniform_charge(b, self.n_slices, self.n_sigma_z, self.z_cuts)
+        else:
+            sd = slice_uniform(b, self.n_slices, self.n_sigma_z, self.z_cuts)
         sd.mean_x = mean_per_slice(sd, b.x)
         sd.mean_y = mean_per_slice(sd, b.y)
         self.slices.appendleft(sd)
@@ -694,3 +698,34 @@ class WakeFieldOracle(object):
         for k in self.kicks:
             k.apply(b, self.slices, self.ages)
         return sd
+
+
+def slice_uniform_charge(beam, n_slices, n_sigma_z=None, z_cuts=None, perm=None):
+    """UniformChargeSlicer.compute_sliceset_kwargs (slicing.py:636-686),
+    with random.sample drawing from Python's global generator.  perm: the
+    sort permutation to use (np.argsort(z) when None)."""
+    from random import sample
+    z_cut_tail, z_cut_head = long_cuts(beam, n_sigma_z, z_cuts)
+    n_part = len(beam.z)
+    id_new = np.argsort(beam.z) if perm is None else np.asarray(perm)
+    id_old = np.arange(n_part).take(id_new)
+    z_sorted = beam.z.take(id_new)
+    n_cut_tail = np.searchsorted(z_sorted, z_cut_tail)
+    n_cut_head = n_part - np.searchsorted(z_sorted, z_cut_head)
+    n_part_within_cuts = n_part - n_cut_tail - n_cut_head
+    rand_slice_i = sample(list(range(n_slices)), n_part_within_cuts % n_slices)
+    n_part_per_slice = (n_part_within_cuts // n_slices) * np.ones(n_slices, dtype=np.int32)
+    n_part_per_slice[rand_slice_i] += 1
+    n_all = np.hstack((n_cut_tail, n_part_per_slice, n_cut_head))
+    first_indices = np.cumsum(n_all, dtype=np.int32)[:-1]
+    z_bins = np.empty(n_slices + 1)
+    z_bins[1:-1] = (z_sorted[first_indices[1:-1] - 1] + z_sorted[first_indices[1:-1]]) / 2
+    z_bins[0], z_bins[-1] = z_cut_tail, z_cut_head
+    idx_sorted = -np.ones(n_part, dtype=np.int32)
+    for i in range(n_slices):
+        idx_sorted[first_indices[i]:first_indices[i + 1]] = i
+    idx = np.empty(n_part, dtype=np.int32)
+    np.put(idx, id_old, idx_sorted)
+    sd = SliceData(z_bins, idx, beam)
+    sd.n_macroparticles_per_slice = n_part_per_slice
+    return sd

@@ -208,6 +208,9 @@ SIGNATURES = {
     "phx_transverse_damper": (C.c_int, [P, P, I64, P, P, P]),
     "phx_spacecharge_kick": (C.c_int, [P, I64, P, P, P, P, P]),
     "phx_math_wofz": (C.c_int, [P, P, I64, P, P, P]),
+    "phx_ucs_workspace_bytes": (I64, [I64]),
+    "phx_ucs_sort": (C.c_int, [P, I64, P, P, P, P, P, P]),
+    "phx_ucs_assign": (C.c_int, [P, P, I64, C.c_int32, P, P, P, P, P]),
     "phx_math_sincos": (C.c_int, [P, I64, P, P, P]),
     "phx_wake_hist_push": (C.c_int, [P, P, P, P, P, D, D, P]),
     "phx_wake_workspace_bytes": (I64, [I32, I32, I32]),

@@ -12,7 +12,7 @@ FLAGS=(-O3 -std=c++17 -fPIC --offload-arch="$ARCH" -ffp-contract=off
 objs=()
 pids=()
 mkdir -p "$here/_build"
-for src in phx_api phx_stats phx_track phx_wake phx_engine phx_spacecharge; do
+for src in phx_api phx_stats phx_track phx_wake phx_engine phx_spacecharge phx_sort; do
   "$HIPCC" "${FLAGS[@]}" -c "$here/$src.hip" -o "$here/_build/$src.o" &
   pids+=($!)
   objs+=("$here/_build/$src.o")

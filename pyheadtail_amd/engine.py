@@ -91,6 +91,9 @@ class FusedOneTurnMap(Element):
             raise ValueError("too many elements for one fused pass")
         if self.wake is not None and not self.wake.wake_kicks:
             raise ValueError("WakeField without kicks")
+        if self.wake is not None and self.wake.slicer.mode != 'uniform_bin':
+            raise ValueError("the fused engine slices with uniform bins; track a WakeField "
+                             "with a %s slicer as an element" % self.wake.slicer.mode)
         if self.wake is not None and len(self.wake.wake_kicks) > _phx.MAX_KICKS:
             raise ValueError("the fused engine takes at most %d wake kicks; track this "
                              "WakeField as an element" % _phx.MAX_KICKS)
