@@ -55,7 +55,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-turns", type=int, default=100)
     ap.add_argument("--graph-turns", type=int, default=10)
-    ap.add_argument("--workload", choices=("resonator", "rw64"), default="resonator")
+    ap.add_argument("--workload", choices=("resonator", "rw64", "scan64"), default="resonator")
+    ap.add_argument("--streams", type=int, default=4)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of the pass's algorithmic bytes "
                          "(48 B read + 48 B written per macroparticle), for scaling the "
@@ -212,6 +213,69 @@ def rw64(args):
     print(json.dumps(out))
 
 
+def scan_job(i, n_scale=1.0, seed_base=1000):
+    """Job i of the batched scan: macroparticles log-uniform in [2e5, 4e6],
+    slices in {100, 250, 500, 1000}, intensity uniform in [1e10, 3e11],
+    chromaticity uniform in [-5, 15], n_turns_wake in {1, 8, 32}; the
+    LinearMap ring with a broadband resonator (one turn of memory) or the
+    parallel-plates resistive wall (8 or 32 turns).  Seeded by seed_base + i."""
+    import numpy as np
+    from scipy.constants import e, m_p
+    from pyheadtail_amd.machines.synchrotron import Synchrotron
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.impedances.wakes import (WakeField, CircularResonator,
+                                                 ParallelHorizontalPlatesResistiveWall)
+    rng = np.random.default_rng(seed_base + i)
+    n = int(np.exp(rng.uniform(np.log(2e5), np.log(4e6))) * n_scale)
+    S = int(rng.choice([100, 250, 500, 1000]))
+    intensity = rng.uniform(1e10, 3e11)
+    qp = rng.uniform(-5., 15.)
+    T = int(rng.choice([1, 8, 32]))
+    mach = Synchrotron(optics_mode='smooth', charge=e, mass=m_p, p0=p0_of_gamma(27.7),
+                       circumference=6911.5, n_segments=1, beta_x=54.6, beta_y=54.6, D_x=0.,
+                       D_y=0., accQ_x=20.13, accQ_y=20.18, Qp_x=qp, Qp_y=qp,
+                       alpha_mom_compaction=1.9e-3, longitudinal_mode='linear', Q_s=0.017)
+    src = (CircularResonator(1e6, 1e9, 1) if T == 1 else
+           ParallelHorizontalPlatesResistiveWall(0.02, 6911.5, 5.9e7, 1e-12, n_turns_wake=T))
+    np.random.seed(seed_base + i)
+    bunch = mach.generate_6D_Gaussian_bunch(n, intensity, EPSN, EPSN, SIGMA_Z)
+    return list(mach.one_turn_map) + [WakeField(UniformBinSlicer(S, n_sigma_z=3), src)], bunch
+
+
+def scan64(args):
+    """64 independent bunches: job by job against batched (one JSON line)."""
+    import torch
+    torch.cuda.set_device(0)
+    from pyheadtail_amd.batch import BatchedScan
+    jobs = [scan_job(i) for i in range(64)]
+    scan = BatchedScan(jobs, n_streams=args.streams, chunk_turns=50,
+                       graph_turns=args.graph_turns)
+    n_mp = scan.macroparticles
+    steps = min(args.steps, 200)
+    scan.track_turns(40)                      # graphs, histories (T <= 32) filled
+    t0 = time.perf_counter()
+    scan.track_turns_job_by_job(steps)
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    scan.track_turns(steps)
+    t_bat = time.perf_counter() - t0
+    out = {
+        "metric": "aggregate macroparticle-turns/s, 64-bunch scan",
+        "value": n_mp * steps / t_bat, "unit": "macroparticle-turns/s", "n_gpus": 1,
+        "steps": steps, "warmup": 40, "ms_per_step": t_bat / steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (64 seeded Gaussian bunches)",
+        "config": {"workload": "64 independent bunches: macroparticles log-uniform in "
+                               "[2e5, 4e6] (%d in total), slices in {100,250,500,1000}, "
+                               "chromaticity in [-5, 15], n_turns_wake in {1,8,32}; LinearMap "
+                               "ring, resonator or resistive-wall wake" % n_mp,
+                   "bunches": 64, "macroparticles": n_mp, "streams": args.streams},
+        "job_by_job": {"value": n_mp * steps / t_seq, "ms_per_step": t_seq / steps * 1e3},
+        "speedup_vs_job_by_job": t_seq / t_bat,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -224,6 +288,9 @@ def main():
         return
     if args.workload == "rw64":
         rw64(args)
+        return
+    if args.workload == "scan64":
+        scan64(args)
         return
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_turns)
 
