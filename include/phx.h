@@ -563,15 +563,23 @@ int phx_engine_reset(const phx_engine *e, phx_stream_t stream);
    pass (pending wake kick, local elements, z moments), the slice histogram
    (cuts, per-block histograms), the histogram merge (slice means, history
    entry) and the wake convolution; without a wake field, the pass and a
-   one-block finalize.  The wake kick of the last turn stays pending until
-   phx_engine_flush. */
+   one-block finalize; one more launch ahead of the first turn records
+   particle 0 (phx_engine_snapshot).  The wake kick of the last turn stays
+   pending until phx_engine_flush. */
 int phx_engine_turns(const phx_engine *e, int32_t n_turns, phx_stream_t stream);
 /* Applies a pending wake kick (and records the final statistics row). */
 int phx_engine_flush(const phx_engine *e, phx_stream_t stream);
 /* Enqueue one stage of a turn: 0 particle pass, 1 slice histogram and
    merge, 2 wake convolution (stages 1-2 need a wake field).  Running 0, 1, 2
-   in order is one turn; used to time the stages separately. */
+   in order is one turn; used to time the stages separately.  Call
+   phx_engine_snapshot before the first stage 0 after the coordinates
+   changed outside the engine. */
 int phx_engine_stage(const phx_engine *e, int32_t stage, phx_stream_t stream);
+/* Records particle 0's coordinates in the workspace: the shifts of the
+   shifted sums of the next particle pass (stats.pyx:cov takes the first
+   particle), read there by every block while block 0 rewrites particle 0.
+   phx_engine_turns and phx_engine_flush call it themselves. */
+int phx_engine_snapshot(const phx_engine *e, phx_stream_t stream);
 
 /* HIP graph of n_turns fused turns (captured on an internal stream);
    replay with phx_graph_launch.  Returns an opaque handle. */

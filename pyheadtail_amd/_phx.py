@@ -221,6 +221,7 @@ SIGNATURES = {
     "phx_engine_turns": (C.c_int, [P, I32, P]),
     "phx_engine_flush": (C.c_int, [P, P]),
     "phx_engine_stage": (C.c_int, [P, I32, P]),
+    "phx_engine_snapshot": (C.c_int, [P, P]),
     "phx_graph_capture": (C.c_int, [P, I32, C.POINTER(C.c_void_p)]),
     "phx_graph_launch": (C.c_int, [P, P]),
     "phx_graph_destroy": (C.c_int, [P]),

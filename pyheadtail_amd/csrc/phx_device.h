@@ -43,8 +43,11 @@ struct P6 {
 // Workgroup barrier for data exchanged through LDS only.  __syncthreads()
 // also waits for every outstanding global load and store of the wave (its
 // release fence covers global memory); this one waits for LDS operations
-// only, so particle loads and stores stay in flight across it.  Every
-// barrier of this library protects LDS data only.
+// only, so particle stores stay in flight across it.  Every barrier of this
+// library protects LDS data only.  No kernel here leaves a global load in
+// flight across it: the slice histogram's fold of the cuts, with the first
+// particle loads issued ahead of it, gave wrong maxima in some blocks on
+// MI355X.
 __device__ __forceinline__ void lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();

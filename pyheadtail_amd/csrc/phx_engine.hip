@@ -38,6 +38,7 @@ inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
 
 struct EngineWs {
   double *partials;   // [EN_NV][pass grid] (slot-major)
+  double *p0;         // particle 0 before the pass: the shifts of its sums
   AccParts acc;       // H partial histograms, T = 2 (x, y)
   double *conv_part;  // [n_kicks][T][S]
 };
@@ -56,6 +57,8 @@ __host__ __device__ inline EngineWs engine_ws(void *ws, int64_t n, int S) {
   const int Se = S > 0 ? S : 1;
   w.partials = reinterpret_cast<double *>(p);
   p += ((size_t)nc * EN_NV * 8 + 255) / 256 * 256;
+  w.p0 = reinterpret_cast<double *>(p);
+  p += 256;
   const int H = hist_blocks(n);
   w.acc = acc_parts_layout(p, H, Se, 2);
   p += ((size_t)H * Se * 4 + 255) / 256 * 256 + (size_t)H * 2 * Se * 16 + (size_t)2 * Se * 8;
@@ -66,7 +69,7 @@ __host__ __device__ inline EngineWs engine_ws(void *ws, int64_t n, int S) {
 size_t engine_ws_bytes(int64_t n, int S, int n_kicks, int T) {
   const int64_t nc = phx_grid_for(n, PHX_MAX_GRID);
   const int Se = S > 0 ? S : 1;
-  size_t b = round256((size_t)nc * EN_NV * 8) + acc_parts_bytes(hist_blocks(n), Se, 2);
+  size_t b = round256((size_t)nc * EN_NV * 8) + 256 + acc_parts_bytes(hist_blocks(n), Se, 2);
   b += (size_t)(n_kicks > 0 ? n_kicks : 1) * T * Se * 8;
   return round256(b);
 }
@@ -178,10 +181,12 @@ __global__ void __launch_bounds__(PHX_BLOCK)
   // state after the pending kick (statistics), and its z after this turn's
   // elements (cuts), which needs the kick only where a kick changes dp and
   // only the longitudinal elements.  Every lane computes it (uniform values;
-  // once per block).
+  // once per block) from the snapshot taken before this pass: block 0
+  // rewrites particle 0 while later blocks may still be starting.
   double src[7];
   {
-    P6 q = P6{x[0], xp[0], y[0], yp[0], z[0], dp[0]};
+    const double *p0 = w.p0;
+    P6 q = P6{p0[0], p0[1], p0[2], p0[3], p0[4], p0[5]};
     bool kick_dp = false;
     if constexpr (KS == PHX_KICKS_GENERIC) {
       for (int k = 0; k < nk; ++k) kick_dp = kick_dp || g.kick[k].plane == PHX_PLANE_DP;
@@ -262,10 +267,20 @@ __global__ void __launch_bounds__(PHX_BLOCK) k_engine_fin(EngineArgs a) {
   fold_partials<L::MINM, L::MAXM>(w.partials, a.grid, 0, L::NV, res);
   if (REC && threadIdx.x < EN_STAT_NV)
     e.stats[(size_t)turn * EN_STAT_NV + threadIdx.x] = res[L::SOFF + threadIdx.x];
+  if (threadIdx.x < 6) w.p0[threadIdx.x] = e.coords[threadIdx.x][0];  // the next pass's shifts
   if (threadIdx.x == 0) {
     if (MODE == 0) *e.turn_counter = turn + 1;
     else *e.pending = 0;
   }
+}
+
+// Particle 0 as it is now, the shifts of the next pass.  The kernels that
+// follow a pass (finalize, histogram) refresh it; this one is for the first
+// pass of a launch sequence, after which the coordinates may have changed.
+__global__ void k_engine_snapshot(EngineArgs a) {
+  const phx_engine &e = a.e;
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
+  if (threadIdx.x < 6) w.p0[threadIdx.x] = e.coords[threadIdx.x][0];
 }
 
 // Slice histogram of the new state.  Prologue: every block folds the z
@@ -290,9 +305,11 @@ __global__ void __launch_bounds__(EN_HIST_THREADS) k_engine_hist(EngineArgs a) {
   constexpr int CH = EN_HIST_THREADS * PHX_PPT;
   const int64_t nc = (e.n + CH - 1) / CH;
   const int64_t n = e.n;
-  // The block's first particles load while the cuts are folded (the fold's
-  // barriers are LDS-only, so the loads stay in flight), and each chunk
-  // loads the next one before it is binned.
+  // Each chunk loads the next one before it is binned.  No particle load is
+  // in flight across a barrier: with the first chunk's loads issued ahead
+  // of the fold of the cuts, some blocks' folds came out wrong on MI355X
+  // (about 1 in 25 block folds on a new engine's first turns; none with
+  // the loads issued after the last barrier).
   double zz[PHX_PPT], xx[PHX_PPT], yy[PHX_PPT];
   double zn[PHX_PPT], xn[PHX_PPT], yn[PHX_PPT];
   auto load_chunk = [&](int64_t c, double (&z)[PHX_PPT], double (&x)[PHX_PPT],
@@ -311,7 +328,7 @@ __global__ void __launch_bounds__(EN_HIST_THREADS) k_engine_hist(EngineArgs a) {
       }
     }
   };
-  load_chunk(blockIdx.x, zz, xx, yy);
+  if (blockIdx.x == 0 && threadIdx.x < 6) w.p0[threadIdx.x] = e.coords[threadIdx.x][0];
   // every block needs the z block; only block 0 writes the statistics row
   fold_partials<L::MINM, L::MAXM>(w.partials, ncp, 0, ZB_NV, res);
   if constexpr (REC) {
@@ -348,6 +365,7 @@ __global__ void __launch_bounds__(EN_HIST_THREADS) k_engine_hist(EngineArgs a) {
   acc_lds_zero(A, S, 2);  // ends with a barrier: the shared cuts are visible
   const double tail = sh_tail, width = sh_width;
   const int ex = sh_ex, ey = sh_ey;
+  load_chunk(blockIdx.x, zz, xx, yy);
   for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
     load_chunk(c + gridDim.x, zn, xn, yn);
     const int64_t base = c * CH + threadIdx.x;
@@ -631,6 +649,12 @@ int enqueue_stage(const EngineArgs &a, int stage, bool has_wake, hipStream_t s) 
   return PHX_OK;
 }
 
+int enqueue_snapshot(const EngineArgs &a, hipStream_t s) {
+  k_engine_snapshot<<<1, 64, 0, s>>>(a);
+  PHX_CHECK_LAUNCH("phx_engine: snapshot");
+  return PHX_OK;
+}
+
 int enqueue_turn(const EngineArgs &a, bool has_wake, hipStream_t s) {
   int rc = enqueue_stage(a, 0, has_wake, s);
   if (rc || !has_wake) return rc;
@@ -679,6 +703,10 @@ extern "C" int phx_engine_turns(const phx_engine *e, int32_t n_turns, phx_stream
   if (rc) return rc;
   const bool has_wake = e->n_slices > 0 && nk > 0;
   EngineArgs a = make_args(e, nk, T);
+  if (n_turns > 0) {
+    rc = enqueue_snapshot(a, phx::as_stream(stream));
+    if (rc) return rc;
+  }
   for (int t = 0; t < n_turns; ++t) {
     rc = enqueue_turn(a, has_wake, phx::as_stream(stream));
     if (rc) return rc;
@@ -704,7 +732,16 @@ extern "C" int phx_engine_flush(const phx_engine *e, phx_stream_t stream) {
   int rc = check_engine(e, &nk, &T);
   if (rc) return rc;
   EngineArgs a = make_args(e, nk, T);
+  rc = enqueue_snapshot(a, phx::as_stream(stream));
+  if (rc) return rc;
   return launch_pass<1>(a, false, phx::as_stream(stream));
+}
+
+extern "C" int phx_engine_snapshot(const phx_engine *e, phx_stream_t stream) {
+  int nk, T;
+  int rc = check_engine(e, &nk, &T);
+  if (rc) return rc;
+  return enqueue_snapshot(make_args(e, nk, T), phx::as_stream(stream));
 }
 
 struct GraphHandle {

@@ -464,7 +464,7 @@ __device__ __forceinline__ double wave_min_nan(double v, int &bad) {
   return v;
 }
 
-#define RW_STATIC_LDS (RW_WAVES * sizeof(double))
+#define RW_STATIC_LDS (RW_WAVES * (sizeof(double) + sizeof(int)))
 // Dynamic LDS of wake_rw_block: the sources, then (reused) the wave sums of
 // one moment at a time.
 inline size_t wake_rw_lds_bytes(int S) {
@@ -479,6 +479,7 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const Wake
                                               int tile, int t) {
   extern __shared__ double rw_lds[];
   __shared__ double wmin[RW_WAVES];
+  __shared__ int wbad[RW_WAVES];
   const phx_wake_kick &w = kk.k[ki];
   const int head = h->head, cap = h->capacity;
   unsigned mask = 0;                     // moments the group reads
@@ -519,9 +520,17 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const Wake
   }
   mn = wave_min_nan(mn, bad);
   const double tmax = -wave_min_nan(-mx, bad);
-  if (lane == 0) wmin[wave] = mn;
-  bad = __syncthreads_or(bad);
-  for (int q = 0; q < RW_WAVES; ++q) mn = wmin[q] < mn ? wmin[q] : mn;
+  const int wave_bad = __ballot(bad) != 0ULL;
+  if (lane == 0) {
+    wmin[wave] = mn;
+    wbad[wave] = wave_bad;
+  }
+  lds_sync();  // the sources, wave minima and NaN flags are the block's
+  bad = 0;
+  for (int q = 0; q < RW_WAVES; ++q) {
+    mn = wmin[q] < mn ? wmin[q] : mn;
+    bad |= wbad[q];
+  }
   const bool uniform = !bad && (tmax - mn) + w.dt_min_abs < 0.0;
 
   const int chunk = (S + RW_WAVES - 1) / RW_WAVES;

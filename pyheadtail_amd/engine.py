@@ -360,6 +360,7 @@ class FusedOneTurnMap(Element):
         eng = self._prepare(beam, n_turns)
         s = _phx.stream()
         n_st = 3 if self.wake is not None else 1
+        _phx.call(L.phx_engine_snapshot, C.byref(eng), s)
         ev = [[torch.cuda.Event(enable_timing=True) for _ in range(n_st + 1)]
               for _ in range(n_turns)]
         for t in range(n_turns):

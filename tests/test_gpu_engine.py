@@ -74,6 +74,43 @@ def test_engine_split_calls_equal_one_call():
         np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)
 
 
+def test_engine_coordinates_changed_between_calls():
+    """The shifts of the pass's sums come from particle 0 as it is when a
+    call starts: the coordinates changed between two calls of one engine
+    give what the element path gives from the changed coordinates, also
+    when the pass grid is the automatic one (many blocks start while block 0
+    has already rewritten particle 0)."""
+    _print_case("engine: coordinates changed between calls, 6+6 turns, N=2e5")
+    ob = W.nonlinear_bunch(200000, seed=14)
+    a, b = to_product(ob), to_product(ob)
+    m_el = nonlinear_machine()
+    eng = _fused(nonlinear_machine(), graph_turns=3, pass_grid=_standalone_grid())
+    for el in m_el * 6:
+        el.track(a)
+    eng.track_turns(b, 6)
+    for pb in (a, b):
+        pb.x[0] += 1e-3
+        pb.z.mul_(1.01)
+    for _ in range(6):
+        for el in m_el:
+            el.track(a)
+    eng.track_turns(b, 6)
+    ga, gb = to_numpy(a), to_numpy(b)
+    for k in COORDS:
+        np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)
+    # automatic grid, twice from the same start: identical
+    outs = []
+    for _ in range(2):
+        pb = to_product(ob)
+        e = _fused(nonlinear_machine(), graph_turns=4)
+        e.track_turns(pb, 5)
+        pb.z.mul_(0.99)
+        e.track_turns(pb, 7)
+        outs.append(to_numpy(pb))
+    for k in COORDS:
+        np.testing.assert_array_equal(outs[0][k], outs[1][k], err_msg=k)
+
+
 def test_engine_vs_oracle_nonlinear():
     _print_case("engine vs oracle, nonlinear+wake, N=2e4, 30 turns")
     n_turns = 30
