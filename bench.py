@@ -55,7 +55,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-turns", type=int, default=100)
     ap.add_argument("--graph-turns", type=int, default=10)
-    ap.add_argument("--workload", choices=("resonator", "rw64", "scan64"), default="resonator")
+    ap.add_argument("--workload", choices=("resonator", "rw64", "scan64", "serve"),
+                    default="resonator")
+    ap.add_argument("--jobs", type=int, default=48)
     ap.add_argument("--streams", type=int, default=4)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of the pass's algorithmic bytes "
@@ -276,6 +278,84 @@ def scan64(args):
     print(json.dumps(out))
 
 
+def serve_jobs(n_jobs, seed=2000):
+    """Served instability-scan stream: Poisson arrivals at 4 jobs/s;
+    macroparticles log-uniform in [1e5, 1e7], slices uniform in [100, 1000],
+    turns log-uniform in [256, 4096], intensity and chromaticity as the
+    batched scan; the LinearMap ring with the broadband resonator (the
+    space-charge kicks of the 1e7 jobs are not included: the fused engine
+    does not take that element).  Host-side bunches, generated up front."""
+    import numpy as np
+    from scipy.constants import e, m_p
+    from pyheadtail_amd.machines.synchrotron import Synchrotron
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for j in range(n_jobs):
+        t += rng.exponential(1 / 4.0)
+        n = int(np.exp(rng.uniform(np.log(1e5), np.log(1e7))))
+        S = int(rng.integers(100, 1001))
+        turns = int(np.exp(rng.uniform(np.log(256), np.log(4096))))
+        intensity, qp = rng.uniform(1e10, 3e11), rng.uniform(-5., 15.)
+        mach = Synchrotron(optics_mode='smooth', charge=e, mass=m_p, p0=p0_of_gamma(27.7),
+                           circumference=6911.5, n_segments=1, beta_x=54.6, beta_y=54.6,
+                           D_x=0., D_y=0., accQ_x=20.13, accQ_y=20.18, Qp_x=qp, Qp_y=qp,
+                           alpha_mom_compaction=1.9e-3, longitudinal_mode='linear', Q_s=0.017)
+        np.random.seed(seed + j)
+        b = mach.generate_6D_Gaussian_bunch(n, intensity, EPSN, EPSN, SIGMA_Z)
+        host = {k: getattr(b, k).cpu().numpy() for k in ('x', 'xp', 'y', 'yp', 'z', 'dp')}
+        out.append(dict(t=t, mach=mach, S=S, turns=turns, n=n, host=host,
+                        params=(b.particlenumber_per_mp, b.charge, b.mass, b.circumference,
+                                b.gamma)))
+        del b
+    return out
+
+
+def serve(args):
+    """Served stream: p50 / p99 time-to-result and aggregate throughput."""
+    import numpy as np
+    import torch
+    torch.cuda.set_device(0)
+    from pyheadtail_amd.serve import ScanServer
+    from pyheadtail_amd.particles.particles import Particles
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.impedances.wakes import WakeField, CircularResonator
+    jobs = serve_jobs(args.jobs)
+    srv = ScanServer(n_streams=args.streams, chunk_turns=64, graph_turns=8)
+    t0 = time.perf_counter()
+    i = 0
+    while i < len(jobs) or srv.busy:
+        now = time.perf_counter() - t0
+        while i < len(jobs) and jobs[i]['t'] <= now:
+            jd = jobs[i]
+            ppm, q, m, C, g = jd['params']
+            bunch = Particles(jd['n'], ppm, q, m, C, g, jd['host'])
+            wake = WakeField(UniformBinSlicer(jd['S'], n_sigma_z=3), CircularResonator(1e6, 1e9, 1))
+            srv.submit(list(jd['mach'].one_turn_map) + [wake], bunch, jd['turns'], job_id=i)
+            jd['host'] = None
+            i += 1
+        if srv.busy:
+            srv.step()
+        elif i < len(jobs):
+            time.sleep(max(0.0, jobs[i]['t'] - (time.perf_counter() - t0)))
+    makespan = time.perf_counter() - t0
+    work = sum(j.bunch.macroparticlenumber * j.n_turns for j in srv.done)
+    p = srv.latency_percentiles((50, 99))
+    out = {
+        "metric": "served scan: p99 time-to-result",
+        "value": p[99], "unit": "s", "n_gpus": 1, "steps": len(srv.done), "warmup": 0,
+        "ms_per_step": makespan / max(1, len(srv.done)) * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Gaussian bunches, Poisson arrivals at 4 jobs/s)",
+        "config": {"workload": "%d jobs: macroparticles log-uniform in [1e5, 1e7], slices in "
+                               "[100, 1000], turns log-uniform in [256, 4096]; LinearMap ring "
+                               "with a broadband resonator" % len(jobs),
+                   "jobs": len(jobs), "streams": args.streams},
+        "p50_s": p[50], "p99_s": p[99],
+        "aggregate_mp_turns_per_s": work / makespan, "makespan_s": makespan,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -291,6 +371,9 @@ def main():
         return
     if args.workload == "scan64":
         scan64(args)
+        return
+    if args.workload == "serve":
+        serve(args)
         return
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_turns)
 
