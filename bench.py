@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--workload", choices=("resonator", "rw64", "scan64", "serve"),
                     default="resonator")
     ap.add_argument("--jobs", type=int, default=48)
+    ap.add_argument("--no-sc", action="store_true",
+                    help="served stream without the space-charge kicks of the largest jobs")
     ap.add_argument("--streams", type=int, default=4)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of the pass's algorithmic bytes "
@@ -278,16 +280,20 @@ def scan64(args):
     print(json.dumps(out))
 
 
-def serve_jobs(n_jobs, seed=2000):
+def serve_jobs(n_jobs, seed=2000, space_charge=True):
     """Served instability-scan stream: Poisson arrivals at 4 jobs/s;
     macroparticles log-uniform in [1e5, 1e7], slices uniform in [100, 1000],
     turns log-uniform in [256, 4096], intensity and chromaticity as the
-    batched scan; the LinearMap ring with the broadband resonator (the
-    space-charge kicks of the 1e7 jobs are not included: the fused engine
-    does not take that element).  Host-side bunches, generated up front."""
+    batched scan; the LinearMap ring with the broadband resonator.  Jobs of
+    5e6 macroparticles and more (the upper end of the range) also get a
+    2.5D transverse Gaussian space-charge kick after each of 8 transverse
+    segments (their own slicing, the job's slice count); those run element
+    by element.  Host-side bunches, generated up front."""
     import numpy as np
     from scipy.constants import e, m_p
     from pyheadtail_amd.machines.synchrotron import Synchrotron
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.spacecharge.spacecharge import TransverseGaussianSpaceCharge
     rng = np.random.default_rng(seed)
     t, out = 0.0, []
     for j in range(n_jobs):
@@ -296,14 +302,21 @@ def serve_jobs(n_jobs, seed=2000):
         S = int(rng.integers(100, 1001))
         turns = int(np.exp(rng.uniform(np.log(256), np.log(4096))))
         intensity, qp = rng.uniform(1e10, 3e11), rng.uniform(-5., 15.)
+        sc = space_charge and n >= 5e6
         mach = Synchrotron(optics_mode='smooth', charge=e, mass=m_p, p0=p0_of_gamma(27.7),
-                           circumference=6911.5, n_segments=1, beta_x=54.6, beta_y=54.6,
-                           D_x=0., D_y=0., accQ_x=20.13, accQ_y=20.18, Qp_x=qp, Qp_y=qp,
-                           alpha_mom_compaction=1.9e-3, longitudinal_mode='linear', Q_s=0.017)
+                           circumference=6911.5, n_segments=8 if sc else 1, beta_x=54.6,
+                           beta_y=54.6, D_x=0., D_y=0., accQ_x=20.13, accQ_y=20.18, Qp_x=qp,
+                           Qp_y=qp, alpha_mom_compaction=1.9e-3, longitudinal_mode='linear',
+                           Q_s=0.017)
+        if sc:
+            mach.install_after_each_transverse_segment(
+                TransverseGaussianSpaceCharge(UniformBinSlicer(S, n_sigma_z=3), 6911.5 / 8))
         np.random.seed(seed + j)
         b = mach.generate_6D_Gaussian_bunch(n, intensity, EPSN, EPSN, SIGMA_Z)
         host = {k: getattr(b, k).cpu().numpy() for k in ('x', 'xp', 'y', 'yp', 'z', 'dp')}
-        out.append(dict(t=t, mach=mach, S=S, turns=turns, n=n, host=host,
+        print("[serve] job %d generated: %d macroparticles, %d slices, %d turns%s"
+              % (j, n, S, turns, ", space charge" if sc else ""), file=sys.stderr, flush=True)
+        out.append(dict(t=t, mach=mach, S=S, turns=turns, n=n, host=host, sc=sc,
                         params=(b.particlenumber_per_mp, b.charge, b.mass, b.circumference,
                                 b.gamma)))
         del b
@@ -319,8 +332,9 @@ def serve(args):
     from pyheadtail_amd.particles.particles import Particles
     from pyheadtail_amd.particles.slicing import UniformBinSlicer
     from pyheadtail_amd.impedances.wakes import WakeField, CircularResonator
-    jobs = serve_jobs(args.jobs)
+    jobs = serve_jobs(args.jobs, space_charge=not args.no_sc)
     srv = ScanServer(n_streams=args.streams, chunk_turns=64, graph_turns=8)
+    quarter = {}
     t0 = time.perf_counter()
     i = 0
     while i < len(jobs) or srv.busy:
@@ -334,7 +348,16 @@ def serve(args):
             jd['host'] = None
             i += 1
         if srv.busy:
-            srv.step()
+            for j in srv.active:
+                q = 4 * (j.n_turns - j.remaining) // j.n_turns
+                if 0 < q < 4 and quarter.get(j.job_id, 0) < q:
+                    quarter[j.job_id] = q
+                    print("[serve] %.1f s: job %d at %d/4 of its turns"
+                          % (time.perf_counter() - t0, j.job_id, q), file=sys.stderr, flush=True)
+            for f in srv.step():
+                print("[serve] %.1f s: job %d done after %.2f s (%d left)"
+                      % (time.perf_counter() - t0, f.job_id, f.time_to_result,
+                         len(jobs) - len(srv.done)), file=sys.stderr, flush=True)
         elif i < len(jobs):
             time.sleep(max(0.0, jobs[i]['t'] - (time.perf_counter() - t0)))
     makespan = time.perf_counter() - t0
@@ -348,8 +371,10 @@ def serve(args):
         "data": "synthetic (seeded Gaussian bunches, Poisson arrivals at 4 jobs/s)",
         "config": {"workload": "%d jobs: macroparticles log-uniform in [1e5, 1e7], slices in "
                                "[100, 1000], turns log-uniform in [256, 4096]; LinearMap ring "
-                               "with a broadband resonator" % len(jobs),
-                   "jobs": len(jobs), "streams": args.streams},
+                               "with a broadband resonator; jobs of >= 5e6 macroparticles with "
+                               "8 space-charge kicks per turn" % len(jobs),
+                   "jobs": len(jobs), "space_charge_jobs": sum(j['sc'] for j in jobs),
+                   "streams": args.streams},
         "p50_s": p[50], "p99_s": p[99],
         "aggregate_mp_turns_per_s": work / makespan, "makespan_s": makespan,
     }

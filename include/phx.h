@@ -578,11 +578,15 @@ int phx_engine_stage(const phx_engine *e, int32_t stage, phx_stream_t stream);
 /* Records particle 0's coordinates in the workspace: the shifts of the
    shifted sums of the next particle pass (stats.pyx:cov takes the first
    particle), read there by every block while block 0 rewrites particle 0.
-   phx_engine_turns and phx_engine_flush call it themselves. */
+   phx_engine_turns and phx_engine_flush call it themselves; graph replays
+   do not. */
 int phx_engine_snapshot(const phx_engine *e, phx_stream_t stream);
 
 /* HIP graph of n_turns fused turns (captured on an internal stream);
-   replay with phx_graph_launch.  Returns an opaque handle. */
+   replay with phx_graph_launch.  Returns an opaque handle.  The graph does
+   not record particle 0 (each turn refreshes it for the next): call
+   phx_engine_snapshot before the first replay after the coordinates changed
+   outside the engine. */
 int phx_graph_capture(const phx_engine *e, int32_t n_turns, void **graph);
 int phx_graph_launch(void *graph, phx_stream_t stream);
 int phx_graph_destroy(void *graph);

@@ -694,7 +694,8 @@ extern "C" int phx_engine_reset(const phx_engine *e, phx_stream_t stream) {
   return PHX_OK;
 }
 
-extern "C" int phx_engine_turns(const phx_engine *e, int32_t n_turns, phx_stream_t stream) {
+// n_turns turns, after a snapshot of particle 0 if snapshot is set.
+static int engine_turns(const phx_engine *e, int32_t n_turns, hipStream_t s, bool snapshot) {
   int nk, T;
   int rc = check_engine(e, &nk, &T);
   if (rc) return rc;
@@ -703,15 +704,19 @@ extern "C" int phx_engine_turns(const phx_engine *e, int32_t n_turns, phx_stream
   if (rc) return rc;
   const bool has_wake = e->n_slices > 0 && nk > 0;
   EngineArgs a = make_args(e, nk, T);
-  if (n_turns > 0) {
-    rc = enqueue_snapshot(a, phx::as_stream(stream));
+  if (snapshot && n_turns > 0) {
+    rc = enqueue_snapshot(a, s);
     if (rc) return rc;
   }
   for (int t = 0; t < n_turns; ++t) {
-    rc = enqueue_turn(a, has_wake, phx::as_stream(stream));
+    rc = enqueue_turn(a, has_wake, s);
     if (rc) return rc;
   }
   return PHX_OK;
+}
+
+extern "C" int phx_engine_turns(const phx_engine *e, int32_t n_turns, phx_stream_t stream) {
+  return engine_turns(e, n_turns, phx::as_stream(stream), true);
 }
 
 extern "C" int phx_engine_stage(const phx_engine *e, int32_t stage, phx_stream_t stream) {
@@ -763,7 +768,7 @@ extern "C" int phx_graph_capture(const phx_engine *e, int32_t n_turns, void **gr
     (void)hipStreamDestroy(cs);
     return phx::fail((int)err, "phx_graph_capture: begin: %s", hipGetErrorString(err));
   }
-  rc = phx_engine_turns(e, n_turns, cs);
+  rc = engine_turns(e, n_turns, cs, false);  // replays continue from the device state
   err = hipStreamEndCapture(cs, &g);
   (void)hipStreamDestroy(cs);
   if (rc) {

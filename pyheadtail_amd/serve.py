@@ -2,10 +2,13 @@
 
 A job is a one-turn map, its bunch (device-resident from submission) and a
 number of turns.  Jobs wait in arrival order until the HBM budget admits
-them; every step then advances each admitted job by a chunk of turns, the
-jobs spread over a few HIP streams so that their kernels overlap, and a job
-leaves as soon as its turns are done, releasing its share of the budget to
-the queue.  Jobs share no device state: each result is the one the job gets
+them; an admitted job then advances by chunks of turns, each chunk enqueued
+as soon as the job's previous one has completed, the jobs spread over a few
+HIP streams so that their kernels overlap, and a job leaves as soon as its
+turns are done, releasing its share of the budget to
+the queue.  A one-turn map the fused engine takes runs fused; one it does not
+take (space-charge kicks, for one) runs element by element, in smaller
+chunks.  Jobs share no device state: each result is the one the job gets
 alone, bit for bit.
 """
 import collections
@@ -18,9 +21,35 @@ from . import _phx
 from .engine import FusedOneTurnMap
 
 
+class ElementLoop(object):
+    """Turns of a one-turn map element by element on the current stream
+    (the elements' own device kernels)."""
+
+    fused = False
+
+    def __init__(self, one_turn_map):
+        self.one_turn_map = list(one_turn_map)
+
+    def track_turns(self, beam, n_turns):
+        for _ in range(n_turns):
+            for el in self.one_turn_map:
+                el.track(beam)
+
+
+def make_runner(one_turn_map, graph_turns):
+    """The fused engine for one_turn_map if it takes every element, else an
+    element loop."""
+    try:
+        eng = FusedOneTurnMap(one_turn_map, graph_turns=graph_turns)
+    except (TypeError, ValueError):
+        return ElementLoop(one_turn_map)
+    eng.fused = True
+    return eng
+
+
 class Job(object):
     __slots__ = ('job_id', 'one_turn_map', 'bunch', 'n_turns', 'remaining', 'bytes', 'engine',
-                 'stream', 't_submit', 't_admit', 't_done')
+                 'stream', 'event', 't_submit', 't_admit', 't_done')
 
     @property
     def time_to_result(self):
@@ -41,13 +70,16 @@ def job_bytes(one_turn_map, bunch):
 
 class ScanServer(object):
 
-    def __init__(self, hbm_budget_bytes=None, n_streams=4, chunk_turns=64, graph_turns=8):
+    def __init__(self, hbm_budget_bytes=None, n_streams=4, chunk_turns=64, graph_turns=8,
+                 element_chunk_turns=8, pause=2e-4):
         if hbm_budget_bytes is None:
             hbm_budget_bytes = int(0.8 * torch.cuda.get_device_properties(0).total_memory)
         self.budget = int(hbm_budget_bytes)
         self.used = 0
         self.streams = [torch.cuda.Stream() for _ in range(max(1, n_streams))]
         self.chunk_turns, self.graph_turns = int(chunk_turns), int(graph_turns)
+        self.element_chunk_turns = int(element_chunk_turns)
+        self.pause = float(pause)
         self.queue = collections.deque()
         self.active = []
         self.done = []
@@ -69,7 +101,7 @@ class ScanServer(object):
         if j.bytes > self.budget:
             raise ValueError("job %r needs %d bytes, more than the budget %d"
                              % (j.job_id, j.bytes, self.budget))
-        j.engine = j.stream = j.t_admit = j.t_done = None
+        j.engine = j.stream = j.event = j.t_admit = j.t_done = None
         j.t_submit = time.perf_counter()
         self.queue.append(j)
         return j
@@ -78,7 +110,7 @@ class ScanServer(object):
         caller = torch.cuda.current_stream()
         while self.queue and self.used + self.queue[0].bytes <= self.budget:
             j = self.queue.popleft()
-            j.engine = FusedOneTurnMap(j.one_turn_map, graph_turns=self.graph_turns)
+            j.engine = make_runner(j.one_turn_map, self.graph_turns)
             j.stream = self.streams[self._slot % len(self.streams)]
             self._slot += 1
             j.stream.wait_stream(caller)      # the bunch was uploaded on the caller's stream
@@ -87,30 +119,43 @@ class ScanServer(object):
             self.active.append(j)
 
     def step(self):
-        """Admit what the budget allows, advance every active job by one
-        chunk, retire the finished ones; returns them."""
+        """Admit what the budget allows; enqueue the next chunk of every
+        active job whose previous chunk has completed; retire the jobs whose
+        last chunk has completed and return them.  A slow job holds no other
+        job back: when no job can move, the host sleeps for pause seconds
+        and returns (waiting on one job's chunk would stall the jobs whose
+        chunks end first)."""
         self._admit()
-        if not self.active:
-            return []
+        finished, moved = [], False
         for j in self.active:
-            k = min(self.chunk_turns, j.remaining)
+            if j.event is not None and not j.event.query():
+                continue
+            if j.remaining == 0:
+                finished.append(j)
+                continue
+            chunk = self.chunk_turns if j.engine.fused else self.element_chunk_turns
+            k = min(chunk, j.remaining)
             with torch.cuda.stream(j.stream):
                 j.engine.track_turns(j.bunch, k)
                 j.remaining -= k
                 if j.remaining == 0 and j.bunch._pending is j.engine:
                     j.bunch._run_pending()
-        for s in self.streams:
-            s.synchronize()
-        now = time.perf_counter()
-        finished = [j for j in self.active if j.remaining == 0]
-        caller = torch.cuda.current_stream()
-        for j in finished:
-            caller.wait_stream(j.stream)
-            j.t_done = now
-            self.used -= j.bytes
-            j.engine = None                   # its buffers go back to the allocator
-        self.active = [j for j in self.active if j.remaining > 0]
-        self.done.extend(finished)
+                j.event = torch.cuda.Event()
+                j.event.record(j.stream)
+            moved = True
+        if finished:
+            now = time.perf_counter()
+            caller = torch.cuda.current_stream()
+            for j in finished:
+                caller.wait_stream(j.stream)
+                j.t_done = now
+                self.used -= j.bytes
+                j.engine = j.event = None         # its buffers go back to the allocator
+            gone = set(map(id, finished))
+            self.active = [j for j in self.active if id(j) not in gone]
+            self.done.extend(finished)
+        elif not moved and self.active:
+            time.sleep(self.pause)
         return finished
 
     def drain(self):

@@ -44,3 +44,48 @@ def test_server_results_equal_each_job_alone():
                                           err_msg="job %d %s" % (j, k))
     p = srv.latency_percentiles()
     assert p[50] > 0 and p[99] >= p[50]
+
+
+def test_server_mixes_fused_and_element_jobs():
+    """A job with space-charge kicks (not fusable) runs element by element
+    beside fused jobs; every job ends as it does alone."""
+    print("\n[case] scan server: space-charge job (elements) beside two fused jobs", flush=True)
+    import numpy as np
+    from scipy.constants import e, m_p
+    import bench
+    from pyheadtail_amd.engine import FusedOneTurnMap
+    from pyheadtail_amd.machines.synchrotron import Synchrotron
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.impedances.wakes import WakeField, CircularResonator
+    from pyheadtail_amd.spacecharge.spacecharge import TransverseGaussianSpaceCharge
+    from pyheadtail_amd.serve import ScanServer, ElementLoop
+
+    def sc_job():
+        mach = Synchrotron(optics_mode='smooth', charge=e, mass=m_p, p0=bench.p0_of_gamma(27.7),
+                           circumference=6911.5, n_segments=4, beta_x=54.6, beta_y=54.6,
+                           D_x=0., D_y=0., accQ_x=20.13, accQ_y=20.18, Qp_x=3., Qp_y=3.,
+                           alpha_mom_compaction=1.9e-3, longitudinal_mode='linear', Q_s=0.017)
+        mach.install_after_each_transverse_segment(
+            TransverseGaussianSpaceCharge(UniformBinSlicer(64, n_sigma_z=3), 6911.5 / 4))
+        np.random.seed(77)
+        b = mach.generate_6D_Gaussian_bunch(20000, 1e11, bench.EPSN, bench.EPSN, bench.SIGMA_Z)
+        return list(mach.one_turn_map) + [WakeField(UniformBinSlicer(100, n_sigma_z=3),
+                                                    CircularResonator(1e6, 1e9, 1))], b
+
+    turns = [23, 19, 30]
+    m, b = sc_job()
+    ElementLoop(m).track_turns(b, turns[0])
+    alone = [{k: getattr(b, k).cpu().numpy() for k in COORDS}]
+    for i, n in zip((1, 3), turns[1:]):
+        m, b = bench.scan_job(i, n_scale=0.01)
+        FusedOneTurnMap(m, graph_turns=8).track_turns(b, n)
+        alone.append({k: getattr(b, k).cpu().numpy() for k in COORDS})
+    srv = ScanServer(n_streams=2, chunk_turns=16, graph_turns=8, element_chunk_turns=5)
+    jobs = [sc_job(), bench.scan_job(1, n_scale=0.01), bench.scan_job(3, n_scale=0.01)]
+    handles = [srv.submit(m, b, n) for (m, b), n in zip(jobs, turns)]
+    srv.drain()
+    assert [h.engine is None for h in handles] == [True] * 3
+    for j, h in enumerate(handles):
+        for k in COORDS:
+            np.testing.assert_array_equal(getattr(h.bunch, k).cpu().numpy(), alone[j][k],
+                                          err_msg="job %d %s" % (j, k))
