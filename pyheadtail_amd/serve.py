@@ -71,7 +71,7 @@ def job_bytes(one_turn_map, bunch):
 class ScanServer(object):
 
     def __init__(self, hbm_budget_bytes=None, n_streams=4, chunk_turns=64, graph_turns=8,
-                 element_chunk_turns=8, pause=2e-4):
+                 element_chunk_turns=8, pause=2e-4, big_job_bytes=128 << 20):
         if hbm_budget_bytes is None:
             hbm_budget_bytes = int(0.8 * torch.cuda.get_device_properties(0).total_memory)
         self.budget = int(hbm_budget_bytes)
@@ -80,6 +80,7 @@ class ScanServer(object):
         self.chunk_turns, self.graph_turns = int(chunk_turns), int(graph_turns)
         self.element_chunk_turns = int(element_chunk_turns)
         self.pause = float(pause)
+        self.big_job_bytes = int(big_job_bytes)
         self.queue = collections.deque()
         self.active = []
         self.done = []
@@ -106,13 +107,25 @@ class ScanServer(object):
         self.queue.append(j)
         return j
 
+    def _stream_for(self, j):
+        """Jobs of at least big_job_bytes share the first stream, so that two
+        of them never interleave: one such job's coordinates stay in the
+        device's last-level cache across the kernels of its turns, two
+        interleaved ones evict each other (two space-charge jobs of 6e6
+        macroparticles: 6.5 ms per job-turn one after the other, 30-50 ms
+        interleaved).  The others take the remaining streams in turn."""
+        if len(self.streams) == 1 or j.bytes >= self.big_job_bytes:
+            return self.streams[0]
+        st = self.streams[1 + self._slot % (len(self.streams) - 1)]
+        self._slot += 1
+        return st
+
     def _admit(self):
         caller = torch.cuda.current_stream()
         while self.queue and self.used + self.queue[0].bytes <= self.budget:
             j = self.queue.popleft()
             j.engine = make_runner(j.one_turn_map, self.graph_turns)
-            j.stream = self.streams[self._slot % len(self.streams)]
-            self._slot += 1
+            j.stream = self._stream_for(j)
             j.stream.wait_stream(caller)      # the bunch was uploaded on the caller's stream
             j.t_admit = time.perf_counter()
             self.used += j.bytes
