@@ -58,9 +58,11 @@ def parse():
     ap.add_argument("--workload", choices=("resonator", "rw64", "scan64", "serve"),
                     default="resonator")
     ap.add_argument("--jobs", type=int, default=48)
+    ap.add_argument("--big-job-mib", type=float, default=128.,
+                    help="batched and served jobs of this many MiB or more share one stream")
     ap.add_argument("--no-sc", action="store_true",
                     help="served stream without the space-charge kicks of the largest jobs")
-    ap.add_argument("--streams", type=int, default=4)
+    ap.add_argument("--streams", type=int, default=12)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of the pass's algorithmic bytes "
                          "(48 B read + 48 B written per macroparticle), for scaling the "
@@ -253,7 +255,8 @@ def scan64(args):
     from pyheadtail_amd.batch import BatchedScan
     jobs = [scan_job(i) for i in range(64)]
     scan = BatchedScan(jobs, n_streams=args.streams, chunk_turns=50,
-                       graph_turns=args.graph_turns)
+                       graph_turns=args.graph_turns,
+                       big_job_bytes=int(args.big_job_mib * (1 << 20)))
     n_mp = scan.macroparticles
     steps = min(args.steps, 200)
     scan.track_turns(40)                      # graphs, histories (T <= 32) filled
@@ -333,7 +336,8 @@ def serve(args):
     from pyheadtail_amd.particles.slicing import UniformBinSlicer
     from pyheadtail_amd.impedances.wakes import WakeField, CircularResonator
     jobs = serve_jobs(args.jobs, space_charge=not args.no_sc)
-    srv = ScanServer(n_streams=args.streams, chunk_turns=64, graph_turns=8)
+    srv = ScanServer(n_streams=args.streams, chunk_turns=64, graph_turns=8,
+                     big_job_bytes=int(args.big_job_mib * (1 << 20)))
     quarter = {}
     t0 = time.perf_counter()
     i = 0

@@ -6,17 +6,22 @@ in chunks of turns: each job's chunk is replayed from its graph on one of
 n_streams HIP streams (default priority; each first waits for the caller's
 stream, which made the bunches), so the short, latency-bound
 per-turn kernels of different bunches overlap on the device instead of
-running one bunch after another.  Jobs share no device state, so every
-job's result is the one it gets alone, bit for bit.
+running one bunch after another.  Jobs of at least big_job_bytes share
+the first stream (two of them interleaved evict each other's coordinates
+from the last-level cache); the others take the remaining streams in turn.
+Jobs share no device state, so every job's result is the one it gets alone,
+bit for bit.
 """
 import torch
 
 from .engine import FusedOneTurnMap
+from .serve import job_bytes
 
 
 class BatchedScan(object):
 
-    def __init__(self, jobs, n_streams=4, chunk_turns=100, graph_turns=10, record_stats=False):
+    def __init__(self, jobs, n_streams=8, chunk_turns=100, graph_turns=10, record_stats=False,
+                 big_job_bytes=128 << 20):
         """jobs: list of (one_turn_map, bunch); the one-turn maps must be
         distinct objects (each keeps its wake history)."""
         self.bunches = [b for _, b in jobs]
@@ -24,6 +29,14 @@ class BatchedScan(object):
                         for m, _ in jobs]
         self.streams = [torch.cuda.Stream() for _ in range(max(1, n_streams))]
         self.chunk_turns = int(chunk_turns)
+        ns, k = len(self.streams), 0
+        self.job_stream = []
+        for m, b in jobs:
+            if ns == 1 or job_bytes(m, b) >= big_job_bytes:
+                self.job_stream.append(self.streams[0])
+            else:
+                self.job_stream.append(self.streams[1 + k % (ns - 1)])
+                k += 1
 
     def track_turns(self, n_turns):
         """Advance every job by n_turns turns; returns when all are done."""
@@ -34,11 +47,11 @@ class BatchedScan(object):
         while done < n_turns:
             k = min(self.chunk_turns, n_turns - done)
             for i, (eng, b) in enumerate(zip(self.engines, self.bunches)):
-                with torch.cuda.stream(self.streams[i % len(self.streams)]):
+                with torch.cuda.stream(self.job_stream[i]):
                     eng.track_turns(b, k)
             done += k
         for i, (eng, b) in enumerate(zip(self.engines, self.bunches)):
-            with torch.cuda.stream(self.streams[i % len(self.streams)]):
+            with torch.cuda.stream(self.job_stream[i]):
                 if b._pending is eng:
                     b._run_pending()
         for s in self.streams:

@@ -70,7 +70,7 @@ def job_bytes(one_turn_map, bunch):
 
 class ScanServer(object):
 
-    def __init__(self, hbm_budget_bytes=None, n_streams=4, chunk_turns=64, graph_turns=8,
+    def __init__(self, hbm_budget_bytes=None, n_streams=8, chunk_turns=64, graph_turns=8,
                  element_chunk_turns=8, pause=2e-4, big_job_bytes=128 << 20):
         if hbm_budget_bytes is None:
             hbm_budget_bytes = int(0.8 * torch.cuda.get_device_properties(0).total_memory)
