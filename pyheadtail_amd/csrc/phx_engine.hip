@@ -432,7 +432,7 @@ __global__ void __launch_bounds__(RD_SL * RD_G) k_engine_merge(EngineArgs a) {
   mm[2 * S + s] = cnt * my;
 }
 
-__global__ void __launch_bounds__(PHX_BLOCK) k_engine_conv(EngineArgs a) {
+__global__ void __launch_bounds__(WK_THREADS) k_engine_conv(EngineArgs a) {
   const phx_engine &e = a.e;
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
   const phx_turn_program &g = e.programs[e.turn_program[e.state->turn]];
@@ -632,7 +632,7 @@ int enqueue_stage(const EngineArgs &a, int stage, bool has_wake, hipStream_t s) 
   }
   if (e.conv_kinds == 0 || (e.conv_kinds & PHX_CONV_GENERAL)) {
     const int tiles = (e.n_slices + WK_TILE - 1) / WK_TILE;
-    k_engine_conv<<<dim3(tiles, a.n_kicks, a.T), PHX_BLOCK,
+    k_engine_conv<<<dim3(tiles, a.n_kicks, a.T), WK_THREADS,
                     wake_conv_lds_bytes(e.n_slices, a.gmax), s>>>(a);
     PHX_CHECK_LAUNCH("phx_engine: conv");
   }

@@ -8,7 +8,8 @@
 #include "phx_device.h"
 
 #define WK_TILE 32                   // target slices per block
-#define WK_GROUPS (PHX_BLOCK / WK_TILE)
+#define WK_THREADS 1024              // threads per block: at most one wake value each
+#define WK_GROUPS (WK_THREADS / WK_TILE)
 
 // Evaluation of the wake function with the reference's operation order, in
 // two parts: the shape, which kicks of the same source share (they differ
@@ -228,7 +229,7 @@ __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const Wa
   // LDS, convolution form: [WK_TILE + S - 1] wake values, then [S] source
   // moments; dot-product form: [S] source times, then [gmax][S] moments.
   extern __shared__ double vbuf[];
-  __shared__ double red[WK_GMAX][PHX_BLOCK];
+  __shared__ double red[WK_GMAX][WK_THREADS];
   __shared__ double rwp[WK_GMAX][3];
   const int tile = blockIdx.x, ki = blockIdx.y, t = blockIdx.z, nk = gridDim.y;
   const phx_wake_kick &w = kk.k[ki];
@@ -596,7 +597,7 @@ __device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const Wake
 
 // Static LDS of wake_conv_block; its dynamic LDS serves up to gmax kicks per
 // block in the dot-product form.
-#define WK_STATIC_LDS ((WK_GMAX * PHX_BLOCK + 3 * WK_GMAX) * sizeof(double))
+#define WK_STATIC_LDS ((WK_GMAX * WK_THREADS + 3 * WK_GMAX) * sizeof(double))
 inline size_t wake_conv_lds_bytes(int S, int gmax) {
   const size_t conv = (size_t)(WK_TILE + 2 * S - 1), dot = (size_t)(1 + gmax) * S;
   return (conv > dot ? conv : dot) * sizeof(double);

@@ -12,7 +12,7 @@ __global__ void k_hist_push(phx_wake_hist *h, const double *times, const int32_t
   hist_push_device(h, times, counts, mean_x, mean_y, age_inc, beta);
 }
 
-__global__ void __launch_bounds__(PHX_BLOCK)
+__global__ void __launch_bounds__(WK_THREADS)
     k_wake_conv(const phx_wake_hist *h, WakeKicks kk, int S, double *part, int T, double *out,
                 int gmax) {
   wake_conv_block(h, kk, S, part, T, out, gmax);
@@ -120,7 +120,7 @@ int phx_wake_convolve_ws(const phx_wake_hist *hist, const phx_wake_kick *kicks, 
   if (rw && wake_rw_lds_bytes(n_slices) + RW_STATIC_LDS > phx_max_lds_bytes())
     return phx::fail(PHX_E_LIMIT, "phx_wake_convolve: n_slices=%d too large for LDS", n_slices);
   if (general) {
-    k_wake_conv<<<dim3(tiles, n_kicks, T), PHX_BLOCK, wake_conv_lds_bytes(n_slices, gmax), s>>>(
+    k_wake_conv<<<dim3(tiles, n_kicks, T), WK_THREADS, wake_conv_lds_bytes(n_slices, gmax), s>>>(
         hist, kk, n_slices, part, T, kicks_out, gmax);
     PHX_CHECK_LAUNCH("phx_wake_convolve");
   }
