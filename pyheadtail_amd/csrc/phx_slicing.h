@@ -197,7 +197,7 @@ __device__ __forceinline__ void acc_lds_store(const AccLds &L, int S, int T, con
 // combined by lane exchanges within each wave and across the waves in LDS.
 // The sums are integers, so the order of the additions does not matter.
 // Threads 0 .. RD_SL-1 receive the totals of their slice.
-#define RD_SL 8
+#define RD_SL 4
 #define RD_G (PHX_BLOCK / RD_SL)
 
 struct SliceTotals {
