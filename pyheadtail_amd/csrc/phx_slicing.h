@@ -191,12 +191,13 @@ __device__ __forceinline__ void acc_lds_store(const AccLds &L, int S, int T, con
   }
 }
 
-// Merge of the H partial histograms for a tile of RD_SL slices, one block of
-// PHX_BLOCK threads: thread t takes slice t % RD_SL and the partials
-// h = t / RD_SL, t / RD_SL + RD_G, ...; the RD_G groups of a slice are then
-// combined by lane exchanges within each wave and across the waves in LDS.
-// The sums are integers, so the order of the additions does not matter.
-// Threads 0 .. RD_SL-1 receive the totals of their slice.
+// Merge of the H partial histograms for a tile of SL slices (RD_SL unless
+// the caller picks another), one block of PHX_BLOCK threads: thread t takes
+// slice t % SL and the partials h = t / SL, t / SL + G, ... (G = PHX_BLOCK /
+// SL); the G groups of a slice are then combined by lane exchanges within
+// each wave and across the waves in LDS.  The sums are integers, so the
+// order of the additions does not matter.  Threads 0 .. SL-1 receive the
+// totals of their slice.
 #define RD_SL 4
 #define RD_G (PHX_BLOCK / RD_SL)
 
@@ -205,19 +206,19 @@ struct SliceTotals {
   unsigned long long hi[PHX_MAX_TERMS], lo[PHX_MAX_TERMS];
 };
 
-template <int T>
+template <int T, int SL = RD_SL>
 __device__ __forceinline__ bool acc_merge(const AccParts &P, int S, SliceTotals &out) {
   constexpr int NVAL = 2 * T + 1;
   constexpr int NW = PHX_BLOCK / PHX_WAVE;
-  __shared__ unsigned long long wsum[NW][NVAL][RD_SL];
-  const int sl = threadIdx.x % RD_SL, g = threadIdx.x / RD_SL;
-  const int s = blockIdx.x * RD_SL + sl;
+  __shared__ unsigned long long wsum[NW][NVAL][SL];
+  const int sl = threadIdx.x % SL, g = threadIdx.x / SL;
+  const int s = blockIdx.x * SL + sl;
   // v[0] = count, v[1 + 2i] = hi of term i, v[2 + 2i] = lo of term i
   unsigned long long v[NVAL];
 #pragma unroll
   for (int i = 0; i < NVAL; ++i) v[i] = 0ULL;
   if (s < S) {
-    for (int h = g; h < P.H; h += RD_G) {
+    for (int h = g; h < P.H; h += (PHX_BLOCK / SL)) {
       v[0] += P.cnt[(size_t)h * S + s];
 #pragma unroll
       for (int i = 0; i < T; ++i) {
@@ -227,18 +228,18 @@ __device__ __forceinline__ bool acc_merge(const AccParts &P, int S, SliceTotals 
       }
     }
   }
-  // lanes of one slice differ in the bits above log2(RD_SL)
+  // lanes of one slice differ in the bits above log2(SL)
 #pragma unroll
-  for (int off = RD_SL; off < PHX_WAVE; off <<= 1)
+  for (int off = SL; off < PHX_WAVE; off <<= 1)
 #pragma unroll
     for (int i = 0; i < NVAL; ++i) v[i] += __shfl_xor(v[i], off);
   const int lane = threadIdx.x % PHX_WAVE, wv = threadIdx.x / PHX_WAVE;
-  if (lane < RD_SL) {
+  if (lane < SL) {
 #pragma unroll
     for (int i = 0; i < NVAL; ++i) wsum[wv][i][lane] = v[i];
   }
   lds_sync();
-  if (threadIdx.x >= RD_SL || s >= S) return false;
+  if (threadIdx.x >= SL || s >= S) return false;
   unsigned long long tot[NVAL];
 #pragma unroll
   for (int i = 0; i < NVAL; ++i) {
