@@ -13,8 +13,13 @@ with the bunch resident in device memory.
 The roofline entry is that of the dominant kernel (the fused particle pass):
 96 algorithmic bytes per macroparticle (read and write of six FP64
 coordinates) divided by its mean duration, measured with HIP events on its
-launch stream.  The CPU baseline is the oracle's NumPy restatement of the
-reference CPU path on one host core, timed on a bounded sample.
+launch stream; turn_roofline_frac is the same 96 bytes over the whole turn.
+A second leg ("large") runs the same map on 1e7 macroparticles, whose 480 MB
+of coordinates exceed the 256 MB last-level cache.  The CPU baseline is the
+oracle's multi-core C++ restatement of the reference CPU path on all host
+threads; cpu_baseline_numpy_1core is its NumPy restatement on one core (the
+reference's CPU path is single-threaded NumPy).  Both run on bounded
+samples of the 1e6 workload.
 
 Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 50]
 
@@ -50,7 +55,13 @@ def parse():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=1000000)
     ap.add_argument("--slices", type=int, default=500)
-    ap.add_argument("--cpu-turns", type=int, default=150)
+    ap.add_argument("--cpu-turns", type=int, default=30,
+                    help="turns of the one-core NumPy baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=10.,
+                    help="length of the all-core C++ baseline's sample")
+    ap.add_argument("--n-large", type=int, default=10000000,
+                    help="macroparticles of the second, cache-exceeding leg (0: none)")
+    ap.add_argument("--steps-large", type=int, default=100)
     ap.add_argument("--cpu-n", type=int, default=1000000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-turns", type=int, default=100)
@@ -84,9 +95,47 @@ def copy_calibration(n, reps=5):
                                            "bytes_written": dst.numel() * 8, "reps": reps}}))
 
 
-def cpu_baseline(n, turns):
-    """The oracle on one host core (runs before the GPU is touched)."""
-    import numpy as np
+def _cpu_threads():
+    """Host threads for the all-core baseline: the process's CPU set, capped
+    by OMP_NUM_THREADS when that is set (the GPU machine's CPU share)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_baseline(n, seconds):
+    """The oracle's multi-core C++ restatement of the reference CPU path on
+    all host threads, on a bounded sample of the same workload: as many
+    turns of the n-macroparticle bunch as take about `seconds`.  Runs before
+    the GPU is touched."""
+    from oracle import workloads as W
+    from oracle import cpu_turn as CT
+    ob = W.nonlinear_bunch(n, seed=SEED)
+    ct = CT.CpuTurns(ob, n_threads=_cpu_threads())
+    ct.track_turns(2)                         # untimed: pages, thread pool
+    t0 = time.perf_counter()
+    ct.track_turns(3)
+    rate = 3 / (time.perf_counter() - t0)
+    turns = max(5, int(seconds * rate))
+    t0 = time.perf_counter()
+    ct.track_turns(turns)
+    ct.flush()
+    dt = time.perf_counter() - t0
+    return dict(value=n * turns / dt, unit="macroparticle-turns/s", cores=ct.threads,
+                kind="port",
+                sample="%d macroparticles x %d turns of the same one-turn map (C++ restatement "
+                       "of the reference CPU path, oracle/cpu_turn.cpp, %d threads, %.1f s)"
+                       % (n, turns, ct.threads, dt))
+
+
+def cpu_baseline_numpy(n, turns):
+    """The oracle's NumPy restatement of the reference CPU path (which is
+    single-threaded NumPy) on one host core."""
     try:
         old = os.sched_getaffinity(0)
         os.sched_setaffinity(0, {sorted(old)[0]})
@@ -112,22 +161,44 @@ def cpu_baseline(n, turns):
                        % (n, turns, dt))
 
 
+TRAFFIC_SOURCES = ("pyheadtail_amd/csrc", "include")
+
+
+def sources_sha256():
+    """Digest of the HIP sources and the ABI header the library is built
+    from, which a traffic record must carry to be used."""
+    import hashlib
+    h = hashlib.sha256()
+    for d in TRAFFIC_SOURCES:
+        base = os.path.join(ROOT, d)
+        for name in sorted(os.listdir(base)):
+            if name.endswith((".hip", ".h")):
+                h.update(name.encode())
+                with open(os.path.join(base, name), "rb") as f:
+                    h.update(f.read())
+    return h.hexdigest()
+
+
 def pass_traffic(n):
-    """HBM bytes per launch of the particle pass from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE runs (profiles/pass_traffic.json), each scaled by
-    what the counter reads for a plain copy of known size; None if the
-    figures are for another size."""
+    """HBM bytes per launch of the particle pass from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs recorded in profiles/pass_traffic.json, each
+    scaled by what the counter reads for a plain copy of known size; None
+    unless the record is for n macroparticles and for the sources this
+    library is built from."""
     path = os.path.join(ROOT, "profiles", "pass_traffic.json")
     try:
         with open(path) as f:
-            t = json.load(f)
+            recs = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("macroparticles") != n:
-        return None
-    read = t["fetch_kib"] * 1024 * t["copy_bytes"] / (t["copy_fetch_kib"] * 1024)
-    written = t["write_kib"] * 1024 * t["copy_bytes"] / (t["copy_write_kib"] * 1024)
-    return read + written
+    sha = sources_sha256()
+    for t in recs if isinstance(recs, list) else [recs]:
+        if t.get("macroparticles") != n or t.get("sources_sha256") != sha:
+            continue
+        read = t["fetch_kib"] * t["copy_bytes"] / t["copy_fetch_kib"]
+        written = t["write_kib"] * t["copy_bytes"] / t["copy_write_kib"]
+        return read + written
+    return None
 
 
 def p0_of_gamma(gamma):
@@ -219,33 +290,47 @@ def rw64(args):
     print(json.dumps(out))
 
 
-def scan_job(i, n_scale=1.0, seed_base=1000):
-    """Job i of the batched scan: macroparticles log-uniform in [2e5, 4e6],
-    slices in {100, 250, 500, 1000}, intensity uniform in [1e10, 3e11],
-    chromaticity uniform in [-5, 15], n_turns_wake in {1, 8, 32}; the
+def scan_params(i, seed_base=1000):
+    """Parameters of batched-scan job i: macroparticles log-uniform in
+    [2e5, 4e6], slices in {100, 250, 500, 1000}, intensity uniform in
+    [1e10, 3e11], chromaticity uniform in [-5, 15], n_turns_wake in
+    {1, 8, 32}; drawn from a generator seeded by seed_base + i."""
+    import numpy as np
+    rng = np.random.default_rng(seed_base + i)
+    n = np.exp(rng.uniform(np.log(2e5), np.log(4e6)))
+    S = int(rng.choice([100, 250, 500, 1000]))
+    intensity = rng.uniform(1e10, 3e11)
+    qp = rng.uniform(-5., 15.)
+    T = int(rng.choice([1, 8, 32]))
+    return dict(n=n, n_slices=S, intensity=intensity, qp=qp, n_turns_wake=T)
+
+
+def scan_job(i, n_scale=1.0, seed_base=1000, bunch=True):
+    """Job i of the batched scan (parameters from scan_params): the
     LinearMap ring with a broadband resonator (one turn of memory) or the
-    parallel-plates resistive wall (8 or 32 turns).  Seeded by seed_base + i."""
+    parallel-plates resistive wall (8 or 32 turns), and its Gaussian bunch
+    seeded by seed_base + i (None unless bunch)."""
     import numpy as np
     from scipy.constants import e, m_p
     from pyheadtail_amd.machines.synchrotron import Synchrotron
     from pyheadtail_amd.particles.slicing import UniformBinSlicer
     from pyheadtail_amd.impedances.wakes import (WakeField, CircularResonator,
                                                  ParallelHorizontalPlatesResistiveWall)
-    rng = np.random.default_rng(seed_base + i)
-    n = int(np.exp(rng.uniform(np.log(2e5), np.log(4e6))) * n_scale)
-    S = int(rng.choice([100, 250, 500, 1000]))
-    intensity = rng.uniform(1e10, 3e11)
-    qp = rng.uniform(-5., 15.)
-    T = int(rng.choice([1, 8, 32]))
+    p = scan_params(i, seed_base)
+    n = int(p['n'] * n_scale)
+    S, T, qp = p['n_slices'], p['n_turns_wake'], p['qp']
     mach = Synchrotron(optics_mode='smooth', charge=e, mass=m_p, p0=p0_of_gamma(27.7),
                        circumference=6911.5, n_segments=1, beta_x=54.6, beta_y=54.6, D_x=0.,
                        D_y=0., accQ_x=20.13, accQ_y=20.18, Qp_x=qp, Qp_y=qp,
                        alpha_mom_compaction=1.9e-3, longitudinal_mode='linear', Q_s=0.017)
     src = (CircularResonator(1e6, 1e9, 1) if T == 1 else
            ParallelHorizontalPlatesResistiveWall(0.02, 6911.5, 5.9e7, 1e-12, n_turns_wake=T))
+    one_turn = list(mach.one_turn_map) + [WakeField(UniformBinSlicer(S, n_sigma_z=3), src)]
+    if not bunch:
+        return one_turn, None
     np.random.seed(seed_base + i)
-    bunch = mach.generate_6D_Gaussian_bunch(n, intensity, EPSN, EPSN, SIGMA_Z)
-    return list(mach.one_turn_map) + [WakeField(UniformBinSlicer(S, n_sigma_z=3), src)], bunch
+    b = mach.generate_6D_Gaussian_bunch(n, p['intensity'], EPSN, EPSN, SIGMA_Z)
+    return one_turn, b
 
 
 def scan64(args):
@@ -385,6 +470,62 @@ def serve(args):
     print(json.dumps(out))
 
 
+def headline_turns(n, args, steps, stage_turns):
+    """The fused one-turn map on an n-macroparticle bunch: warm-up (graph
+    capture), per-stage kernel times, then `steps` timed turns.  Returns the
+    timed seconds, the stage times and the host-to-device copy time."""
+    import numpy as np
+    import torch
+    from pyheadtail_amd.engine import FusedOneTurnMap
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.impedances.wakes import WakeField, CircularResonator
+
+    mach = build_machine(p0_of_gamma(27.7))
+    wake = WakeField(UniformBinSlicer(args.slices, n_sigma_z=3), CircularResonator(1e6, 1e9, 1))
+    one_turn = list(mach.one_turn_map) + [wake]
+    # the Gaussian bunch cut at the RF separatrix, drawn with NumPy's legacy
+    # global RNG as PyHEADTAIL does (machines/synchrotron.py:309-360)
+    np.random.seed(SEED)
+    bunch = mach.generate_6D_Gaussian_bunch(n, INTENSITY, EPSN, EPSN, SIGMA_Z)
+    # the one host-to-device copy of the bunch (pageable NumPy arrays, as the
+    # Particles constructor receives them); not part of the timed region
+    host_coords = {k: getattr(bunch, k).cpu().numpy() for k in
+                   ('x', 'xp', 'y', 'yp', 'z', 'dp')}
+    torch.cuda.synchronize()
+    t_h2d = time.perf_counter()
+    bunch.update(host_coords)
+    torch.cuda.synchronize()
+    h2d_ms = (time.perf_counter() - t_h2d) * 1e3
+    del host_coords
+    eng = FusedOneTurnMap(one_turn, graph_turns=args.graph_turns)
+    # buffers for the longest call, so that no later call re-allocates them
+    # or captures its graph again
+    eng.reserve(bunch, max(steps, stage_turns, args.warmup, args.graph_turns))
+    # warm-up (the graph is captured here)
+    eng.track_turns(bunch, max(args.warmup, args.graph_turns))
+    bunch.x  # applies the pending kick
+    torch.cuda.synchronize()
+    # per-stage kernel durations with HIP events on the launch stream
+    stage_ms = eng.timed_turns(bunch, stage_turns)
+    bunch.x
+    torch.cuda.synchronize()
+    # timed region: `steps` turns, graph-launched, then the final kick
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.track_turns(bunch, steps)
+    bunch.x
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return dt, stage_ms, h2d_ms
+
+
+def roofline(n, kernel_ms, kernel):
+    achieved = BYTES_PER_MP_PASS * n / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": pass_traffic(n), "kernel": kernel,
+            "algorithmic_bytes_per_launch": BYTES_PER_MP_PASS * n}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -404,52 +545,16 @@ def main():
     if args.workload == "serve":
         serve(args)
         return
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_turns)
+    cpu = cpu_np = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cpu_n, args.cpu_seconds)
+        cpu_np = cpu_baseline_numpy(args.cpu_n, args.cpu_turns)
 
-    import numpy as np
     import torch
     torch.cuda.set_device(0)
-    from pyheadtail_amd.engine import FusedOneTurnMap
-    from pyheadtail_amd.particles.slicing import UniformBinSlicer
-    from pyheadtail_amd.impedances.wakes import WakeField, CircularResonator
-
-    mach = build_machine(p0_of_gamma(27.7))
-    wake = WakeField(UniformBinSlicer(args.slices, n_sigma_z=3), CircularResonator(1e6, 1e9, 1))
-    one_turn = list(mach.one_turn_map) + [wake]
-    # the Gaussian bunch cut at the RF separatrix, drawn with NumPy's legacy
-    # global RNG as PyHEADTAIL does (machines/synchrotron.py:309-360)
-    np.random.seed(SEED)
-    bunch = mach.generate_6D_Gaussian_bunch(args.n, INTENSITY, EPSN, EPSN, SIGMA_Z)
-    # the one host-to-device copy of the bunch (pageable NumPy arrays, as the
-    # Particles constructor receives them); not part of the timed region
-    host_coords = {k: getattr(bunch, k).cpu().numpy() for k in
-                   ('x', 'xp', 'y', 'yp', 'z', 'dp')}
-    torch.cuda.synchronize()
-    t_h2d = time.perf_counter()
-    bunch.update(host_coords)
-    torch.cuda.synchronize()
-    h2d_ms = (time.perf_counter() - t_h2d) * 1e3
-    eng = FusedOneTurnMap(one_turn, graph_turns=args.graph_turns)
-
-    # warm-up (graph capture happens here)
-    eng.track_turns(bunch, max(args.warmup, args.graph_turns))
-    bunch.x  # applies the pending kick
-    torch.cuda.synchronize()
-
-    # per-stage kernel durations with HIP events on the launch stream
-    stage_ms = eng.timed_turns(bunch, args.stage_turns)
-    bunch.x
-    torch.cuda.synchronize()
-
-    # timed region: K turns, graph-launched, then the final kick
-    t0 = time.perf_counter()
-    eng.track_turns(bunch, args.steps)
-    bunch.x
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-
+    dt, stage_ms, h2d_ms = headline_turns(args.n, args, args.steps, args.stage_turns)
     value = args.n * args.steps / dt
-    achieved = BYTES_PER_MP_PASS * args.n / (stage_ms[0] * 1e-3) / 1e9
+    turn_ms = sum(stage_ms)
     out = {
         "metric": "macroparticle-turns/s @1e6 MP+wake; %HBM roofline; batched-scan p99 latency",
         "value": value,
@@ -469,14 +574,25 @@ def main():
                                % (args.n, args.slices),
                    "macroparticles": args.n, "n_slices": args.slices, "turns": args.steps,
                    "graph_turns": args.graph_turns},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": pass_traffic(args.n),
-                     "kernel": "k_engine_pass (fused particle pass)",
-                     "algorithmic_bytes_per_launch": BYTES_PER_MP_PASS * args.n},
+        "roofline": roofline(args.n, stage_ms[0], "k_engine_pass (fused particle pass)"),
+        "turn_roofline_frac": BYTES_PER_MP_PASS * args.n / (turn_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
         "stage_ms": {"pass": stage_ms[0], "hist": stage_ms[1], "conv": stage_ms[2]},
         "h2d_bunch_ms": h2d_ms,
         "cpu_baseline": cpu,
+        "cpu_baseline_numpy_1core": cpu_np,
     }
+    if args.n_large > 0:
+        # the same map on a bunch ten times larger than the last-level cache
+        steps_l = min(args.steps, args.steps_large)
+        dt_l, st_l, _ = headline_turns(args.n_large, args, steps_l, min(args.stage_turns, 50))
+        out["large"] = {
+            "macroparticles": args.n_large, "turns": steps_l,
+            "value": args.n_large * steps_l / dt_l, "ms_per_step": dt_l / steps_l * 1e3,
+            "stage_ms": {"pass": st_l[0], "hist": st_l[1], "conv": st_l[2]},
+            "roofline": roofline(args.n_large, st_l[0], "k_engine_pass (fused particle pass)"),
+            "turn_roofline_frac": (BYTES_PER_MP_PASS * args.n_large / (sum(st_l) * 1e-3) / 1e9 /
+                                   HBM_PEAK_GBS),
+        }
     print(json.dumps(out))
 
 

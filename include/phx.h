@@ -209,6 +209,7 @@ typedef struct {
     double gx, ax2, bx, gy, ay2, by;
     double dapp_x, dapp_y, dapp_xy, dapp_yx;
     double p0;                /* beam.p0 at track time */
+    double inv_p0;            /* 1.0 / p0 (IEEE division) */
 } phx_detuner;
 
 typedef struct {
@@ -247,6 +248,10 @@ typedef struct {
     double harmonic, two_pi, circumference, phi_offset, phi_lock;
     double amplitude, p_increment, p0_before, p0_after;
     double D_x, D_y;
+    /* 1.0 / circumference and 1.0 / p0_after (IEEE divisions): the kick's
+       two divisions are formed from them and rounded exactly as the
+       divisions themselves */
+    double inv_circumference, inv_p0_after;
 } phx_long_op;
 
 typedef struct {
@@ -322,11 +327,24 @@ enum { PHX_TABLE_TRANS_UR = 0, PHX_TABLE_TRANS_LOWBETA = 1, PHX_TABLE_LONG_BL = 
 enum { PHX_MOM_N = 0, PHX_MOM_X = 1, PHX_MOM_Y = 2 };
 enum { PHX_PLANE_XP = 0, PHX_PLANE_YP = 1, PHX_PLANE_DP = 2 };
 /* Arithmetic of a multi-turn (dot-product) resistive-wall convolution:
-   FP64 throughout, or FP32 wake values (from the FP64 time differences) and
-   FP32 products summed in FP32 over each run of at most ceil(n_slices/8)
-   sources, those partial sums added in FP64.  Every other convolution is
-   FP64 whatever the field says. */
-enum { PHX_PREC_FP64 = 0, PHX_PREC_FP32 = 1 };
+     FP64         FP64; the source turns gone by (every pair beyond |dt_min|,
+                  time offsets within 1e-3 of the distance of the two
+                  slicings) through a six-term separable expansion of
+                  (-dt)^(-1/2), exact to below 1e-18 of the sum of the
+                  terms' magnitudes (O(n_slices) per turn); the current turn
+                  pair by pair;
+     FP32         FP32 wake values (from the FP64 time differences) and FP32
+                  products summed in FP32 over each run of at most
+                  ceil(n_slices/8) sources, those partial sums added in FP64;
+     FP64_DIRECT  FP64 pair by pair for every turn (the reference's sums);
+     MFMA_F32     every pair on the matrix cores: 4x4x1 FP32 MFMA, FP32 wake
+                  values and moments, FP32 accumulation over 256 sources,
+                  FP64 beyond;
+     MFMA_BF16X3  the same with 4x4x4 bfloat16 MFMA on split operands
+                  (hi + lo for wake values and moments, three products).
+   Every other convolution is FP64 whatever the field says. */
+enum { PHX_PREC_FP64 = 0, PHX_PREC_FP32 = 1, PHX_PREC_FP64_DIRECT = 2, PHX_PREC_MFMA_F32 = 3,
+       PHX_PREC_MFMA_BF16X3 = 4 };
 
 typedef struct {
     int32_t fn;               /* PHX_WAKE_* */
@@ -506,8 +524,15 @@ enum {
 };
 /* Wake kick sets with a compiled kick step (other sets: PHX_KICKS_GENERIC):
    DXDY is [dipole x (xp += k), dipole y (yp += k)], as a round chamber's
-   resonator or resistive wall gives. */
-enum { PHX_KICKS_GENERIC = 0, PHX_KICKS_DXDY = 1, PHX_KICKS_COUNT = 2 };
+   resonator or resistive wall gives; PLATES is [dipole x, quadrupole x
+   (xp += k x), dipole y, quadrupole y (yp += k y)], a parallel-plates
+   chamber's. */
+enum { PHX_KICKS_GENERIC = 0, PHX_KICKS_DXDY = 1, PHX_KICKS_PLATES = 2, PHX_KICKS_COUNT = 3 };
+/* Detuner sets of the transverse segments with compiled code (other sets:
+   PHX_DETS_GENERIC): none, first-order chromaticity, first-order
+   chromaticity then amplitude detuning (Synchrotron's order). */
+enum { PHX_DETS_GENERIC = 0, PHX_DETS_NONE = 1, PHX_DETS_C1 = 2, PHX_DETS_C1_AMP = 3,
+       PHX_DETS_COUNT = 4 };
 /* Wake convolution kernels: resistive-wall kicks in the dot-product form
    (multi-turn memory) have their own; every other kick is GENERAL. */
 enum { PHX_CONV_GENERAL = 1, PHX_CONV_RW_DOT = 2 };
@@ -534,6 +559,10 @@ typedef struct {
     int32_t conv_kinds;                 /* PHX_CONV_* bits: the convolution
                                            kernels the programs' kicks need;
                                            0: both (always valid) */
+    int32_t det_shape;                  /* PHX_DETS_*: the detuners of every
+                                           program's transverse segments
+                                           (GENERIC is always valid) */
+    int32_t reserved;                   /* 0 */
     const phx_turn_program *programs;   /* [n_programs] */
     const int32_t *turn_program;        /* [n_turns_total] -> program index */
     double *stats;                      /* raw moment sums per turn, see below */
@@ -581,6 +610,23 @@ int phx_engine_stage(const phx_engine *e, int32_t stage, phx_stream_t stream);
    phx_engine_turns and phx_engine_flush call it themselves; graph replays
    do not. */
 int phx_engine_snapshot(const phx_engine *e, phx_stream_t stream);
+
+/* Allocation-free variant of phx_engine_turns.  It has no counterpart in
+   the reference: it is for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  e->ws is ignored: the workspace is the library's own static
+   device scratch of PHX_NOALLOC_SCRATCH_BYTES per device (held until the
+   process ends); an engine whose phx_engine_ws_bytes exceeds it is
+   rejected with PHX_E_LIMIT.  flags: PHX_NOALLOC_RESET resets the engine
+   state first (as phx_engine_reset), PHX_NOALLOC_FLUSH applies the pending
+   wake kick after the turns (as phx_engine_flush).  Every call on a device
+   shares that device's scratch and leaves it in the state of its last
+   turn: calls that overlap on one device (other streams, other threads)
+   race.  Needs phx_init(device) first.  No hipMalloc, hipFree or memset. */
+#define PHX_NOALLOC_SCRATCH_BYTES (24 << 20)
+enum { PHX_NOALLOC_RESET = 1, PHX_NOALLOC_FLUSH = 2 };
+int phx_engine_turns_noalloc(const phx_engine *e, int32_t n_turns, int32_t flags,
+                             phx_stream_t stream);
 
 /* HIP graph of n_turns fused turns (captured on an internal stream);
    replay with phx_graph_launch.  Returns an opaque handle.  The graph does

@@ -41,10 +41,13 @@ PLANE_XP, PLANE_YP, PLANE_DP = 0, 1, 2
 EL_TSEG, EL_LONG, EL_LINMAP = 1, 2, 3
 (SHAPE_GENERIC, SHAPE_RF1_T, SHAPE_RF2_T, SHAPE_T_RF1, SHAPE_T_RF2, SHAPE_LIN_T, SHAPE_T_LIN,
  SHAPE_T) = range(8)
-KICKS_GENERIC, KICKS_DXDY = 0, 1
+KICKS_GENERIC, KICKS_DXDY, KICKS_PLATES = 0, 1, 2
+DETS_GENERIC, DETS_NONE, DETS_C1, DETS_C1_AMP = 0, 1, 2, 3
 CONV_GENERAL, CONV_RW_DOT = 1, 2
-PREC_FP64, PREC_FP32 = 0, 1
+PREC_FP64, PREC_FP32, PREC_FP64_DIRECT, PREC_MFMA_F32, PREC_MFMA_BF16X3 = range(5)
 STAT_NV = 33
+NOALLOC_SCRATCH_BYTES = 24 << 20
+NOALLOC_RESET, NOALLOC_FLUSH = 1, 2
 
 
 class PhxError(RuntimeError):
@@ -77,7 +80,7 @@ class Detuner(C.Structure):
                 ("gy", C.c_double), ("ay2", C.c_double), ("by", C.c_double),
                 ("dapp_x", C.c_double), ("dapp_y", C.c_double),
                 ("dapp_xy", C.c_double), ("dapp_yx", C.c_double),
-                ("p0", C.c_double)]
+                ("p0", C.c_double), ("inv_p0", C.c_double)]
 
 
 _M = ["00", "01", "10", "11", "22", "23", "32", "33"]
@@ -101,7 +104,8 @@ class LongOp(C.Structure):
                 ("circumference", C.c_double), ("phi_offset", C.c_double),
                 ("phi_lock", C.c_double), ("amplitude", C.c_double),
                 ("p_increment", C.c_double), ("p0_before", C.c_double),
-                ("p0_after", C.c_double), ("D_x", C.c_double), ("D_y", C.c_double)]
+                ("p0_after", C.c_double), ("D_x", C.c_double), ("D_y", C.c_double),
+                ("inv_circumference", C.c_double), ("inv_p0_after", C.c_double)]
 
 
 class LongSeq(C.Structure):
@@ -168,7 +172,8 @@ class Engine(C.Structure):
                 ("n_kicks", C.c_int32), ("hist_capacity", C.c_int32),
                 ("record_stats", C.c_int32), ("program_shape", C.c_int32),
                 ("kick_shape", C.c_int32), ("pass_grid", C.c_int32),
-                ("conv_kinds", C.c_int32), ("programs", C.c_void_p),
+                ("conv_kinds", C.c_int32), ("det_shape", C.c_int32),
+                ("reserved", C.c_int32), ("programs", C.c_void_p),
                 ("turn_program", C.c_void_p), ("stats", C.c_void_p),
                 ("turn_counter", C.c_void_p), ("pending", C.c_void_p),
                 ("state", C.c_void_p), ("hist", C.c_void_p), ("z_bins", C.c_void_p),
@@ -222,6 +227,7 @@ SIGNATURES = {
     "phx_engine_flush": (C.c_int, [P, P]),
     "phx_engine_stage": (C.c_int, [P, I32, P]),
     "phx_engine_snapshot": (C.c_int, [P, P]),
+    "phx_engine_turns_noalloc": (C.c_int, [P, I32, I32, P]),
     "phx_graph_capture": (C.c_int, [P, I32, C.POINTER(C.c_void_p)]),
     "phx_graph_launch": (C.c_int, [P, P]),
     "phx_graph_destroy": (C.c_int, [P]),
@@ -309,19 +315,26 @@ def device():
 
 class Workspace:
     """Zero-filled device scratch that the library leaves zero-filled again;
-    grown (re-allocated zeroed) on demand, one per device."""
+    grown (re-allocated zeroed) on demand, one per (device, stream).
+
+    Calls enqueued on one stream run one after the other, so they can share
+    one buffer; calls on different streams may run at the same time and
+    each stream has its own.  A buffer is allocated while its stream is
+    current, so when a grown buffer replaces it the caching allocator
+    reuses the old memory only in that stream's order, after the calls
+    already enqueued there."""
     _cache = {}
 
     @classmethod
     def get(cls, nbytes):
-        dev = torch.cuda.current_device()
-        buf = cls._cache.get(dev)
+        key = (torch.cuda.current_device(), torch.cuda.current_stream().cuda_stream)
+        buf = cls._cache.get(key)
         if buf is None or buf.numel() < nbytes:
             nb = max(int(nbytes), 1 << 20)
             if buf is not None:
                 nb = max(nb, 2 * buf.numel())
             buf = torch.zeros(nb, dtype=torch.uint8, device=device())
-            cls._cache[dev] = buf
+            cls._cache[key] = buf
         return buf
 
 

@@ -53,7 +53,7 @@ extern "C" int phx_init(int device) {
   PHX_CHECK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device),
                 "phx_init");
   if (cus > 0) g_num_cus = cus;
-  return PHX_OK;
+  return phx_engine_scratch_init(device);
 }
 
 extern "C" int64_t phx_workspace_bytes(int64_t n, int32_t n_slices) {

@@ -75,6 +75,29 @@ __device__ __forceinline__ double slice_floor(double z, double tail, double widt
   return floor((z - tail) / width);
 }
 
+// a / b, given r = 1.0 / b (correctly rounded): q0 = a r, the residual
+// a - b q0 (exact, one FMA) and q0 + (a - b q0) r rounded once, which is the
+// correctly rounded quotient when nothing overflows or underflows
+// (Markstein's theorem on division by a correctly rounded reciprocal) --
+// the same bits as the division, in three operations instead of the
+// division's scaled Newton sequence.  Quotients outside [2^-900, 2^900]
+// (zero, infinity, NaN, extreme magnitudes) are recomputed by the
+// division itself.
+__device__ __forceinline__ double div_rcp(double a, double b, double r) {
+  const double q0 = a * r;
+  const double e = fma(-q0, b, a);
+  const double q = fma(e, r, q0);
+  const double m = fabs(q);
+  if (__builtin_expect(!(m >= 0x1p-900 && m <= 0x1p900), 0)) return a / b;
+  return q;
+}
+
+// slice_floor with inv_width = 1.0 / width (the same value).
+__device__ __forceinline__ double slice_floor_r(double z, double tail, double width,
+                                                double inv_width) {
+  return floor(div_rcp(z - tail, width, inv_width));
+}
+
 __device__ __forceinline__ int32_t x86_cvt_i32(double f) {
   // cvttsd2si semantics: NaN and out-of-range values give INT32_MIN
   if (!(f >= -2147483648.0 && f < 2147483648.0)) return INT32_MIN;
@@ -85,6 +108,15 @@ __device__ __forceinline__ int32_t x86_cvt_i32(double f) {
 // Transverse segment map (trackers/transverse_tracking.py:138-196) with the
 // detuners (trackers/detuners.py:64-98 chromaticity, :128-151 amplitude).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void detune_amplitude(const phx_detuner &d, const P6 &p, double &qx,
+                                                 double &qy) {
+  // Jx = 1/2*((1+a^2)/b*x**2 + 2*a*x*xp + b*xp**2)
+  const double jx = 0.5 * ((d.gx * (p.x * p.x) + (d.ax2 * p.x) * p.xp) + d.bx * (p.xp * p.xp));
+  const double jy = 0.5 * ((d.gy * (p.y * p.y) + (d.ay2 * p.y) * p.yp) + d.by * (p.yp * p.yp));
+  qx = div_rcp(d.dapp_x * jx + d.dapp_xy * jy, d.p0, d.inv_p0);
+  qy = div_rcp(d.dapp_y * jy + d.dapp_yx * jx, d.p0, d.inv_p0);
+}
+
 __device__ __forceinline__ void detune(const phx_detuner &d, const P6 &p, double &qx,
                                        double &qy) {
   if (d.type == PHX_DET_CHROMA) {
@@ -98,11 +130,26 @@ __device__ __forceinline__ void detune(const phx_detuner &d, const P6 &p, double
     qx = ax * p.dp;
     qy = ay * p.dp;
   } else {
-    // Jx = 1/2*((1+a^2)/b*x**2 + 2*a*x*xp + b*xp**2)
-    double jx = 0.5 * ((d.gx * (p.x * p.x) + (d.ax2 * p.x) * p.xp) + d.bx * (p.xp * p.xp));
-    double jy = 0.5 * ((d.gy * (p.y * p.y) + (d.ay2 * p.y) * p.yp) + d.by * (p.yp * p.yp));
-    qx = (d.dapp_x * jx + d.dapp_xy * jy) / d.p0;
-    qy = (d.dapp_y * jy + d.dapp_yx * jx) / d.p0;
+    detune_amplitude(d, p, qx, qy);
+  }
+}
+
+// The detuner sets with straight-line code (PHX_DETS_*): first-order
+// chromaticity (order-1 Horner: c0 * dp), then amplitude detuning; the
+// arithmetic of detune() for those sets.
+template <int DS>
+__device__ __forceinline__ void detune_set(const phx_tseg &t, const P6 &p, double &dx,
+                                           double &dy) {
+  dx = t.dQ_x;
+  dy = t.dQ_y;
+  const phx_detuner &c = t.det[0];
+  dx = dx + c.cx[0] * p.dp;
+  dy = dy + c.cy[0] * p.dp;
+  if constexpr (DS == PHX_DETS_C1_AMP) {
+    double qx, qy;
+    detune_amplitude(t.det[1], p, qx, qy);
+    dx = dx + qx;
+    dy = dy + qy;
   }
 }
 
@@ -110,10 +157,10 @@ __device__ __forceinline__ void detune(const phx_detuner &d, const P6 &p, double
 // are read once and the N independent particles interleave (instruction-level
 // parallelism for the long FP64 chains).  The arithmetic of each particle is
 // the same for every N.
-template <int N>
+template <int N, int DS = PHX_DETS_GENERIC>
 __device__ __forceinline__ void apply_tseg_n(P6 (&p)[N], const phx_tseg &t) {
   double M00[N], M01[N], M10[N], M11[N], M22[N], M23[N], M32[N], M33[N];
-  const int nd = t.n_detuners;
+  const int nd = DS == PHX_DETS_GENERIC ? t.n_detuners : (DS == PHX_DETS_NONE ? 0 : 1);
   if (nd == 0) {
 #pragma unroll
     for (int k = 0; k < N; ++k) {
@@ -122,20 +169,25 @@ __device__ __forceinline__ void apply_tseg_n(P6 (&p)[N], const phx_tseg &t) {
     }
   } else {
     double dphi_x[N], dphi_y[N];
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-      dphi_x[k] = t.dQ_x;
-      dphi_y[k] = t.dQ_y;
-    }
-    for (int i = 0; i < nd; ++i) {
-      const phx_detuner &d = t.det[i];
+    if constexpr (DS == PHX_DETS_GENERIC) {
 #pragma unroll
       for (int k = 0; k < N; ++k) {
-        double qx, qy;
-        detune(d, p[k], qx, qy);
-        dphi_x[k] = dphi_x[k] + qx;
-        dphi_y[k] = dphi_y[k] + qy;
+        dphi_x[k] = t.dQ_x;
+        dphi_y[k] = t.dQ_y;
       }
+      for (int i = 0; i < nd; ++i) {
+        const phx_detuner &d = t.det[i];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          double qx, qy;
+          detune(d, p[k], qx, qy);
+          dphi_x[k] = dphi_x[k] + qx;
+          dphi_y[k] = dphi_y[k] + qy;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < N; ++k) detune_set<DS>(t, p[k], dphi_x[k], dphi_y[k]);
     }
     const double I00 = t.I00, I01 = t.I01, I10 = t.I10, I11 = t.I11;
     const double I22 = t.I22, I23 = t.I23, I32 = t.I32, I33 = t.I33;
@@ -204,10 +256,10 @@ __device__ __forceinline__ void rf_kick_n(P6 (&p)[N], const phx_long_op &o) {
       q.x = q.x - Dx * q.dp;
       q.y = q.y - Dy * q.dp;
     }
-    const double phi = (h * ((tp * q.z) / C) + off) + lock;
+    const double phi = (h * div_rcp(tp * q.z, C, o.inv_circumference) + off) + lock;
     double delta_p = q.dp * p0b;
     delta_p = delta_p + (amp * phx_sin(phi) - pinc);
-    q.dp = delta_p / p0a;
+    q.dp = div_rcp(delta_p, p0a, o.inv_p0_after);
     if (disp) {
       q.x = q.x + Dx * q.dp;
       q.y = q.y + Dy * q.dp;
@@ -292,28 +344,28 @@ __device__ __forceinline__ void apply_program_n(P6 (&p)[N], const phx_turn_progr
 // A program of a known element order (PHX_SHAPE_*): the elements are
 // applied in straight-line code, which the compiler schedules and allocates
 // registers for far better than the interpreter's loop over element kinds.
-template <int SHAPE, int N>
+template <int SHAPE, int N, int DS = PHX_DETS_GENERIC>
 __device__ __forceinline__ void apply_shape_n(P6 (&p)[N], const phx_turn_program &g) {
   if constexpr (SHAPE == PHX_SHAPE_RF1_T) {
     apply_rf_seq_n<1, N>(p, g.lseq[0]);
-    apply_tseg_n<N>(p, g.tseg[0]);
+    apply_tseg_n<N, DS>(p, g.tseg[0]);
   } else if constexpr (SHAPE == PHX_SHAPE_RF2_T) {
     apply_rf_seq_n<2, N>(p, g.lseq[0]);
-    apply_tseg_n<N>(p, g.tseg[0]);
+    apply_tseg_n<N, DS>(p, g.tseg[0]);
   } else if constexpr (SHAPE == PHX_SHAPE_T_RF1) {
-    apply_tseg_n<N>(p, g.tseg[0]);
+    apply_tseg_n<N, DS>(p, g.tseg[0]);
     apply_rf_seq_n<1, N>(p, g.lseq[0]);
   } else if constexpr (SHAPE == PHX_SHAPE_T_RF2) {
-    apply_tseg_n<N>(p, g.tseg[0]);
+    apply_tseg_n<N, DS>(p, g.tseg[0]);
     apply_rf_seq_n<2, N>(p, g.lseq[0]);
   } else if constexpr (SHAPE == PHX_SHAPE_LIN_T) {
     apply_linmap_n<N>(p, g.lin[0]);
-    apply_tseg_n<N>(p, g.tseg[0]);
+    apply_tseg_n<N, DS>(p, g.tseg[0]);
   } else if constexpr (SHAPE == PHX_SHAPE_T_LIN) {
-    apply_tseg_n<N>(p, g.tseg[0]);
+    apply_tseg_n<N, DS>(p, g.tseg[0]);
     apply_linmap_n<N>(p, g.lin[0]);
   } else if constexpr (SHAPE == PHX_SHAPE_T) {
-    apply_tseg_n<N>(p, g.tseg[0]);
+    apply_tseg_n<N, DS>(p, g.tseg[0]);
   } else {
     apply_program_n<N>(p, g);
   }
@@ -381,6 +433,12 @@ __device__ __forceinline__ void apply_kick_set(P6 &p, int32_t idx, int n_slices,
     if (idx < 0 || idx >= n_slices) return;
     p.xp = p.xp + kicks[idx];
     p.yp = p.yp + kicks[(size_t)n_slices + idx];
+  } else if constexpr (KS == PHX_KICKS_PLATES) {
+    if (idx < 0 || idx >= n_slices) return;
+    p.xp = p.xp + kicks[idx];
+    p.xp = p.xp + kicks[(size_t)n_slices + idx] * p.x;
+    p.yp = p.yp + kicks[(size_t)2 * n_slices + idx];
+    p.yp = p.yp + kicks[(size_t)3 * n_slices + idx] * p.y;
   } else {
     apply_wake_kicks(p, idx, n_slices, kd, n_kicks, kicks);
   }

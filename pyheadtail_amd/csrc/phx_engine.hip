@@ -105,10 +105,41 @@ __device__ __forceinline__ void store_p(const phx_engine &e, int64_t j, const P6
 // nk = min(program kicks, engine kicks): the kick table holds the latter.
 template <int KS>
 __device__ __forceinline__ void kick_pending(P6 &p, const phx_turn_program &g, int nk,
-                                             double tail, double width, int S,
-                                             const double *kicks) {
-  const double f = slice_floor(p.z, tail, width);
+                                             double tail, double width, double inv_width,
+                                             int S, const double *kicks) {
+  const double f = slice_floor_r(p.z, tail, width, inv_width);
   if (f >= 0.0 && f < (double)S) apply_kick_set<KS>(p, (int32_t)f, S, g.kick, nk, kicks);
+}
+
+// Particles k*PHX_BLOCK + t (k < PHX_PPT) of the group starting at the
+// six pointers c (uniform: scalar base plus the thread's 32-bit offset).
+// lim: the group's particle count if it is the partial last group, else
+// PHX_CHUNK (no guard needed).
+struct GroupPtr {
+  double *c[6];
+};
+
+__device__ __forceinline__ void group_load(const GroupPtr &gp, int lim, P6 (&p)[PHX_PPT]) {
+#pragma unroll
+  for (int k = 0; k < PHX_PPT; ++k) {
+    const int o = (int)threadIdx.x + k * PHX_BLOCK;
+    if (lim == PHX_CHUNK || o < lim)
+      p[k] = P6{gp.c[0][o], gp.c[1][o], gp.c[2][o], gp.c[3][o], gp.c[4][o], gp.c[5][o]};
+    else
+      p[k] = P6{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  }
+}
+
+__device__ __forceinline__ void group_store(const GroupPtr &gp, int lim,
+                                            const P6 (&p)[PHX_PPT]) {
+#pragma unroll
+  for (int k = 0; k < PHX_PPT; ++k) {
+    const int o = (int)threadIdx.x + k * PHX_BLOCK;
+    if (lim == PHX_CHUNK || o < lim) {
+      gp.c[0][o] = p[k].x; gp.c[1][o] = p[k].xp; gp.c[2][o] = p[k].y;
+      gp.c[3][o] = p[k].yp; gp.c[4][o] = p[k].z; gp.c[5][o] = p[k].dp;
+    }
+  }
 }
 
 // stats slots: [0..6) sum u, [6..12) sum (u-u0), [12..33) sum (u-u0)(v-v0)
@@ -149,7 +180,7 @@ struct PassLayout {
 // written and reloads it with vector loads that wait on the particle loads.
 // So particle 0 is computed by every lane, and the kick table is read from
 // global memory (it stays in the caches).  Dynamic LDS: the block reduction.
-template <int MODE, bool REC, bool WAKE, int SHAPE, int KS>
+template <int MODE, bool REC, bool WAKE, int SHAPE, int KS, int DS>
 __global__ void __launch_bounds__(PHX_BLOCK)
     k_engine_pass(EngineArgs a, double *__restrict__ x, double *__restrict__ xp,
                   double *__restrict__ y, double *__restrict__ yp, double *__restrict__ z,
@@ -165,18 +196,23 @@ __global__ void __launch_bounds__(PHX_BLOCK)
   const phx_turn_program &g = programs[e.turn_program[prog_turn]];
   const int pend = *e.pending;
   const double tail = e.state->tail, width = e.state->width;
+  const double inv_width = 1.0 / width;
   const int S = e.n_slices;
   const int nk = g.n_kicks < a.n_kicks ? g.n_kicks : a.n_kicks;
   const int64_t ng = phx_n_groups(n);
+  // the group's base pointers and particle count (uniform)
+  auto group = [&](int64_t gg, GroupPtr &gp) {
+    const int64_t b = gg * PHX_CHUNK;
+    gp.c[0] = x + b; gp.c[1] = xp + b; gp.c[2] = y + b;
+    gp.c[3] = yp + b; gp.c[4] = z + b; gp.c[5] = dp + b;
+    const int64_t rest = n - b;
+    return rest >= PHX_CHUNK ? PHX_CHUNK : (int)rest;
+  };
   int64_t gi = blockIdx.x;
+  GroupPtr gc, gnx;
+  int lc = group(gi, gc);
   P6 cur[PHX_PPT], nxt[PHX_PPT];
-#pragma unroll
-  for (int k = 0; k < PHX_PPT; ++k) {
-    const int64_t j = gi * PHX_CHUNK + threadIdx.x + (int64_t)k * PHX_BLOCK;
-    cur[k] = j < n ? P6{x[j], xp[j], y[j], yp[j], z[j], dp[j]}
-                   : P6{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    nxt[k] = P6{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  }
+  group_load(gc, lc, cur);
   // Particle 0 sets the shifts of the shifted sums (stats.pyx:cov): its
   // state after the pending kick (statistics), and its z after this turn's
   // elements (cuts), which needs the kick only where a kick changes dp and
@@ -191,7 +227,7 @@ __global__ void __launch_bounds__(PHX_BLOCK)
     if constexpr (KS == PHX_KICKS_GENERIC) {
       for (int k = 0; k < nk; ++k) kick_dp = kick_dp || g.kick[k].plane == PHX_PLANE_DP;
     }
-    if (pend && (REC || kick_dp)) kick_pending<KS>(q, g, nk, tail, width, S, kicks);
+    if (pend && (REC || kick_dp)) kick_pending<KS>(q, g, nk, tail, width, inv_width, S, kicks);
     src[0] = q.x; src[1] = q.xp; src[2] = q.y; src[3] = q.yp; src[4] = q.z; src[5] = q.dp;
     if (MODE == 0) apply_shape_longitudinal<SHAPE>(q, g);
     src[6] = q.z;
@@ -204,31 +240,29 @@ __global__ void __launch_bounds__(PHX_BLOCK)
   for (int i = 0; i < 6; ++i) shs[i] = src[i];
   const double zshift = src[6];
   for (; gi < ng; gi += gridDim.x) {
-    const int64_t base = gi * PHX_CHUNK + threadIdx.x;
     const int64_t gn = gi + gridDim.x;
+    int ln = 0;
     if (gn < ng) {
-#pragma unroll
-      for (int k = 0; k < PHX_PPT; ++k) {
-        const int64_t j = gn * PHX_CHUNK + threadIdx.x + (int64_t)k * PHX_BLOCK;
-        if (j < n) nxt[k] = P6{x[j], xp[j], y[j], yp[j], z[j], dp[j]};
-      }
+      ln = group(gn, gnx);
+      group_load(gnx, ln, nxt);
     }
     if (pend) {
 #pragma unroll
-      for (int k = 0; k < PHX_PPT; ++k) {
-        kick_pending<KS>(cur[k], g, nk, tail, width, S, kicks);
-      }
+      for (int k = 0; k < PHX_PPT; ++k)
+        kick_pending<KS>(cur[k], g, nk, tail, width, inv_width, S, kicks);
     }
+    const bool full = lc == PHX_CHUNK;
     if (REC) {
 #pragma unroll
       for (int k = 0; k < PHX_PPT; ++k)
-        if (base + (int64_t)k * PHX_BLOCK < n) stats_accumulate(&v[L::SOFF], cur[k], shs);
+        if (full || (int)threadIdx.x + k * PHX_BLOCK < lc)
+          stats_accumulate(&v[L::SOFF], cur[k], shs);
     }
-    if (MODE == 0) apply_shape_n<SHAPE, PHX_PPT>(cur, g);
+    if (MODE == 0) apply_shape_n<SHAPE, PHX_PPT, DS>(cur, g);
     if (WAKE) {
 #pragma unroll
       for (int k = 0; k < PHX_PPT; ++k) {
-        if (base + (int64_t)k * PHX_BLOCK < n) {
+        if (full || (int)threadIdx.x + k * PHX_BLOCK < lc) {
           double zb[ZB_NV];
 #pragma unroll
           for (int i = 0; i < ZB_NV; ++i) zb[i] = v[i];
@@ -238,18 +272,11 @@ __global__ void __launch_bounds__(PHX_BLOCK)
         }
       }
     }
-    if (MODE == 0 || pend) {
-#pragma unroll
-      for (int k = 0; k < PHX_PPT; ++k) {
-        const int64_t j = base + (int64_t)k * PHX_BLOCK;
-        if (j < n) {
-          x[j] = cur[k].x; xp[j] = cur[k].xp; y[j] = cur[k].y;
-          yp[j] = cur[k].yp; z[j] = cur[k].z; dp[j] = cur[k].dp;
-        }
-      }
-    }
+    if (MODE == 0 || pend) group_store(gc, lc, cur);
 #pragma unroll
     for (int k = 0; k < PHX_PPT; ++k) cur[k] = nxt[k];
+    gc = gnx;
+    lc = ln;
   }
   const double r = block_reduce_slots<L::NV, L::MINM, L::MAXM>(v, pass_lds);
   if (threadIdx.x < L::NV) w.partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = r;
@@ -364,6 +391,7 @@ __global__ void __launch_bounds__(EN_HIST_THREADS) k_engine_hist(EngineArgs a) {
   AccLds A = acc_lds_layout(acc_lds, S, 2);
   acc_lds_zero(A, S, 2);  // ends with a barrier: the shared cuts are visible
   const double tail = sh_tail, width = sh_width;
+  const double inv_width = 1.0 / width;
   const int ex = sh_ex, ey = sh_ey;
   load_chunk(blockIdx.x, zz, xx, yy);
   for (int64_t c = blockIdx.x; c < nc; c += gridDim.x) {
@@ -373,7 +401,7 @@ __global__ void __launch_bounds__(EN_HIST_THREADS) k_engine_hist(EngineArgs a) {
     for (int k = 0; k < PHX_PPT; ++k) {
       const int64_t j = base + (int64_t)k * EN_HIST_THREADS;
       if (j < n) {
-        const double f = slice_floor(zz[k], tail, width);
+        const double f = slice_floor_r(zz[k], tail, width, inv_width);
         if (f >= 0.0 && f < (double)S) {
           const int s = (int)f;
           atomicAdd(&A.counts[s], 1u);
@@ -502,6 +530,11 @@ int check_engine(const phx_engine *e, int *n_kicks, int *T) {
   if (e->kick_shape == PHX_KICKS_DXDY && e->n_kicks != 2)
     return phx::fail(PHX_E_INVALID, "phx_engine: kick_shape DXDY needs 2 kicks, not %d",
                      e->n_kicks);
+  if (e->kick_shape == PHX_KICKS_PLATES && e->n_kicks != 4)
+    return phx::fail(PHX_E_INVALID, "phx_engine: kick_shape PLATES needs 4 kicks, not %d",
+                     e->n_kicks);
+  if (e->det_shape < 0 || e->det_shape >= PHX_DETS_COUNT)
+    return phx::fail(PHX_E_INVALID, "phx_engine: det_shape %d", e->det_shape);
   if (e->n_slices > 0 && (!e->hist || !e->z_bins || !e->kicks || !e->counts || !e->slice_means))
     return phx::fail(PHX_E_INVALID, "phx_engine: wake buffers null");
   *n_kicks = e->n_kicks;
@@ -515,21 +548,22 @@ size_t pass_lds_bytes() {
   return BR_LDS(NV) * sizeof(double);
 }
 
-template <int MODE, bool R, bool W, int SH, int KS>
+template <int MODE, bool R, bool W, int SH, int KS, int DS>
 void launch_pass_kernel(const EngineArgs &a, hipStream_t s) {
   const phx_engine &e = a.e;
   double *const *c = e.coords;
-  k_engine_pass<MODE, R, W, SH, KS><<<dim3((unsigned)a.grid), PHX_BLOCK, pass_lds_bytes<R, W>(), s>>>(
-      a, c[0], c[1], c[2], c[3], c[4], c[5], e.programs, e.kicks);
+  k_engine_pass<MODE, R, W, SH, KS, DS>
+      <<<dim3((unsigned)a.grid), PHX_BLOCK, pass_lds_bytes<R, W>(), s>>>(
+          a, c[0], c[1], c[2], c[3], c[4], c[5], e.programs, e.kicks);
 }
 
 // Blocks of this pass that stay resident on one CU (queried once).
-template <int MODE, bool R, bool W, int SH, int KS>
+template <int MODE, bool R, bool W, int SH, int KS, int DS>
 int pass_blocks_per_cu() {
   static int cached = 0;
   if (cached == 0) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_engine_pass<MODE, R, W, SH, KS>,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_engine_pass<MODE, R, W, SH, KS, DS>,
                                                      PHX_BLOCK, pass_lds_bytes<R, W>()) !=
             hipSuccess ||
         nb < 1)
@@ -544,35 +578,54 @@ struct PassKernel {
   int (*blocks_per_cu)();
 };
 
-template <int MODE, bool R, bool W, int SH, int KS>
+template <int MODE, bool R, bool W, int SH, int KS, int DS>
 PassKernel pass_kernel() {
-  return PassKernel{launch_pass_kernel<MODE, R, W, SH, KS>, pass_blocks_per_cu<MODE, R, W, SH, KS>};
+  return PassKernel{launch_pass_kernel<MODE, R, W, SH, KS, DS>,
+                    pass_blocks_per_cu<MODE, R, W, SH, KS, DS>};
 }
 
-// The pass compiled for (program shape, kick set); a flush applies no
-// program and runs the generic one.
-template <int MODE, bool R, bool W, int SH>
-PassKernel pick_kicks(int ks) {
-  if constexpr (W && MODE == 0)
-    if (ks == PHX_KICKS_DXDY) return pass_kernel<MODE, R, W, SH, PHX_KICKS_DXDY>();
-  return pass_kernel<MODE, R, W, SH, PHX_KICKS_GENERIC>();
-}
-
-template <int MODE, bool R, bool W>
-PassKernel pick_shape(int shape, int ks) {
-  if constexpr (MODE == 0) {
-    switch (shape) {
-      case PHX_SHAPE_RF1_T: return pick_kicks<MODE, R, W, PHX_SHAPE_RF1_T>(ks);
-      case PHX_SHAPE_RF2_T: return pick_kicks<MODE, R, W, PHX_SHAPE_RF2_T>(ks);
-      case PHX_SHAPE_T_RF1: return pick_kicks<MODE, R, W, PHX_SHAPE_T_RF1>(ks);
-      case PHX_SHAPE_T_RF2: return pick_kicks<MODE, R, W, PHX_SHAPE_T_RF2>(ks);
-      case PHX_SHAPE_LIN_T: return pick_kicks<MODE, R, W, PHX_SHAPE_LIN_T>(ks);
-      case PHX_SHAPE_T_LIN: return pick_kicks<MODE, R, W, PHX_SHAPE_T_LIN>(ks);
-      case PHX_SHAPE_T: return pick_kicks<MODE, R, W, PHX_SHAPE_T>(ks);
+// The detuner set compiled in straight-line code: turns of the specialised
+// shapes without statistics records (the statistics runs take the generic
+// detuner loop, the same arithmetic).
+template <int MODE, bool R, bool W, int SH, int KS>
+PassKernel pick_dets(int ds) {
+  if constexpr (MODE == 0 && !R && SH != PHX_SHAPE_GENERIC) {
+    switch (ds) {
+      case PHX_DETS_NONE: return pass_kernel<MODE, R, W, SH, KS, PHX_DETS_NONE>();
+      case PHX_DETS_C1: return pass_kernel<MODE, R, W, SH, KS, PHX_DETS_C1>();
+      case PHX_DETS_C1_AMP: return pass_kernel<MODE, R, W, SH, KS, PHX_DETS_C1_AMP>();
       default: break;
     }
   }
-  return pick_kicks<MODE, R, W, PHX_SHAPE_GENERIC>(ks);
+  return pass_kernel<MODE, R, W, SH, KS, PHX_DETS_GENERIC>();
+}
+
+// The pass compiled for (program shape, kick set, detuner set); a flush
+// applies no program and runs the generic one.
+template <int MODE, bool R, bool W, int SH>
+PassKernel pick_kicks(int ks, int ds) {
+  if constexpr (W && MODE == 0) {
+    if (ks == PHX_KICKS_DXDY) return pick_dets<MODE, R, W, SH, PHX_KICKS_DXDY>(ds);
+    if (ks == PHX_KICKS_PLATES) return pick_dets<MODE, R, W, SH, PHX_KICKS_PLATES>(ds);
+  }
+  return pick_dets<MODE, R, W, SH, PHX_KICKS_GENERIC>(ds);
+}
+
+template <int MODE, bool R, bool W>
+PassKernel pick_shape(int shape, int ks, int ds) {
+  if constexpr (MODE == 0) {
+    switch (shape) {
+      case PHX_SHAPE_RF1_T: return pick_kicks<MODE, R, W, PHX_SHAPE_RF1_T>(ks, ds);
+      case PHX_SHAPE_RF2_T: return pick_kicks<MODE, R, W, PHX_SHAPE_RF2_T>(ks, ds);
+      case PHX_SHAPE_T_RF1: return pick_kicks<MODE, R, W, PHX_SHAPE_T_RF1>(ks, ds);
+      case PHX_SHAPE_T_RF2: return pick_kicks<MODE, R, W, PHX_SHAPE_T_RF2>(ks, ds);
+      case PHX_SHAPE_LIN_T: return pick_kicks<MODE, R, W, PHX_SHAPE_LIN_T>(ks, ds);
+      case PHX_SHAPE_T_LIN: return pick_kicks<MODE, R, W, PHX_SHAPE_T_LIN>(ks, ds);
+      case PHX_SHAPE_T: return pick_kicks<MODE, R, W, PHX_SHAPE_T>(ks, ds);
+      default: break;
+    }
+  }
+  return pick_kicks<MODE, R, W, PHX_SHAPE_GENERIC>(ks, ds);
 }
 
 // MODE 0 with a wake field: the histogram kernel folds the pass's partials;
@@ -580,10 +633,12 @@ PassKernel pick_shape(int shape, int ks) {
 template <int MODE>
 PassKernel pick_pass(const phx_engine &e, bool wake_pass) {
   const bool rec = e.record_stats != 0;
-  const int shape = e.program_shape, ks = e.kick_shape;
+  const int shape = e.program_shape, ks = e.kick_shape, ds = e.det_shape;
   if (MODE == 0 && wake_pass)
-    return rec ? pick_shape<MODE, true, true>(shape, ks) : pick_shape<MODE, false, true>(shape, ks);
-  return rec ? pick_shape<MODE, true, false>(shape, ks) : pick_shape<MODE, false, false>(shape, ks);
+    return rec ? pick_shape<MODE, true, true>(shape, ks, ds)
+               : pick_shape<MODE, false, true>(shape, ks, ds);
+  return rec ? pick_shape<MODE, true, false>(shape, ks, ds)
+             : pick_shape<MODE, false, false>(shape, ks, ds);
 }
 
 // The pass grid: the caller's, or as many blocks as stay resident.
@@ -753,6 +808,89 @@ extern "C" int phx_engine_snapshot(const phx_engine *e, phx_stream_t stream) {
   int rc = check_engine(e, &nk, &T);
   if (rc) return rc;
   return enqueue_snapshot(make_args(e, nk, T), phx::as_stream(stream));
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free engine calls (phx_engine_turns_noalloc).  Their workspace
+// is this file-scope device array: each device holds its own copy from
+// module load until the process ends.  Host code never hands the array's
+// name to a kernel (on the host it is only a stand-in symbol); its device
+// address comes from hipGetSymbolAddress, once per device at phx_init with
+// that device current, and is kept per device in g_scratch.
+// ---------------------------------------------------------------------------
+#define PHX_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[PHX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+void *g_scratch[PHX_MAX_DEVICES];
+
+// Zeroes the FP64 fallback accumulators of a workspace layout, the only
+// part of the workspace a turn reads before writing it (the merge leaves
+// them zero again; another engine's call may have left the scratch in any
+// state).
+__global__ void k_engine_clear_fallback(EngineArgs a) {
+  const phx_engine &e = a.e;
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
+  const int nf = 2 * (e.n_slices > 0 ? e.n_slices : 1);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nf; i += gridDim.x * blockDim.x)
+    w.acc.fallback[i] = 0.0;
+}
+}  // namespace
+
+int phx_engine_scratch_init(int device) {
+  if (device < 0 || device >= PHX_MAX_DEVICES)
+    return phx::fail(PHX_E_LIMIT, "phx_init: device %d beyond %d", device, PHX_MAX_DEVICES);
+  if (g_scratch[device]) return PHX_OK;
+  void *p = nullptr;
+  const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (err != hipSuccess || !p)
+    return phx::fail(err != hipSuccess ? (int)err : PHX_E_STATE,
+                     "phx_init: address of the engine scratch on device %d: %s", device,
+                     hipGetErrorString(err));
+  g_scratch[device] = p;
+  return PHX_OK;
+}
+
+extern "C" int phx_engine_turns_noalloc(const phx_engine *e, int32_t n_turns, int32_t flags,
+                                        phx_stream_t stream) {
+  if (!e) return phx::fail(PHX_E_INVALID, "phx_engine_turns_noalloc: null engine");
+  if (flags & ~(PHX_NOALLOC_RESET | PHX_NOALLOC_FLUSH))
+    return phx::fail(PHX_E_INVALID, "phx_engine_turns_noalloc: flags %d", flags);
+  if (n_turns < 0) return phx::fail(PHX_E_INVALID, "phx_engine_turns_noalloc: n_turns < 0");
+  int dev = -1;
+  PHX_CHECK_HIP(hipGetDevice(&dev), "phx_engine_turns_noalloc");
+  if (dev < 0 || dev >= PHX_MAX_DEVICES || !g_scratch[dev])
+    return phx::fail(PHX_E_NODEVICE, "phx_engine_turns_noalloc: phx_init(%d) not called", dev);
+  phx_engine c = *e;
+  c.ws = g_scratch[dev];
+  int nk, T;
+  int rc = check_engine(&c, &nk, &T);
+  if (rc) return rc;
+  rc = check_lds(&c);
+  if (rc) return rc;
+  const size_t need = engine_ws_bytes(c.n, c.n_slices, nk, T);
+  if (need > (size_t)PHX_NOALLOC_SCRATCH_BYTES)
+    return phx::fail(PHX_E_LIMIT,
+                     "phx_engine_turns_noalloc: %lld macroparticles, %d slices, %d kicks and "
+                     "%d turns of memory need %zu bytes of scratch; the limit is %d",
+                     (long long)c.n, c.n_slices, nk, T, need, PHX_NOALLOC_SCRATCH_BYTES);
+  hipStream_t s = phx::as_stream(stream);
+  EngineArgs a = make_args(&c, nk, T);
+  if (flags & PHX_NOALLOC_RESET) {
+    k_engine_reset<<<256, 256, 0, s>>>(a, need);
+    PHX_CHECK_LAUNCH("phx_engine_turns_noalloc: reset");
+  } else {
+    k_engine_clear_fallback<<<8, 256, 0, s>>>(a);
+    PHX_CHECK_LAUNCH("phx_engine_turns_noalloc: clear");
+  }
+  rc = engine_turns(&c, n_turns, s, true);
+  if (rc) return rc;
+  if (flags & PHX_NOALLOC_FLUSH) {
+    rc = enqueue_snapshot(a, s);
+    if (rc) return rc;
+    return launch_pass<1>(a, false, s);
+  }
+  return PHX_OK;
 }
 
 struct GraphHandle {

@@ -25,6 +25,10 @@ struct WsLayout {
 
 }  // namespace phx
 
+// Device address of the allocation-free engine's scratch on `device`
+// (current), looked up once per device (phx_engine.hip).
+int phx_engine_scratch_init(int device);
+
 #define PHX_CHECK_LAUNCH(name)                                                   \
   do {                                                                           \
     hipError_t _e = hipGetLastError();                                           \

@@ -456,6 +456,128 @@ __device__ __forceinline__ void rw_dispatch(int nu, bool uniform, const double *
   }
 }
 
+// ---------------------------------------------------------------------------
+// Source turns gone by: separable expansion.  For a past turn every pair has
+// tau = -dt = ts_i - tja_j > |dt_min| and tau = A + u_i - v_j with
+// A = c_s - c_t the distance of the two centres (about one revolution
+// period per turn of age) and |u_i| + |v_j| <= r A, r <= RW_EXP_R.  Then
+//   tau^(-1/2) = A^(-1/2) sum_k c_k ((u_i - v_j)/A)^k,  c_k = binom(-1/2, k),
+// and sum_i m_i tau_ij^(-1/2) = A^(-1/2) sum_q G_q (-y_j)^q with
+// y_j = v_j / A, G_q = sum_p c_(p+q) C(p+q, p) U_p, U_p = sum_i m_i (u_i/A)^p:
+// O(S) per turn instead of O(S^2).  With RW_EXP_K = 6 terms the truncation
+// is below c_6 r^6 / (1 - r) < 2.3e-19 of sum_i |m_i| tau_ij^(-1/2); the sums
+// themselves round like the pair sums (about 1e-16 relative).
+// ---------------------------------------------------------------------------
+#define RW_EXP_K 6
+#define RW_EXP_R 1e-3
+
+__device__ __forceinline__ double rw_binom_c(int k) {  // binom(-1/2, k), exact
+  constexpr double c[RW_EXP_K] = {1.0, -0.5, 0.375, -0.3125, 0.2734375, -0.24609375};
+  return c[k];
+}
+
+__device__ __forceinline__ double rw_choose(int n, int k) {
+  constexpr double t[RW_EXP_K][RW_EXP_K] = {{1, 0, 0, 0, 0, 0},   {1, 1, 0, 0, 0, 0},
+                                           {1, 2, 1, 0, 0, 0},   {1, 3, 3, 1, 0, 0},
+                                           {1, 4, 6, 4, 1, 0},   {1, 5, 10, 10, 5, 1}};
+  return t[n][k];
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-core pair loops (PHX_PREC_MFMA_F32, PHX_PREC_MFMA_BF16X3).  One
+// 4x4x1 (f32) or 4x4x4 (bf16) MFMA of 16 blocks covers 16 sources x 4
+// targets: lane l takes source s = i0 + l/4 of block l/4 and target
+// j = 4q + l%4 of its quad; the A operand is the moment of row l%4 of source
+// s (rows: the group's distinct moments, zero beyond), the B operand the wake
+// value W(s, j), so block b accumulates D[moment][target] over its sources.
+// Each wave owns 4 quads (16 targets) and runs over every source; the FP32
+// accumulators are added into FP64 every RW_MF_FLUSH steps (256 sources),
+// and the 16 blocks are summed in FP64 by lane exchanges in a fixed order.
+// bf16x3: W = W_hi + W_lo and m = m_hi + m_lo in bfloat16 (round to
+// nearest even); the 4-deep product is m_hi W_hi + m_lo W_hi + m_hi W_lo.
+// ---------------------------------------------------------------------------
+typedef float rw_f4 __attribute__((ext_vector_type(4)));
+typedef short rw_s4 __attribute__((ext_vector_type(4)));
+#define RW_MF_QUADS 4
+#define RW_MF_FLUSH 16
+
+__device__ __forceinline__ short rw_bf16_bits(float v) {
+  return __builtin_bit_cast(short, (__bf16)v);
+}
+
+__device__ __forceinline__ float rw_bf16_value(short b) {
+  return __builtin_bit_cast(float, (unsigned)(unsigned short)b << 16);
+}
+
+template <bool UNIFORM, bool BF16>
+__device__ __forceinline__ void rw_mfma(const double *ts, const double *m0, const double *m1,
+                                        const double *m2, int S, int nu,
+                                        const double (&tq)[RW_MF_QUADS], double dmin,
+                                        double (&accd)[RW_MF_QUADS][4]) {
+  const int lane = threadIdx.x & 63;
+  const int row = lane & 3, sub = lane >> 2;
+  const double *mrow = row == 0 ? m0 : (row == 1 ? m1 : m2);
+  const bool live_row = row < nu;
+  rw_f4 acc[RW_MF_QUADS];
+#pragma unroll
+  for (int q = 0; q < RW_MF_QUADS; ++q) acc[q] = rw_f4{0.f, 0.f, 0.f, 0.f};
+  int steps = 0;
+  for (int i0 = 0; i0 < S; i0 += 16) {
+    const int src = i0 + sub;
+    const bool valid = src < S;
+    const double t = valid ? ts[src] : 0.0;
+    const float a = (valid && live_row) ? (float)mrow[src] : 0.f;
+    rw_s4 av;
+    if (BF16) {
+      const short ah = rw_bf16_bits(a);
+      const short al = rw_bf16_bits(a - rw_bf16_value(ah));
+      av = rw_s4{ah, al, ah, 0};
+    }
+#pragma unroll
+    for (int q = 0; q < RW_MF_QUADS; ++q) {
+      const double dt = tq[q] - t;
+      float w;
+      if (UNIFORM) {
+        w = __builtin_amdgcn_rsqf((float)(-dt));
+      } else {
+        const double g = dt + dmin;
+        const double x = dt > -dmin ? dmin : -dt;
+        const float sel = g < 0.0 ? 1.f : (g == 0.0 ? 0.5f : (g > 0.0 ? 0.f : (float)g));
+        w = sel * __builtin_amdgcn_rsqf((float)x);
+      }
+      w = valid ? w : 0.f;
+      if (BF16) {
+        const short wh = rw_bf16_bits(w);
+        const short wl = rw_bf16_bits(w - rw_bf16_value(wh));
+        const rw_s4 bv = rw_s4{wh, wh, wl, 0};
+        acc[q] = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(av, bv, acc[q], 0, 0, 0);
+      } else {
+        acc[q] = __builtin_amdgcn_mfma_f32_4x4x1f32(a, w, acc[q], 0, 0, 0);
+      }
+    }
+    if (++steps == RW_MF_FLUSH) {
+      steps = 0;
+#pragma unroll
+      for (int q = 0; q < RW_MF_QUADS; ++q) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) accd[q][r] = accd[q][r] + (double)acc[q][r];
+        acc[q] = rw_f4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < RW_MF_QUADS; ++q)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) accd[q][r] = accd[q][r] + (double)acc[q][r];
+  // sum of the 16 blocks: lanes of one target column differ in bits 2..5
+#pragma unroll
+  for (int off = 4; off < 64; off <<= 1)
+#pragma unroll
+    for (int q = 0; q < RW_MF_QUADS; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) accd[q][r] = accd[q][r] + __shfl_xor(accd[q][r], off);
+}
+
 __device__ __forceinline__ double wave_min_nan(double v, int &bad) {
   bad |= v != v;
   for (int off = 32; off > 0; off >>= 1) {
@@ -465,7 +587,7 @@ __device__ __forceinline__ double wave_min_nan(double v, int &bad) {
   return v;
 }
 
-#define RW_STATIC_LDS (RW_WAVES * (sizeof(double) + sizeof(int)))
+#define RW_STATIC_LDS (RW_WAVES * (4 * sizeof(double) + sizeof(int)))
 // Dynamic LDS of wake_rw_block: the sources, then (reused) the wave sums of
 // one moment at a time.
 inline size_t wake_rw_lds_bytes(int S) {
@@ -479,7 +601,7 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const Wake
                                               int S, double *part, int T, double *out, int ki,
                                               int tile, int t) {
   extern __shared__ double rw_lds[];
-  __shared__ double wmin[RW_WAVES];
+  __shared__ double wmin[RW_WAVES], wmax[RW_WAVES], wtmax[RW_WAVES], wtmin[RW_WAVES];
   __shared__ int wbad[RW_WAVES];
   const phx_wake_kick &w = kk.k[ki];
   const int head = h->head, cap = h->capacity;
@@ -499,64 +621,169 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const Wake
   const double *mom = h->mom + (size_t)slot * 3 * S;
   double *ts = rw_lds, *m0 = rw_lds + S, *m1 = rw_lds + 2 * S, *m2 = rw_lds + 3 * S;
   int bad = 0;
-  double mn = tsrc[0];
+  double mn = tsrc[0], mxs = tsrc[0];
   for (int i = threadIdx.x; i < S; i += RW_THREADS) {
     const double v = tsrc[i];
     ts[i] = v;
     bad |= v != v;
     mn = v < mn ? v : mn;
+    mxs = v > mxs ? v : mxs;
     m0[i] = mom[(size_t)mo0 * S + i];
     if (nu > 1) m1[i] = mom[(size_t)mo1 * S + i];
     if (nu > 2) m2[i] = mom[(size_t)mo2 * S + i];
   }
   // dt = (t_target - age) - t_source: the age leaves the pair loop
   double tja[RW_JT];
-  double mx = -INFINITY;
+  double mx = -INFINITY, mnt = INFINITY;
 #pragma unroll
   for (int q = 0; q < RW_JT; ++q) {
     const int j = tile * RW_TILE + q * 64 + lane;
     tja[q] = h->times[(size_t)head * S + (j < S ? j : S - 1)] - age;
     bad |= tja[q] != tja[q];
     mx = tja[q] > mx ? tja[q] : mx;
+    mnt = tja[q] < mnt ? tja[q] : mnt;
   }
   mn = wave_min_nan(mn, bad);
+  mxs = -wave_min_nan(-mxs, bad);
   const double tmax = -wave_min_nan(-mx, bad);
+  const double tmin = wave_min_nan(mnt, bad);
   const int wave_bad = __ballot(bad) != 0ULL;
   if (lane == 0) {
     wmin[wave] = mn;
+    wmax[wave] = mxs;
+    wtmax[wave] = tmax;
+    wtmin[wave] = tmin;
     wbad[wave] = wave_bad;
   }
-  lds_sync();  // the sources, wave minima and NaN flags are the block's
+  lds_sync();  // the sources, wave extremes and NaN flags are the block's
   bad = 0;
+  double smax = mxs, tgmax = tmax, tgmin = tmin;
   for (int q = 0; q < RW_WAVES; ++q) {
     mn = wmin[q] < mn ? wmin[q] : mn;
+    smax = wmax[q] > smax ? wmax[q] : smax;
+    tgmax = wtmax[q] > tgmax ? wtmax[q] : tgmax;
+    tgmin = wtmin[q] < tgmin ? wtmin[q] : tgmin;
     bad |= wbad[q];
   }
-  const bool uniform = !bad && (tmax - mn) + w.dt_min_abs < 0.0;
-
-  const int chunk = (S + RW_WAVES - 1) / RW_WAVES;
-  const int ib = min(wave * chunk, S), ie = min(ib + chunk, S);
+  // the extremes of the tile's targets: every wave holds all of them
+  const bool uniform = !bad && (tgmax - mn) + w.dt_min_abs < 0.0;
   const double dmin = w.dt_min_abs;
-  double acc[3][RW_JT] = {};
-  if (w.precision == PHX_PREC_FP32)
-    rw_dispatch<true>(nu, uniform, ts, m0, m1, m2, ib, ie, tja, dmin, acc);
-  else
-    rw_dispatch<false>(nu, uniform, ts, m0, m1, m2, ib, ie, tja, dmin, acc);
-  // the sources' LDS takes the wave sums, one moment at a time; thread
-  // r < RW_TILE finishes target j0 + r
-  double *red = rw_lds;  // [wave][RW_TILE]
+  double *red = rw_lds;  // [wave][RW_TILE], reused after the sources
   double tot[3] = {0.0, 0.0, 0.0};
+  // the past-turn expansion (FP64 precision): centres, their distance A and
+  // the largest relative offset r = (max|u| + max|v|) / A
+  const double c_s = 0.5 * (mn + smax), c_t = 0.5 * (tgmin + tgmax);
+  const double A = c_s - c_t;
+  const double r_off = ((smax - c_s) + (tgmax - c_t)) / A;
+  const bool expand = w.precision == PHX_PREC_FP64 && uniform && t > 0 && A > 0.0 &&
+                      r_off <= RW_EXP_R;
+  if (expand) {
+    // U_p of each distinct moment: every thread sums its sources in order,
+    // then the lanes (fixed butterfly) and the waves in order
+    double U[3][RW_EXP_K] = {};
+    const double inv_A = 1.0 / A;
+    for (int i = threadIdx.x; i < S; i += RW_THREADS) {
+      const double x = (ts[i] - c_s) * inv_A;
+      const double mm[3] = {m0[i], nu > 1 ? m1[i] : 0.0, nu > 2 ? m2[i] : 0.0};
 #pragma unroll
-  for (int u = 0; u < 3; ++u) {
-    if (u >= nu) break;
-    lds_sync();
+      for (int u = 0; u < 3; ++u) {
+        double pw = mm[u];
 #pragma unroll
-    for (int q = 0; q < RW_JT; ++q) red[wave * RW_TILE + q * 64 + lane] = acc[u][q];
+        for (int p = 0; p < RW_EXP_K; ++p) {
+          U[u][p] = U[u][p] + pw;
+          pw = pw * x;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+#pragma unroll
+      for (int p = 0; p < RW_EXP_K; ++p)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) U[u][p] = U[u][p] + __shfl_xor(U[u][p], off);
+    lds_sync();  // every wave is done with the sources
+    if (lane == 0) {
+#pragma unroll
+      for (int u = 0; u < 3; ++u)
+#pragma unroll
+        for (int p = 0; p < RW_EXP_K; ++p) red[(wave * 3 + u) * RW_EXP_K + p] = U[u][p];
+    }
     lds_sync();
     if (threadIdx.x < RW_TILE) {
-      double r = red[threadIdx.x];
-      for (int q = 1; q < RW_WAVES; ++q) r = r + red[q * RW_TILE + threadIdx.x];
-      tot[u] = r;
+      const int jt = tile * RW_TILE + threadIdx.x;
+      const double tj = h->times[(size_t)head * S + (jt < S ? jt : S - 1)] - age;
+      const double ny = -((tj - c_t) * inv_A);
+      const double scale = 1.0 / sqrt(A);
+      for (int u = 0; u < nu; ++u) {
+        double Up[RW_EXP_K];
+#pragma unroll
+        for (int p = 0; p < RW_EXP_K; ++p) {
+          double v = red[u * RW_EXP_K + p];
+          for (int q = 1; q < RW_WAVES; ++q) v = v + red[(q * 3 + u) * RW_EXP_K + p];
+          Up[p] = v;
+        }
+        double f = 0.0;  // Horner in -y over q = K-1 .. 0
+#pragma unroll
+        for (int q = RW_EXP_K - 1; q >= 0; --q) {
+          double g = 0.0;
+#pragma unroll
+          for (int p = 0; p + q < RW_EXP_K; ++p)
+            g = g + (rw_binom_c(p + q) * rw_choose(p + q, p)) * Up[p];
+          f = f * ny + g;
+        }
+        tot[u] = scale * f;
+      }
+    }
+  } else if (w.precision == PHX_PREC_MFMA_F32 || w.precision == PHX_PREC_MFMA_BF16X3) {
+    // wave w: targets [16 w, 16 w + 16) of the tile, in quads
+    double tq[RW_MF_QUADS];
+#pragma unroll
+    for (int q = 0; q < RW_MF_QUADS; ++q) {
+      const int jt = tile * RW_TILE + wave * 4 * RW_MF_QUADS + 4 * q + (lane & 3);
+      tq[q] = h->times[(size_t)head * S + (jt < S ? jt : S - 1)] - age;
+    }
+    double accd[RW_MF_QUADS][4] = {};
+    const bool bf = w.precision == PHX_PREC_MFMA_BF16X3;
+    if (uniform) {
+      if (bf) rw_mfma<true, true>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
+      else rw_mfma<true, false>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
+    } else {
+      if (bf) rw_mfma<false, true>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
+      else rw_mfma<false, false>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
+    }
+    lds_sync();  // every wave is done with the sources
+    if (lane < 4) {
+#pragma unroll
+      for (int q = 0; q < RW_MF_QUADS; ++q)
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+          red[u * RW_TILE + wave * 4 * RW_MF_QUADS + 4 * q + lane] = accd[q][u];
+    }
+    lds_sync();
+    if (threadIdx.x < RW_TILE)
+      for (int u = 0; u < nu; ++u) tot[u] = red[u * RW_TILE + threadIdx.x];
+  } else {
+    const int chunk = (S + RW_WAVES - 1) / RW_WAVES;
+    const int ib = min(wave * chunk, S), ie = min(ib + chunk, S);
+    double acc[3][RW_JT] = {};
+    if (w.precision == PHX_PREC_FP32)
+      rw_dispatch<true>(nu, uniform, ts, m0, m1, m2, ib, ie, tja, dmin, acc);
+    else
+      rw_dispatch<false>(nu, uniform, ts, m0, m1, m2, ib, ie, tja, dmin, acc);
+    // the sources' LDS takes the wave sums, one moment at a time; thread
+    // r < RW_TILE finishes target j0 + r
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+      if (u >= nu) break;
+      lds_sync();
+#pragma unroll
+      for (int q = 0; q < RW_JT; ++q) red[wave * RW_TILE + q * 64 + lane] = acc[u][q];
+      lds_sync();
+      if (threadIdx.x < RW_TILE) {
+        double r = red[threadIdx.x];
+        for (int q = 1; q < RW_WAVES; ++q) r = r + red[q * RW_TILE + threadIdx.x];
+        tot[u] = r;
+      }
     }
   }
   const int j = tile * RW_TILE + threadIdx.x;

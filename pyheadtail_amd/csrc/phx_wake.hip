@@ -90,7 +90,7 @@ static int check_kicks(const phx_wake_kick *kicks, int32_t n_kicks, const char *
         (!k.table_t || !k.table_w || k.table_n < 2 || k.table_kind < PHX_TABLE_TRANS_UR ||
          k.table_kind > PHX_TABLE_LONG))
       return phx::fail(PHX_E_INVALID, "%s: kick %d bad wake table", who, i);
-    if (k.precision != PHX_PREC_FP64 && k.precision != PHX_PREC_FP32)
+    if (k.precision < PHX_PREC_FP64 || k.precision > PHX_PREC_MFMA_BF16X3)
       return phx::fail(PHX_E_INVALID, "%s: kick %d bad precision %d", who, i, k.precision);
   }
   return PHX_OK;

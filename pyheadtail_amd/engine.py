@@ -103,8 +103,11 @@ class FusedOneTurnMap(Element):
         self._bufs = None
         self._graphs = {}
         self._engine = None
+        # the bunch whose last wake kick this engine still holds (its device
+        # state: pending flag, cuts and kick table, is the engine's)
+        self._pending_beam = None
         self.last_stats = None
-        self._shape = (_phx.SHAPE_GENERIC, _phx.KICKS_GENERIC)
+        self._shape = (_phx.SHAPE_GENERIC, _phx.KICKS_GENERIC, _phx.DETS_GENERIC)
         self._conv_kinds = 0
         # False: always the generic pass (the same arithmetic, for checks)
         self.specialise = bool(specialise)
@@ -174,12 +177,30 @@ class FusedOneTurnMap(Element):
                  ('LIN', 'T'): _phx.SHAPE_LIN_T, ('T', 'LIN'): _phx.SHAPE_T_LIN,
                  ('T',): _phx.SHAPE_T}.get(tuple(kinds), _phx.SHAPE_GENERIC)
         kicks = _phx.KICKS_GENERIC
-        if wake is not None and prog.n_kicks == 2:
-            k0, k1 = prog.kick[0], prog.kick[1]
-            if ((k0.plane, k0.multiply, k1.plane, k1.multiply) ==
-                    (_phx.PLANE_XP, -1, _phx.PLANE_YP, -1)):
+        if wake is not None:
+            sig = tuple((prog.kick[i].plane, prog.kick[i].multiply)
+                        for i in range(prog.n_kicks))
+            if sig == ((_phx.PLANE_XP, -1), (_phx.PLANE_YP, -1)):
                 kicks = _phx.KICKS_DXDY
-        return shape, kicks
+            elif sig == ((_phx.PLANE_XP, -1), (_phx.PLANE_XP, 0), (_phx.PLANE_YP, -1),
+                         (_phx.PLANE_YP, 2)):
+                kicks = _phx.KICKS_PLATES
+        dets = _phx.DETS_GENERIC
+        if prog.n_tseg >= 1 and all(FusedOneTurnMap._det_key(prog.tseg[i]) ==
+                                    FusedOneTurnMap._det_key(prog.tseg[0])
+                                    for i in range(prog.n_tseg)):
+            dets = {(): _phx.DETS_NONE, ((_phx.DET_CHROMA, 1),): _phx.DETS_C1,
+                    ((_phx.DET_CHROMA, 1), (_phx.DET_AMPLITUDE, 0)): _phx.DETS_C1_AMP
+                    }.get(FusedOneTurnMap._det_key(prog.tseg[0]), _phx.DETS_GENERIC)
+        return shape, kicks, dets
+
+    @staticmethod
+    def _det_key(tseg):
+        """(type, order) of each detuner of a segment record (order 0 for
+        amplitude detuning)."""
+        return tuple((tseg.det[i].type,
+                      tseg.det[i].order if tseg.det[i].type == _phx.DET_CHROMA else 0)
+                     for i in range(tseg.n_detuners))
 
     @staticmethod
     def _state_key(state):
@@ -247,8 +268,11 @@ class FusedOneTurnMap(Element):
             b['prog_cap'] = max(n_programs, 2 * b['prog_cap'], 4)
             psz = C.sizeof(_phx.TurnProgram)
             b['programs'] = torch.zeros(b['prog_cap'] * psz, dtype=torch.uint8, device=dev)
-            b['uploaded'] = None
             b['table'] = torch.zeros(b['turn_cap'], dtype=torch.int32, device=dev)
+            # host mirrors of the device programs and table: a call uploads
+            # only what differs from them
+            b['host_programs'] = b''
+            b['host_table'] = np.zeros(b['turn_cap'], dtype=np.int32)
             b['stats'] = torch.zeros((b['turn_cap'] + 1, _phx.STAT_NV), dtype=torch.float64,
                                      device=dev)
         return b
@@ -262,7 +286,7 @@ class FusedOneTurnMap(Element):
         eng.n_kicks = b['n_kicks']
         eng.hist_capacity = b['T']
         eng.record_stats = 1 if self.record_stats else 0
-        eng.program_shape, eng.kick_shape = self._shape
+        eng.program_shape, eng.kick_shape, eng.det_shape = self._shape
         eng.pass_grid = self.pass_grid
         eng.conv_kinds = self._conv_kinds
         eng.programs = b['programs'].data_ptr()
@@ -300,26 +324,41 @@ class FusedOneTurnMap(Element):
     def track(self, beam):
         self.track_turns(beam, 1)
 
+    def reserve(self, beam, n_turns, n_programs=4):
+        """Size the per-turn device buffers (programs, turn table, statistics
+        rows) for calls of up to n_turns turns, so that later calls neither
+        re-allocate them nor capture their graphs again."""
+        self._ensure_buffers(beam, int(n_turns), int(n_programs))
+
     def _prepare(self, beam, n_turns):
         """Schedule n_turns, upload the programs, reset the turn counter;
         returns the engine record."""
         if beam._pending is not None and beam._pending is not self:
             beam._run_pending()
+        held = self._pending_beam
+        resetting = self._bufs is not None and self._bufs.get('needs_reset')
+        if held is not None and (held is not beam or resetting):
+            # the kick of the held bunch's last turn: applied to that bunch
+            # before its cuts and kicks are replaced by another bunch's, or
+            # cleared by a reset
+            if held._pending is self:
+                held._run_pending()
+            self._pending_beam = None
         for k in COORDS:
             beam._raw(k)  # all six coordinates must exist
         L = _phx.lib()
         progs, table = self._schedule(beam, n_turns)
         b = self._ensure_buffers(beam, n_turns, len(progs))
-        psz = C.sizeof(_phx.TurnProgram)
         blob = b"".join(progs)
-        key = (blob, table.tobytes())
-        if b.get('uploaded') != key:
-            # copies from pageable memory wait for the stream: skipped when
-            # a call repeats the last call's programs and table
+        # copies from pageable memory wait for the stream: only what differs
+        # from the device's current programs and table is uploaded
+        if b['host_programs'][:len(blob)] != blob:
             host = torch.frombuffer(bytearray(blob), dtype=torch.uint8)
-            b['programs'][:len(progs) * psz].copy_(host)
+            b['programs'][:len(blob)].copy_(host)
+            b['host_programs'] = blob + b['host_programs'][len(blob):]
+        if not np.array_equal(b['host_table'][:n_turns], table):
             b['table'][:n_turns].copy_(torch.from_numpy(table))
-            b['uploaded'] = key
+            b['host_table'][:n_turns] = table
         eng = self._engine_record(beam, b, n_turns)
         self._engine = eng
         if b.get('needs_reset'):
@@ -333,6 +372,7 @@ class FusedOneTurnMap(Element):
         beam.clean_slices()
         if self.wake is not None:
             beam._set_pending(self)
+            self._pending_beam = beam
         elif self.record_stats:
             self._finish_stats(beam)
 
@@ -371,6 +411,10 @@ class FusedOneTurnMap(Element):
         _phx.call(L.phx_engine_snapshot, C.byref(eng), s)
         ev = [[torch.cuda.Event(enable_timing=True) for _ in range(n_st + 1)]
               for _ in range(n_turns)]
+        # a spin kernel holds the stream while the host enqueues every stage
+        # and event, so that the stages then run back to back and the events
+        # bracket the kernels rather than the host's launch latency
+        torch.cuda._sleep(int(n_turns) * 400000)
         for t in range(n_turns):
             ev[t][0].record()
             for st in range(n_st):
@@ -386,6 +430,8 @@ class FusedOneTurnMap(Element):
         final statistics row)."""
         L = _phx.lib()
         _phx.call(L.phx_engine_flush, C.byref(self._engine), _phx.stream())
+        if self._pending_beam is beam:
+            self._pending_beam = None
         if self.record_stats:
             self._finish_stats(beam)
 

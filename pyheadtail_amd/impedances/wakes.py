@@ -337,13 +337,23 @@ ParallelPlatesResistiveWall = ParallelHorizontalPlatesResistiveWall
 class WakeField(Element):
     """Collective kick from the wake of the current and past slicings."""
 
-    PRECISIONS = {'fp64': _phx.PREC_FP64, 'fp32': _phx.PREC_FP32}
+    PRECISIONS = {'fp64': _phx.PREC_FP64, 'fp32': _phx.PREC_FP32,
+                  'fp64_direct': _phx.PREC_FP64_DIRECT, 'mfma_fp32': _phx.PREC_MFMA_F32,
+                  'mfma_bf16x3': _phx.PREC_MFMA_BF16X3}
 
     def __init__(self, slicer, *wake_sources, precision='fp64'):
-        """precision (not in PyHEADTAIL): 'fp32' evaluates the multi-turn
-        resistive-wall convolutions with FP32 wake values and products
-        (summed in FP32 over runs of n_slices/8 sources, in FP64 beyond);
-        every other convolution stays FP64."""
+        """precision (not in PyHEADTAIL) of the multi-turn resistive-wall
+        convolutions; every other convolution stays FP64.
+          'fp64'        FP64; past turns by a separable expansion of the
+                        wall's (-dt)^(-1/2), exact to below 1e-18 relative,
+                        the current turn pair by pair;
+          'fp64_direct' FP64, every pair (the reference's sums);
+          'fp32'        FP32 wake values and products (summed in FP32 over
+                        runs of n_slices/8 sources, in FP64 beyond);
+          'mfma_fp32'   every pair on the matrix cores, FP32 MFMA with FP32
+                        accumulation over 256 sources (FP64 beyond);
+          'mfma_bf16x3' the same with bfloat16 MFMA on split (hi + lo)
+                        wake values and moments."""
         if precision not in self.PRECISIONS:
             raise ValueError("precision must be one of %s" % sorted(self.PRECISIONS))
         self.slicer = slicer

@@ -4,7 +4,7 @@ copy of the reference).  The fixtures hold inputs and outputs only.
 
 Usage, from the repository root:
 
-    make -C oracle                       # builds the reference's Cython
+    make -C oracle reference             # builds the reference's Cython
                                          # statistics module into oracle/_ref
     python tests/golden/make_golden.py [REFERENCE_ROOT [FIXTURE ...]]
 
@@ -53,6 +53,12 @@ Fixtures (all float64 unless noted; seeds fixed; 'gamma' is the bunch's):
                  separatrix-cut Gaussian bunch of 1e5 macroparticles:
                  checksums and the first 64 particles of the initial bunch,
                  bunch statistics after every turn, fitted growth rates
+  config2.npz    1000 turns of the full plain one-turn map (RF h = 4620,
+                 V = 4.5 MV, Q' = 5, amplitude detuning, 500 slices, broadband
+                 resonator 1 MOhm/m at 1 GHz) from its separatrix-cut Gaussian
+                 bunch of 1e5 macroparticles (seed 12345): checksums of the
+                 initial bunch, bunch statistics after every turn, growth
+                 rates fitted over turns 500-999
   machines.npz   Synchrotron element order and Q_s for linear (given Q_s
                  or from the RF bucket) and non-linear longitudinal modes,
                  1 / 4 / 5 segments, RF_at 'middle' / 'end_of_transverse'
@@ -78,7 +84,7 @@ def load_reference(ref_root):
     sys.dont_write_bytecode = True
     so = glob.glob(os.path.join(ROOT, 'oracle', '_ref', 'stats*.so'))
     if not so:
-        raise SystemExit("build the statistics module first: make -C oracle")
+        raise SystemExit("build the statistics module first: make -C oracle reference")
     spec = importlib.util.spec_from_file_location('PyHEADTAIL.cobra_functions.stats', so[0])
     mod = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mod)
@@ -450,6 +456,45 @@ def gen_growth():
     return out
 
 
+# the full plain one-turn map (RF h = 4620, V = 4.5 MV; Q' = 5 and amplitude
+# detuning; 500 slices over +-3 sigma_z; CircularResonator 1 MOhm/m, 1 GHz,
+# Q = 1) at 1e5 macroparticles for 1000 turns
+CONFIG2 = dict(n=100000, turns=1000, fit_from=500, seed=12345, intensity=1.5e11)
+
+
+def gen_config2():
+    """1000 turns of the full plain one-turn map from its separatrix-cut
+    Gaussian bunch of 1e5 macroparticles: bunch statistics after every turn
+    and the growth rates fitted as test_bb_resonator.py:97-104 fits them."""
+    from PyHEADTAIL.impedances.wakes import WakeField, CircularResonator
+    from PyHEADTAIL.particles.slicing import UniformBinSlicer
+    from scipy.constants import c, m_p
+    G = CONFIG2
+    p0 = np.sqrt(GAMMA**2 - 1) * m_p * c
+    mach, _ = machine('non-linear', Qp_x=5., Qp_y=5., app_x=1e-10 * p0, app_y=1e-10 * p0,
+                      app_xy=-5e-11 * p0, h_RF=4620, V_RF=4.5e6, dphi_RF=0., p_increment=0.)
+    np.random.seed(G['seed'])
+    beam = mach.generate_6D_Gaussian_bunch(G['n'], G['intensity'], EPSN, EPSN, 0.23)
+    wake = WakeField(UniformBinSlicer(500, n_sigma_z=3), CircularResonator(1e6, 1e9, 1))
+    one_turn = list(mach.one_turn_map) + [wake]
+    init = snapshot(beam)
+    out = dict(order=np.array([type(el).__name__ for el in one_turn]),
+               gamma=np.array([beam.gamma]), initial_sum=init.sum(axis=1),
+               initial_sumsq=(init * init).sum(axis=1), initial_head=init[:, :64])
+    names = ('mean_x', 'mean_y', 'mean_z', 'sigma_z', 'epsn_x', 'epsn_y', 'epsn_z')
+    hist = []
+    for turn in range(G['turns']):
+        for el in one_turn:
+            el.track(beam)
+        hist.append([getattr(beam, k)() for k in names])
+    hist = np.array(hist)
+    out.update(stat_names=np.array(names), history=hist, final_head=snapshot(beam)[:, :64],
+               growth=np.array([growth_rate(hist[:, 0], G['fit_from']),
+                                growth_rate(hist[:, 1], G['fit_from'])]))
+    print("growth rates x, y: %.3e %.3e per turn" % tuple(out['growth']))
+    return out
+
+
 MACHINE_CASES = [(mode, n_seg, at) for mode in ('linear Q_s', 'non-linear', 'linear bucket')
                  for n_seg in (1, 4, 5) for at in ('middle', 'end_of_transverse')]
 
@@ -475,7 +520,7 @@ def gen_machines():
 
 
 GENERATORS = ('stats', 'linear', 'nonlinear', 'multiturn', 'elements', 'extra_wakes',
-              'variants', 'growth', 'machines')
+              'variants', 'growth', 'machines', 'config2')
 
 
 def main():
@@ -485,7 +530,8 @@ def main():
     cp = load_reference(ref)
     fns = dict(stats=lambda: gen_stats(cp), linear=gen_linear, nonlinear=gen_nonlinear,
                multiturn=gen_multiturn, elements=gen_elements, extra_wakes=gen_extra_wakes,
-               variants=gen_variants, growth=gen_growth, machines=gen_machines)
+               variants=gen_variants, growth=gen_growth, machines=gen_machines,
+               config2=gen_config2)
     for name in only:
         out = fns[name]()
         path = os.path.join(HERE, name + '.npz')

@@ -66,3 +66,39 @@ def coord_err(pb_np, ob):
             continue
         worst = max(worst, float(np.max(np.abs(a - b)) / rms))
     return worst
+
+
+PLATES_YOKOYA = {'DipoleX': np.pi**2 / 24., 'QuadX': -np.pi**2 / 24., 'DipoleY': np.pi**2 / 12.,
+                 'QuadY': np.pi**2 / 24.}
+
+
+def oracle_scan_map(qp, n_slices, n_turns_wake):
+    """The oracle's one-turn map of a batched-scan job (LinearMap ring,
+    chromaticity qp, broadband resonator for one turn of memory, else the
+    parallel-plates resistive wall with its kicks in the product's order)."""
+    seg = R.smooth_segment(W.Q_X, W.Q_Y, W.BETA_X, W.BETA_Y, Qp_x=qp, Qp_y=qp)
+    lin = R.LinearMapOracle([W.ALPHA_C], W.CIRCUMFERENCE, 0.017)
+    if n_turns_wake == 1:
+        kicks = [R.Kick('DipoleX', R.resonator_transverse(1., 1e6, 1e9, 1.), 1),
+                 R.Kick('DipoleY', R.resonator_transverse(1., 1e6, 1e9, 1.), 1)]
+    else:
+        kicks = [R.Kick(k, R.resistive_wall_transverse(PLATES_YOKOYA[k], 0.02, W.CIRCUMFERENCE,
+                                                      5.9e7, 1e-12), n_turns_wake)
+                 for k in ('DipoleX', 'QuadX', 'DipoleY', 'QuadY')]
+    return [lin, seg, R.WakeFieldOracle(n_slices, kicks, n_sigma_z=3)]
+
+
+def oracle_bunch_from(pb):
+    """An oracle bunch with the product bunch's coordinates and scalars."""
+    return R.Bunch(pb.macroparticlenumber, pb.particlenumber_per_mp, pb.charge, pb.mass,
+                   pb.circumference, pb.gamma, to_numpy(pb))
+
+
+def clone_bunch(pb):
+    """A product bunch with copies of pb's coordinates (device to device)."""
+    from pyheadtail_amd.particles.particles import Particles
+    q = Particles(pb.macroparticlenumber, pb.particlenumber_per_mp, pb.charge, pb.mass,
+                  pb.circumference, pb.gamma, {k: getattr(pb, k).clone() for k in COORDS})
+    # the same kinematic scalars bit for bit (not re-derived from gamma)
+    q._gamma, q._beta, q._betagamma, q._p0 = pb._gamma, pb._beta, pb._betagamma, pb._p0
+    return q

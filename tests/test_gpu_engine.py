@@ -212,8 +212,12 @@ def _shape_case_machines():
         return RFSystems(W.CIRCUMFERENCE, h, v, [0., np.pi][:nh], [W.ALPHA_C], W.GAMMA,
                          p_increment=0., charge=e, mass=m_p)
 
-    def tseg():
+    def tseg(dets='c1_amp'):
         from tests.machines import smooth_transverse
+        if dets == 'none':
+            return smooth_transverse()[0]
+        if dets == 'c1':
+            return smooth_transverse(5.)[0]
         return smooth_transverse(5., 1e-10 * W.P0, -5e-11 * W.P0)[0]
 
     def lin():
@@ -233,6 +237,9 @@ def _shape_case_machines():
         'T_LIN': lambda: [tseg(), lin(), wake(True)],
         'T': lambda: [tseg(), wake(False)],
         'RF1_T_nowake': lambda: [rf(1), tseg()],
+        'LIN_T_nodet': lambda: [lin(), tseg('none'), wake(True)],
+        'T_RF1_c1': lambda: [tseg('c1'), rf(1), wake(False)],
+        'RF1_T_c1_nowake': lambda: [rf(1), tseg('c1')],
     }
 
 
@@ -256,9 +263,16 @@ def test_specialised_pass_equals_generic(case, rec):
         shapes.append(eng._shape)
         if rec:
             stats.append(eng.last_stats['raw'])
-    assert shapes[1] == (_phx.SHAPE_GENERIC, _phx.KICKS_GENERIC)
-    expect = getattr(_phx, 'SHAPE_' + case.replace('_nowake', ''))
-    assert shapes[0][0] == expect
+    assert shapes[1] == (_phx.SHAPE_GENERIC, _phx.KICKS_GENERIC, _phx.DETS_GENERIC)
+    base = case.replace('_nowake', '').replace('_nodet', '').replace('_c1', '')
+    assert shapes[0][0] == getattr(_phx, 'SHAPE_' + base)
+    want_dets = (_phx.DETS_NONE if 'nodet' in case else
+                 _phx.DETS_C1 if '_c1' in case else _phx.DETS_C1_AMP)
+    assert shapes[0][2] == want_dets
+    if case in ('RF2_T', 'T_RF1', 'T_LIN', 'LIN_T_nodet'):
+        assert shapes[0][1] == _phx.KICKS_PLATES
+    elif 'nowake' not in case:
+        assert shapes[0][1] == _phx.KICKS_DXDY
     for k in COORDS:
         np.testing.assert_array_equal(outs[0][k], outs[1][k], err_msg=k)
     if rec:
@@ -293,6 +307,86 @@ def test_engine_resistive_wall_memory_equals_elements(mixed):
     eng.track_turns(b, 12)
     want = _phx.CONV_RW_DOT | (_phx.CONV_GENERAL if mixed else 0)
     assert eng._conv_kinds == want
+    ga, gb = to_numpy(a), to_numpy(b)
+    for k in COORDS:
+        np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)
+
+
+def test_engine_reset_between_calls_keeps_last_kick():
+    """reset() between calls: the kick of the last turn before the reset is
+    applied, then the history restarts; the same as the element path with a
+    new WakeField after the first call."""
+    _print_case("engine: reset() between calls, 5 + 4 turns, vs elements")
+    ob = W.nonlinear_bunch(50000, seed=41)
+    a, b = to_product(ob), to_product(ob)
+    m1 = nonlinear_machine(n_turns_wake=3)
+    for _ in range(5):
+        for el in m1:
+            el.track(a)
+    m2 = nonlinear_machine(n_turns_wake=3)  # a fresh wake history
+    for _ in range(4):
+        for el in m2:
+            el.track(a)
+    eng = _fused(nonlinear_machine(n_turns_wake=3), graph_turns=0, pass_grid=_standalone_grid())
+    eng.track_turns(b, 5)
+    eng.reset()
+    eng.track_turns(b, 4)
+    ga, gb = to_numpy(a), to_numpy(b)
+    for k in COORDS:
+        np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)
+
+
+def test_engine_two_bunches_alternating():
+    """Two bunches tracked in turn by one engine (one shared wake history,
+    as the reference's WakeField shares its deque): each bunch's last kick
+    goes to that bunch, bit for bit as the element path."""
+    _print_case("engine: two bunches alternating on one engine, 3 rounds of 2 turns")
+    oa, ob_ = W.nonlinear_bunch(30000, seed=42), W.nonlinear_bunch(30000, seed=43)
+    ea, eb = to_product(oa), to_product(ob_)
+    fa, fb = to_product(oa), to_product(ob_)
+    m_el = nonlinear_machine(n_turns_wake=2)
+    eng = _fused(nonlinear_machine(n_turns_wake=2), graph_turns=0, pass_grid=_standalone_grid())
+    for _ in range(3):
+        for bunch_el, bunch_f in ((ea, fa), (eb, fb)):
+            for _ in range(2):
+                for el in m_el:
+                    el.track(bunch_el)
+            eng.track_turns(bunch_f, 2)
+    for x, y in ((ea, fa), (eb, fb)):
+        gx, gy = to_numpy(x), to_numpy(y)
+        for k in COORDS:
+            np.testing.assert_array_equal(gx[k], gy[k], err_msg=k)
+
+
+@pytest.mark.parametrize("source", ["resonator", "resistive_wall"])
+def test_engine_1000_slices_equals_elements(source):
+    """Above 512 slices the merge takes 8 slices per block: at 1000 slices
+    the engine equals the element path bit for bit (counts, means and
+    coordinates), with a one-turn resonator and an 8-turn wall."""
+    _print_case("engine vs elements, 1000 slices, %s" % source)
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.impedances.wakes import (WakeField, CircularResonator,
+                                                 ParallelHorizontalPlatesResistiveWall)
+
+    def make():
+        src = (CircularResonator(1e6, 1e9, 1) if source == "resonator" else
+               ParallelHorizontalPlatesResistiveWall(0.02, W.CIRCUMFERENCE, 5.9e7, 1e-12,
+                                                     n_turns_wake=8))
+        return nonlinear_machine(wake=False) + [WakeField(UniformBinSlicer(1000, n_sigma_z=3),
+                                                          src)]
+    ob = W.nonlinear_bunch(200000, seed=44)
+    a, b = to_product(ob), to_product(ob)
+    m_el = make()
+    for _ in range(10):
+        for el in m_el:
+            el.track(a)
+    eng = _fused(make(), graph_turns=5, pass_grid=_standalone_grid())
+    eng.track_turns(b, 10)
+    ss = m_el[-1].slice_set_deque[0]
+    z_bins, counts, mx, my = eng.slice_data
+    np.testing.assert_array_equal(counts.cpu().numpy(), ss.n_macroparticles_per_slice.cpu().numpy())
+    np.testing.assert_array_equal(mx.cpu().numpy(), ss.mean_x.cpu().numpy())
+    np.testing.assert_array_equal(my.cpu().numpy(), ss.mean_y.cpu().numpy())
     ga, gb = to_numpy(a), to_numpy(b)
     for k in COORDS:
         np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)

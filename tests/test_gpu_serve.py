@@ -46,6 +46,56 @@ def test_server_results_equal_each_job_alone():
     assert p[50] > 0 and p[99] >= p[50]
 
 
+def _sc_job(seed, n=20000, n_seg=4, qp=3.):
+    import numpy as np
+    from scipy.constants import e, m_p
+    import bench
+    from pyheadtail_amd.machines.synchrotron import Synchrotron
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.impedances.wakes import WakeField, CircularResonator
+    from pyheadtail_amd.spacecharge.spacecharge import TransverseGaussianSpaceCharge
+    mach = Synchrotron(optics_mode='smooth', charge=e, mass=m_p, p0=bench.p0_of_gamma(27.7),
+                       circumference=6911.5, n_segments=n_seg, beta_x=54.6, beta_y=54.6,
+                       D_x=0., D_y=0., accQ_x=20.13, accQ_y=20.18, Qp_x=qp, Qp_y=qp,
+                       alpha_mom_compaction=1.9e-3, longitudinal_mode='linear', Q_s=0.017)
+    mach.install_after_each_transverse_segment(
+        TransverseGaussianSpaceCharge(UniformBinSlicer(64, n_sigma_z=3), 6911.5 / n_seg))
+    np.random.seed(seed)
+    b = mach.generate_6D_Gaussian_bunch(n, 1e11, bench.EPSN, bench.EPSN, bench.SIGMA_Z)
+    return list(mach.one_turn_map) + [WakeField(UniformBinSlicer(100, n_sigma_z=3),
+                                                CircularResonator(1e6, 1e9, 1))], b
+
+
+def test_server_two_element_jobs_on_two_streams():
+    """Two space-charge jobs (element by element: slicing, statistics and
+    space-charge kernels with the library's scratch workspace) on two
+    streams at once, and a statistics call on the caller's stream between
+    the server's steps: each job ends as it does alone."""
+    print("\n[case] scan server: two space-charge jobs on two streams", flush=True)
+    from pyheadtail_amd.serve import ScanServer, ElementLoop
+    turns = [17, 13]
+    alone = []
+    for seed, n in zip((81, 82), turns):
+        m, b = _sc_job(seed, n=30000 + seed)
+        ElementLoop(m).track_turns(b, n)
+        alone.append({k: getattr(b, k).cpu().numpy() for k in COORDS})
+    srv = ScanServer(n_streams=3, chunk_turns=16, graph_turns=4, element_chunk_turns=3,
+                     big_job_bytes=1 << 40)
+    jobs = [_sc_job(seed, n=30000 + seed) for seed in (81, 82)]
+    handles = [srv.submit(m, b, n) for (m, b), n in zip(jobs, turns)]
+    assert handles[0].bytes < srv.big_job_bytes
+    side = _sc_job(90, n=50000)[1]
+    while srv.busy:
+        srv.step()
+        side.sigma_x()
+        side.epsn_y()
+    assert handles[0].stream is not handles[1].stream
+    for j, h in enumerate(handles):
+        for k in COORDS:
+            np.testing.assert_array_equal(getattr(h.bunch, k).cpu().numpy(), alone[j][k],
+                                          err_msg="job %d %s" % (j, k))
+
+
 def test_server_mixes_fused_and_element_jobs():
     """A job with space-charge kicks (not fusable) runs element by element
     beside fused jobs; every job ends as it does alone."""

@@ -147,27 +147,38 @@ def test_rw_kicks_of_one_source_share_sums():
     np.testing.assert_allclose(qx, -qy, rtol=1e-15, atol=0)
 
 
-def test_fp32_resistive_wall_kick_error_per_component():
-    """FP32 wake values and products against the FP64 path, 1000 slices x 64
-    turns of a parallel-plates wall: max and RMS error per kick component,
-    relative to the component's max and RMS; bound 1e-6 (max)."""
+@pytest.mark.parametrize("prec,bound", [("fp64", 1e-13), ("fp32", 1e-6), ("mfma_fp32", 1e-6),
+                                        ("mfma_bf16x3", 1e-6)])
+def test_resistive_wall_kick_error_per_component(prec, bound):
+    """Each precision of the multi-turn wall against FP64 pair by pair (the
+    reference's sums), 1000 slices x 64 turns of a parallel-plates wall: max
+    and RMS error per kick component, relative to the component's max and
+    RMS.  FP64 takes the past turns through the separable expansion (bound
+    1e-13); the FP32 and matrix-core paths sum every pair (bound 1e-6)."""
     from pyheadtail_amd import _phx
-    print("\n[case] FP32 vs FP64 resistive wall, S=1000 T=64", flush=True)
+    from pyheadtail_amd.impedances.wakes import WakeField
+    print("\n[case] %s vs FP64 pair by pair, resistive wall, S=1000 T=64" % prec, flush=True)
     h, times, moms, ages, betas = _history(1000, 64, 64, seed=64)
-    kicks, _ = _kicks(64, False, 1e-12)
+    kicks, oracle = _kicks(64, False, 1e-12)
     b = _Bunch()
-    recs64 = [k.record(b) for k in kicks]
-    recs32 = [k.record(b) for k in kicks]
-    for r in recs32:
-        r.precision = _phx.PREC_FP32
-    k64 = h.convolve(recs64).cpu().numpy()
-    k32 = h.convolve(recs32).cpu().numpy()
+    ref_recs = [k.record(b) for k in kicks]
+    for r in ref_recs:
+        r.precision = _phx.PREC_FP64_DIRECT
+    recs = [k.record(b) for k in kicks]
+    for r in recs:
+        r.precision = WakeField.PRECISIONS[prec]
+    kref = h.convolve(ref_recs).cpu().numpy()
+    got = h.convolve(recs).cpu().numpy()
+    # the direct FP64 path itself against the formula on the host
+    fn, mom, ntw = oracle[0]
+    host = _reference(fn, mom, ntw, kicks[0]._wake_factor(b), times, moms, ages, betas)
+    assert np.max(np.abs(kref[0] - host)) <= 1e-12 * np.max(np.abs(host))
     for i, k in enumerate(kicks):
-        d = k32[i] - k64[i]
-        emax = np.max(np.abs(d)) / np.max(np.abs(k64[i]))
-        erms = np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(k64[i] ** 2))
-        print("[err] %s FP32 max %.3e rms %.3e" % (type(k).__name__, emax, erms), flush=True)
-        assert emax <= 1e-6 and erms <= 1e-6, (type(k).__name__, emax, erms)
+        d = got[i] - kref[i]
+        emax = np.max(np.abs(d)) / np.max(np.abs(kref[i]))
+        erms = np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(kref[i] ** 2))
+        print("[err] %s %s max %.3e rms %.3e" % (type(k).__name__, prec, emax, erms), flush=True)
+        assert emax <= bound and erms <= bound, (type(k).__name__, emax, erms)
 
 
 def test_fp32_resistive_wall_tracking():
