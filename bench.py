@@ -27,9 +27,11 @@ Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 50]
 1e6 macroparticles, LinearMap (alpha_c = 1.9e-3, Q_s = 0.017), one smooth
 transverse segment, 1000 slices over +-3 sigma_z and a parallel-plates
 resistive wall (b = 0.02 m, L = 6911.5 m, sigma_c = 5.9e7 S/m, dt_min =
-1e-12 s, dipolar and quadrupolar x/y kicks) with 64 turns of memory, in FP64
-and with the FP32 convolution, and the FP32 kicks' error against FP64 per
-kick component on the bunch's own 64-turn history.
+1e-12 s, dipolar and quadrupolar x/y kicks) with 64 turns of memory, in
+every precision of the wall convolution (FP64 with the past-turn
+expansion, FP64 pair by pair, FP32, FP32 MFMA, split-bfloat16 MFMA), and
+each precision's kick error against FP64 pair by pair per kick component
+on the bunch's own 64-turn history.
 """
 import argparse
 import json
@@ -219,8 +221,14 @@ def build_machine(p0_ref):
     return mach
 
 
+RW64_PRECISIONS = ('fp64', 'fp64_direct', 'fp32', 'mfma_fp32', 'mfma_bf16x3', 'mfma_bf16x6')
+
+
 def rw64(args):
-    """The multi-turn resistive-wall workload in FP64 and FP32 (one JSON line)."""
+    """The multi-turn resistive-wall workload in every precision of the wall
+    convolution (one JSON line): turn rate, stage times, and each
+    precision's kick error per component against FP64 pair by pair on the
+    bunch's own 64-turn history."""
     import numpy as np
     import torch
     torch.cuda.set_device(0)
@@ -237,7 +245,7 @@ def rw64(args):
                        D_x=0., D_y=0., accQ_x=20.13, accQ_y=20.18,
                        alpha_mom_compaction=1.9e-3, longitudinal_mode='linear', Q_s=0.017)
     res, engines = {}, {}
-    for prec in ('fp64', 'fp32'):
+    for prec in RW64_PRECISIONS:
         wake = WakeField(UniformBinSlicer(S, n_sigma_z=3),
                          ParallelHorizontalPlatesResistiveWall(0.02, 6911.5, 5.9e7, 1e-12,
                                                                n_turns_wake=T),
@@ -245,7 +253,9 @@ def rw64(args):
         np.random.seed(SEED)
         bunch = mach.generate_6D_Gaussian_bunch(n, INTENSITY, EPSN, EPSN, SIGMA_Z)
         eng = FusedOneTurnMap(list(mach.one_turn_map) + [wake], graph_turns=args.graph_turns)
-        eng.track_turns(bunch, T + max(args.warmup, args.graph_turns))  # history filled
+        warm = T + max(args.warmup, args.graph_turns)
+        eng.reserve(bunch, max(warm, steps, args.stage_turns))
+        eng.track_turns(bunch, warm)  # history filled
         bunch.x
         torch.cuda.synchronize()
         stage_ms = eng.timed_turns(bunch, min(args.stage_turns, 50))
@@ -259,22 +269,34 @@ def rw64(args):
         res[prec] = {"value": n * steps / dt, "ms_per_step": dt / steps * 1e3,
                      "stage_ms": {"pass": stage_ms[0], "hist": stage_ms[1],
                                   "conv": stage_ms[2]}}
-        engines[prec] = (eng, wake, bunch)
-    # FP32 against FP64 on the FP64 run's own 64-turn history
+        print("[rw64] %s: %.1f us/turn, convolution stage %.1f us"
+              % (prec, dt / steps * 1e6, stage_ms[2] * 1e3), file=sys.stderr, flush=True)
+        if prec == 'fp64':
+            engines[prec] = (eng, wake, bunch)
+        else:
+            del eng, wake, bunch
+    # every precision against FP64 pair by pair on the FP64 run's own history
     eng, wake, bunch = engines['fp64']
     ring = eng._bufs['history']
     ring.n_pushed = T
     recs = [k.record(bunch) for k in wake.wake_kicks]
-    k64 = ring.convolve(recs).cpu().numpy()
     for r in recs:
-        r.precision = _phx.PREC_FP32
-    k32 = ring.convolve(recs).cpu().numpy()
-    err = {}
-    for i, k in enumerate(wake.wake_kicks):
-        d = k32[i] - k64[i]
-        err[type(k).__name__] = {
-            "max_rel": float(np.max(np.abs(d)) / np.max(np.abs(k64[i]))),
-            "rms_rel": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(k64[i] ** 2)))}
+        r.precision = _phx.PREC_FP64_DIRECT
+    kref = ring.convolve(recs).cpu().numpy()
+    errs = {}
+    for prec in RW64_PRECISIONS:
+        if prec == 'fp64_direct':
+            continue
+        for r in recs:
+            r.precision = WakeField.PRECISIONS[prec]
+        kp = ring.convolve(recs).cpu().numpy()
+        errs[prec] = {}
+        for i, k in enumerate(wake.wake_kicks):
+            d = kp[i] - kref[i]
+            errs[prec][type(k).__name__] = {
+                "max_rel": float(np.max(np.abs(d)) / np.max(np.abs(kref[i]))),
+                "rms_rel": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(kref[i] ** 2)))}
+    flop = 2.0 * S * S * T * 3   # pairs x the three distinct moments, per turn
     out = {
         "metric": "macroparticle-turns/s, multi-turn resistive-wall wake",
         "value": res['fp64']['value'], "unit": "macroparticle-turns/s", "n_gpus": 1,
@@ -285,7 +307,11 @@ def rw64(args):
                                "slices +-3 sigma_z, parallel-plates resistive wall (4 kicks), "
                                "%d turns of memory" % (n, S, T),
                    "macroparticles": n, "n_slices": S, "n_turns_wake": T},
-        "fp64": res['fp64'], "fp32": res['fp32'], "fp32_kick_error_vs_fp64": err,
+        "precisions": res,
+        "contraction_flop_per_turn": flop,
+        "contraction_tflops": {p: flop / (res[p]['stage_ms']['conv'] * 1e-3) / 1e12
+                               for p in RW64_PRECISIONS},
+        "kick_error_vs_fp64_direct": errs,
     }
     print(json.dumps(out))
 

@@ -340,11 +340,13 @@ enum { PHX_PLANE_XP = 0, PHX_PLANE_YP = 1, PHX_PLANE_DP = 2 };
      MFMA_F32     every pair on the matrix cores: 4x4x1 FP32 MFMA, FP32 wake
                   values and moments, FP32 accumulation over 256 sources,
                   FP64 beyond;
-     MFMA_BF16X3  the same with 4x4x4 bfloat16 MFMA on split operands
-                  (hi + lo for wake values and moments, three products).
+     MFMA_BF16X3  the same with 4x4x4 bfloat16 MFMA on split operands: two
+                  bfloat16 parts of each wake value and moment, three
+                  products (about 16 significant bits);
+     MFMA_BF16X6  three parts each, six products (two MFMAs per step).
    Every other convolution is FP64 whatever the field says. */
 enum { PHX_PREC_FP64 = 0, PHX_PREC_FP32 = 1, PHX_PREC_FP64_DIRECT = 2, PHX_PREC_MFMA_F32 = 3,
-       PHX_PREC_MFMA_BF16X3 = 4 };
+       PHX_PREC_MFMA_BF16X3 = 4, PHX_PREC_MFMA_BF16X6 = 5 };
 
 typedef struct {
     int32_t fn;               /* PHX_WAKE_* */
@@ -534,8 +536,9 @@ enum { PHX_KICKS_GENERIC = 0, PHX_KICKS_DXDY = 1, PHX_KICKS_PLATES = 2, PHX_KICK
 enum { PHX_DETS_GENERIC = 0, PHX_DETS_NONE = 1, PHX_DETS_C1 = 2, PHX_DETS_C1_AMP = 3,
        PHX_DETS_COUNT = 4 };
 /* Wake convolution kernels: resistive-wall kicks in the dot-product form
-   (multi-turn memory) have their own; every other kick is GENERAL. */
-enum { PHX_CONV_GENERAL = 1, PHX_CONV_RW_DOT = 2 };
+   (multi-turn memory) have their own, RW_DOT for the VALU precisions and
+   RW_MFMA for the matrix-core ones; every other kick is GENERAL. */
+enum { PHX_CONV_GENERAL = 1, PHX_CONV_RW_DOT = 2, PHX_CONV_RW_MFMA = 4 };
 
 /* Everything the fused engine needs; all pointers are device memory owned
    by the caller.  The engine handles one-turn maps of the form

@@ -35,7 +35,9 @@ TABLE_TRANS_UR, TABLE_TRANS_LOWBETA, TABLE_LONG_BL, TABLE_LONG = 0, 1, 2, 3
 
 def conv_kind(kick):
     """The convolution kernel a phx_wake_kick record needs (CONV_*)."""
-    return CONV_RW_DOT if kick.dot_product and kick.fn == WAKE_RESISTIVE_WALL else CONV_GENERAL
+    if kick.dot_product and kick.fn == WAKE_RESISTIVE_WALL:
+        return CONV_RW_MFMA if kick.precision in MFMA_PRECISIONS else CONV_RW_DOT
+    return CONV_GENERAL
 MOM_N, MOM_X, MOM_Y = 0, 1, 2
 PLANE_XP, PLANE_YP, PLANE_DP = 0, 1, 2
 EL_TSEG, EL_LONG, EL_LINMAP = 1, 2, 3
@@ -43,8 +45,10 @@ EL_TSEG, EL_LONG, EL_LINMAP = 1, 2, 3
  SHAPE_T) = range(8)
 KICKS_GENERIC, KICKS_DXDY, KICKS_PLATES = 0, 1, 2
 DETS_GENERIC, DETS_NONE, DETS_C1, DETS_C1_AMP = 0, 1, 2, 3
-CONV_GENERAL, CONV_RW_DOT = 1, 2
-PREC_FP64, PREC_FP32, PREC_FP64_DIRECT, PREC_MFMA_F32, PREC_MFMA_BF16X3 = range(5)
+CONV_GENERAL, CONV_RW_DOT, CONV_RW_MFMA = 1, 2, 4
+(PREC_FP64, PREC_FP32, PREC_FP64_DIRECT, PREC_MFMA_F32, PREC_MFMA_BF16X3,
+ PREC_MFMA_BF16X6) = range(6)
+MFMA_PRECISIONS = (PREC_MFMA_F32, PREC_MFMA_BF16X3, PREC_MFMA_BF16X6)
 STAT_NV = 33
 NOALLOC_SCRATCH_BYTES = 24 << 20
 NOALLOC_RESET, NOALLOC_FLUSH = 1, 2

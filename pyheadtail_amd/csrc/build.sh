@@ -9,13 +9,17 @@ ARCH="${PHX_ARCH:-gfx950}"
 FLAGS=(-O3 -std=c++17 -fPIC --offload-arch="$ARCH" -ffp-contract=off
        -Xclang -target-feature -Xclang -packed-fp32-ops
        -Wall -Wno-unused-function -Wno-unused-variable)
+# PHX_DEFINES: extra -D options (alternative compile-time choices);
+# PHX_BUILD_DIR: where the objects go
+read -r -a defs <<< "${PHX_DEFINES:-}"
+bdir="${PHX_BUILD_DIR:-$here/_build}"
 objs=()
 pids=()
-mkdir -p "$here/_build"
+mkdir -p "$bdir"
 for src in phx_api phx_stats phx_track phx_wake phx_engine phx_spacecharge phx_sort; do
-  "$HIPCC" "${FLAGS[@]}" -c "$here/$src.hip" -o "$here/_build/$src.o" &
+  "$HIPCC" "${FLAGS[@]}" "${defs[@]}" -c "$here/$src.hip" -o "$bdir/$src.o" &
   pids+=($!)
-  objs+=("$here/_build/$src.o")
+  objs+=("$bdir/$src.o")
 done
 for p in "${pids[@]}"; do wait "$p"; done
 "$HIPCC" -shared --offload-arch="$ARCH" -o "$out/libphx.so.tmp" "${objs[@]}"

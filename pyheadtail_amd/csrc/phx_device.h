@@ -12,7 +12,9 @@
 
 #define PHX_BLOCK 256
 #define PHX_WAVE 64
+#ifndef PHX_PPT
 #define PHX_PPT 2
+#endif
 #define PHX_CHUNK (PHX_BLOCK * PHX_PPT)
 // Reducing particle passes run a persistent grid of G blocks: block b
 // handles the particle groups b, b + G, b + 2G, ... of PHX_CHUNK particles

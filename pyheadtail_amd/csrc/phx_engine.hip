@@ -180,8 +180,14 @@ struct PassLayout {
 // written and reloads it with vector loads that wait on the particle loads.
 // So particle 0 is computed by every lane, and the kick table is read from
 // global memory (it stays in the caches).  Dynamic LDS: the block reduction.
+#ifndef PHX_PASS_MIN_BLOCKS
+#define PHX_PASS_MIN_BLOCKS 1
+#endif
+#ifndef PHX_PASS_LAUNDER
+#define PHX_PASS_LAUNDER 0
+#endif
 template <int MODE, bool REC, bool WAKE, int SHAPE, int KS, int DS>
-__global__ void __launch_bounds__(PHX_BLOCK)
+__global__ void __launch_bounds__(PHX_BLOCK, PHX_PASS_MIN_BLOCKS)
     k_engine_pass(EngineArgs a, double *__restrict__ x, double *__restrict__ xp,
                   double *__restrict__ y, double *__restrict__ yp, double *__restrict__ z,
                   double *__restrict__ dp, const phx_turn_program *__restrict__ programs,
@@ -258,7 +264,17 @@ __global__ void __launch_bounds__(PHX_BLOCK)
         if (full || (int)threadIdx.x + k * PHX_BLOCK < lc)
           stats_accumulate(&v[L::SOFF], cur[k], shs);
     }
-    if (MODE == 0) apply_shape_n<SHAPE, PHX_PPT, DS>(cur, g);
+    if (MODE == 0) {
+      // PHX_PASS_LAUNDER: read the element parameters from the program in
+      // every group (a pointer the compiler cannot follow across
+      // iterations) instead of holding them in scalar registers over the
+      // loop; measured slower (more vector registers, 2 waves per SIMD)
+      const phx_turn_program *gl = &g;
+#if PHX_PASS_LAUNDER
+      asm volatile("" : "+s"(gl));
+#endif
+      apply_shape_n<SHAPE, PHX_PPT, DS>(cur, *gl);
+    }
     if (WAKE) {
 #pragma unroll
       for (int k = 0; k < PHX_PPT; ++k) {
@@ -308,6 +324,39 @@ __global__ void k_engine_snapshot(EngineArgs a) {
   const phx_engine &e = a.e;
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
   if (threadIdx.x < 6) w.p0[threadIdx.x] = e.coords[threadIdx.x][0];
+}
+
+// The pending wake kick alone (a flush without statistics): one pass over
+// the coordinates the kick set reads, writing the momenta it changes
+// (16 B of z and 32 B of xp, yp per macroparticle for dipolar kicks).
+template <int KS>
+__global__ void __launch_bounds__(PHX_BLOCK) k_engine_kick(EngineArgs a) {
+  const phx_engine &e = a.e;
+  if (!*e.pending) return;
+  const int turn = *e.turn_counter;
+  const phx_turn_program &g = e.programs[e.turn_program[turn > 0 ? turn - 1 : 0]];
+  const int nk = g.n_kicks < a.n_kicks ? g.n_kicks : a.n_kicks;
+  const double tail = e.state->tail, width = e.state->width;
+  const double inv_width = 1.0 / width;
+  const int S = e.n_slices;
+  double *const *c = e.coords;
+  for (int64_t j = blockIdx.x * (int64_t)PHX_BLOCK + threadIdx.x; j < e.n;
+       j += (int64_t)gridDim.x * PHX_BLOCK) {
+    P6 p;
+    p.z = c[4][j];
+    p.xp = c[1][j];
+    p.yp = c[3][j];
+    p.x = KS == PHX_KICKS_DXDY ? 0.0 : c[0][j];
+    p.y = KS == PHX_KICKS_DXDY ? 0.0 : c[2][j];
+    p.dp = KS == PHX_KICKS_GENERIC ? c[5][j] : 0.0;
+    const double f = slice_floor_r(p.z, tail, width, inv_width);
+    if (f >= 0.0 && f < (double)S) {
+      apply_kick_set<KS>(p, (int32_t)f, S, g.kick, nk, e.kicks);
+      c[1][j] = p.xp;
+      c[3][j] = p.yp;
+      if (KS == PHX_KICKS_GENERIC) c[5][j] = p.dp;
+    }
+  }
 }
 
 // Slice histogram of the new state.  Prologue: every block folds the z
@@ -470,13 +519,14 @@ __global__ void __launch_bounds__(WK_THREADS) k_engine_conv(EngineArgs a) {
   wake_conv_block(e.hist, kk, e.n_slices, w.conv_part, a.T, e.kicks, a.gmax);
 }
 
+template <int KM>
 __global__ void __launch_bounds__(RW_THREADS) k_engine_conv_rw(EngineArgs a) {
   const phx_engine &e = a.e;
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
   const phx_turn_program &g = e.programs[e.turn_program[e.state->turn]];
   const WakeKicks &kk = *reinterpret_cast<const WakeKicks *>(g.kick);
   if (blockIdx.x == 0 && threadIdx.x == 0) *e.pending = 1;
-  wake_rw_block(e.hist, kk, a.n_kicks, e.n_slices, w.conv_part, a.T, e.kicks);
+  wake_rw_block<KM>(e.hist, kk, a.n_kicks, e.n_slices, w.conv_part, a.T, e.kicks);
 }
 
 __global__ void k_engine_conv_reduce(EngineArgs a) {
@@ -525,7 +575,7 @@ int check_engine(const phx_engine *e, int *n_kicks, int *T) {
   if (e->pass_grid < 0 || e->pass_grid > PHX_MAX_GRID)
     return phx::fail(PHX_E_INVALID, "phx_engine: pass_grid %d not in 0..%d", e->pass_grid,
                      PHX_MAX_GRID);
-  if (e->conv_kinds < 0 || e->conv_kinds > (PHX_CONV_GENERAL | PHX_CONV_RW_DOT))
+  if (e->conv_kinds < 0 || e->conv_kinds > (PHX_CONV_GENERAL | PHX_CONV_RW_DOT | PHX_CONV_RW_MFMA))
     return phx::fail(PHX_E_INVALID, "phx_engine: conv_kinds %d", e->conv_kinds);
   if (e->kick_shape == PHX_KICKS_DXDY && e->n_kicks != 2)
     return phx::fail(PHX_E_INVALID, "phx_engine: kick_shape DXDY needs 2 kicks, not %d",
@@ -698,9 +748,14 @@ int enqueue_stage(const EngineArgs &a, int stage, bool has_wake, hipStream_t s) 
     PHX_CHECK_LAUNCH("phx_engine: conv");
   }
   if (e.conv_kinds == 0 || (e.conv_kinds & PHX_CONV_RW_DOT)) {
-    k_engine_conv_rw<<<dim3((e.n_slices + RW_TILE - 1) / RW_TILE * a.T), RW_THREADS,
-                       wake_rw_lds_bytes(e.n_slices), s>>>(a);
+    k_engine_conv_rw<0><<<dim3((e.n_slices + RW_TILE - 1) / RW_TILE * a.T), RW_THREADS,
+                          wake_rw_lds_bytes(e.n_slices), s>>>(a);
     PHX_CHECK_LAUNCH("phx_engine: conv resistive wall");
+  }
+  if (e.conv_kinds == 0 || (e.conv_kinds & PHX_CONV_RW_MFMA)) {
+    k_engine_conv_rw<1><<<dim3((e.n_slices + RW_TILE - 1) / RW_TILE * a.T), RW_THREADS,
+                          wake_rw_lds_bytes(e.n_slices), s>>>(a);
+    PHX_CHECK_LAUNCH("phx_engine: conv resistive wall (matrix cores)");
   }
   if (a.T > 1) {
     const int tot = a.n_kicks * e.n_slices;
@@ -716,6 +771,30 @@ int enqueue_snapshot(const EngineArgs &a, hipStream_t s) {
   return PHX_OK;
 }
 
+// The pending kick of the last turn: without statistics, the kick alone
+// and the one-block bookkeeping; with them, a snapshot and the flush pass
+// (kick, then the final statistics row).
+int enqueue_flush(const EngineArgs &a, hipStream_t s) {
+  const phx_engine &e = a.e;
+  if (e.record_stats || e.n_slices == 0) {
+    int rc = enqueue_snapshot(a, s);
+    if (rc) return rc;
+    return launch_pass<1>(a, false, s);
+  }
+  const int64_t ng = (e.n + PHX_BLOCK - 1) / PHX_BLOCK;
+  const unsigned grid = (unsigned)(ng < PHX_MAX_GRID * 4 ? ng : PHX_MAX_GRID * 4);
+  if (e.kick_shape == PHX_KICKS_DXDY)
+    k_engine_kick<PHX_KICKS_DXDY><<<grid, PHX_BLOCK, 0, s>>>(a);
+  else if (e.kick_shape == PHX_KICKS_PLATES)
+    k_engine_kick<PHX_KICKS_PLATES><<<grid, PHX_BLOCK, 0, s>>>(a);
+  else
+    k_engine_kick<PHX_KICKS_GENERIC><<<grid, PHX_BLOCK, 0, s>>>(a);
+  PHX_CHECK_LAUNCH("phx_engine: flush kick");
+  k_engine_fin<1, false><<<1, PHX_BLOCK, 0, s>>>(a);
+  PHX_CHECK_LAUNCH("phx_engine: flush finalize");
+  return PHX_OK;
+}
+
 int enqueue_turn(const EngineArgs &a, bool has_wake, hipStream_t s) {
   int rc = enqueue_stage(a, 0, has_wake, s);
   if (rc || !has_wake) return rc;
@@ -728,7 +807,7 @@ int check_lds(const phx_engine *e) {
   if (e->n_slices > 0 && (acc_lds_bytes(e->n_slices, 2) > phx_max_lds_bytes() ||
                           wake_conv_lds_bytes(e->n_slices, 1) + WK_STATIC_LDS > phx_max_lds_bytes()))
     return phx::fail(PHX_E_LIMIT, "phx_engine: n_slices=%d exceeds LDS", e->n_slices);
-  const bool rw = e->conv_kinds == 0 || (e->conv_kinds & PHX_CONV_RW_DOT);
+  const bool rw = e->conv_kinds == 0 || (e->conv_kinds & (PHX_CONV_RW_DOT | PHX_CONV_RW_MFMA));
   if (e->n_slices > 0 && rw && wake_rw_lds_bytes(e->n_slices) + RW_STATIC_LDS > phx_max_lds_bytes())
     return phx::fail(PHX_E_LIMIT,
                      "phx_engine: n_slices=%d exceeds LDS of the resistive-wall convolution",
@@ -797,10 +876,7 @@ extern "C" int phx_engine_flush(const phx_engine *e, phx_stream_t stream) {
   int nk, T;
   int rc = check_engine(e, &nk, &T);
   if (rc) return rc;
-  EngineArgs a = make_args(e, nk, T);
-  rc = enqueue_snapshot(a, phx::as_stream(stream));
-  if (rc) return rc;
-  return launch_pass<1>(a, false, phx::as_stream(stream));
+  return enqueue_flush(make_args(e, nk, T), phx::as_stream(stream));
 }
 
 extern "C" int phx_engine_snapshot(const phx_engine *e, phx_stream_t stream) {
@@ -885,11 +961,7 @@ extern "C" int phx_engine_turns_noalloc(const phx_engine *e, int32_t n_turns, in
   }
   rc = engine_turns(&c, n_turns, s, true);
   if (rc) return rc;
-  if (flags & PHX_NOALLOC_FLUSH) {
-    rc = enqueue_snapshot(a, s);
-    if (rc) return rc;
-    return launch_pass<1>(a, false, s);
-  }
+  if (flags & PHX_NOALLOC_FLUSH) return enqueue_flush(a, s);
   return PHX_OK;
 }
 

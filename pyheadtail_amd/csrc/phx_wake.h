@@ -493,8 +493,11 @@ __device__ __forceinline__ double rw_choose(int n, int k) {
 // Each wave owns 4 quads (16 targets) and runs over every source; the FP32
 // accumulators are added into FP64 every RW_MF_FLUSH steps (256 sources),
 // and the 16 blocks are summed in FP64 by lane exchanges in a fixed order.
-// bf16x3: W = W_hi + W_lo and m = m_hi + m_lo in bfloat16 (round to
-// nearest even); the 4-deep product is m_hi W_hi + m_lo W_hi + m_hi W_lo.
+// bf16x3: W = W1 + W2 and m = m1 + m2 in bfloat16 (round to nearest even,
+// each part the rounded remainder of the previous ones); one 4-deep product
+// m1 W1 + m2 W1 + m1 W2 (about 16 significant bits).  bf16x6: three parts
+// each and two 4-deep products, m1 W1 + m1 W2 + m2 W1 + m1 W3 and
+// m2 W2 + m3 W1 (every term down to 2^-16 of the leading one).
 // ---------------------------------------------------------------------------
 typedef float rw_f4 __attribute__((ext_vector_type(4)));
 typedef short rw_s4 __attribute__((ext_vector_type(4)));
@@ -509,7 +512,8 @@ __device__ __forceinline__ float rw_bf16_value(short b) {
   return __builtin_bit_cast(float, (unsigned)(unsigned short)b << 16);
 }
 
-template <bool UNIFORM, bool BF16>
+// MF: 0 FP32, 1 bf16x3, 2 bf16x6
+template <bool UNIFORM, int MF>
 __device__ __forceinline__ void rw_mfma(const double *ts, const double *m0, const double *m1,
                                         const double *m2, int S, int nu,
                                         const double (&tq)[RW_MF_QUADS], double dmin,
@@ -527,11 +531,18 @@ __device__ __forceinline__ void rw_mfma(const double *ts, const double *m0, cons
     const bool valid = src < S;
     const double t = valid ? ts[src] : 0.0;
     const float a = (valid && live_row) ? (float)mrow[src] : 0.f;
-    rw_s4 av;
-    if (BF16) {
-      const short ah = rw_bf16_bits(a);
-      const short al = rw_bf16_bits(a - rw_bf16_value(ah));
-      av = rw_s4{ah, al, ah, 0};
+    rw_s4 av, av2;
+    if (MF > 0) {
+      const short a1 = rw_bf16_bits(a);
+      const float ar = a - rw_bf16_value(a1);
+      const short a2 = rw_bf16_bits(ar);
+      if (MF == 1) {
+        av = rw_s4{a1, a2, a1, 0};
+      } else {
+        const short a3 = rw_bf16_bits(ar - rw_bf16_value(a2));
+        av = rw_s4{a1, a1, a2, a1};
+        av2 = rw_s4{a2, a3, 0, 0};
+      }
     }
 #pragma unroll
     for (int q = 0; q < RW_MF_QUADS; ++q) {
@@ -546,11 +557,20 @@ __device__ __forceinline__ void rw_mfma(const double *ts, const double *m0, cons
         w = sel * __builtin_amdgcn_rsqf((float)x);
       }
       w = valid ? w : 0.f;
-      if (BF16) {
-        const short wh = rw_bf16_bits(w);
-        const short wl = rw_bf16_bits(w - rw_bf16_value(wh));
-        const rw_s4 bv = rw_s4{wh, wh, wl, 0};
-        acc[q] = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(av, bv, acc[q], 0, 0, 0);
+      if (MF > 0) {
+        const short w1 = rw_bf16_bits(w);
+        const float wr = w - rw_bf16_value(w1);
+        const short w2 = rw_bf16_bits(wr);
+        if (MF == 1) {
+          const rw_s4 bv = rw_s4{w1, w1, w2, 0};
+          acc[q] = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(av, bv, acc[q], 0, 0, 0);
+        } else {
+          const short w3 = rw_bf16_bits(wr - rw_bf16_value(w2));
+          const rw_s4 bv = rw_s4{w1, w2, w1, w3};
+          const rw_s4 bv2 = rw_s4{w2, w1, 0, 0};
+          acc[q] = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(av, bv, acc[q], 0, 0, 0);
+          acc[q] = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(av2, bv2, acc[q], 0, 0, 0);
+        }
       } else {
         acc[q] = __builtin_amdgcn_mfma_f32_4x4x1f32(a, w, acc[q], 0, 0, 0);
       }
@@ -597,6 +617,15 @@ inline size_t wake_rw_lds_bytes(int S) {
 
 // The group of kick ki (its first kick) for targets [tile * RW_TILE, +RW_TILE)
 // and source turn t; called by all threads of the block.
+// KM: the kernel's kind, 0 for the VALU precisions (FP64 with the past-turn
+// expansion, FP64 pair by pair, FP32), 1 for the matrix-core ones; each
+// kind is its own kernel, so that neither carries the other's registers.
+__host__ __device__ __forceinline__ bool rw_is_mfma(int precision) {
+  return precision == PHX_PREC_MFMA_F32 || precision == PHX_PREC_MFMA_BF16X3 ||
+         precision == PHX_PREC_MFMA_BF16X6;
+}
+
+template <int KM>
 __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const WakeKicks &kk, int nk,
                                               int S, double *part, int T, double *out, int ki,
                                               int tile, int t) {
@@ -677,7 +706,7 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const Wake
   const double r_off = ((smax - c_s) + (tgmax - c_t)) / A;
   const bool expand = w.precision == PHX_PREC_FP64 && uniform && t > 0 && A > 0.0 &&
                       r_off <= RW_EXP_R;
-  if (expand) {
+  if (KM == 0 && expand) {
     // U_p of each distinct moment: every thread sums its sources in order,
     // then the lanes (fixed butterfly) and the waves in order
     double U[3][RW_EXP_K] = {};
@@ -734,7 +763,7 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const Wake
         tot[u] = scale * f;
       }
     }
-  } else if (w.precision == PHX_PREC_MFMA_F32 || w.precision == PHX_PREC_MFMA_BF16X3) {
+  } else if (KM == 1) {
     // wave w: targets [16 w, 16 w + 16) of the tile, in quads
     double tq[RW_MF_QUADS];
 #pragma unroll
@@ -743,13 +772,15 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const Wake
       tq[q] = h->times[(size_t)head * S + (jt < S ? jt : S - 1)] - age;
     }
     double accd[RW_MF_QUADS][4] = {};
-    const bool bf = w.precision == PHX_PREC_MFMA_BF16X3;
+    const int mf = w.precision == PHX_PREC_MFMA_F32 ? 0 : (w.precision == PHX_PREC_MFMA_BF16X3 ? 1 : 2);
     if (uniform) {
-      if (bf) rw_mfma<true, true>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
-      else rw_mfma<true, false>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
+      if (mf == 0) rw_mfma<true, 0>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
+      else if (mf == 1) rw_mfma<true, 1>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
+      else rw_mfma<true, 2>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
     } else {
-      if (bf) rw_mfma<false, true>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
-      else rw_mfma<false, false>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
+      if (mf == 0) rw_mfma<false, 0>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
+      else if (mf == 1) rw_mfma<false, 1>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
+      else rw_mfma<false, 2>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
     }
     lds_sync();  // every wave is done with the sources
     if (lane < 4) {
@@ -806,6 +837,7 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const Wake
 // turn, so that consecutive blocks (which go to different XCDs) cover
 // different turns.  The block serves every resistive-wall group (in kick
 // order) whose memory reaches turn t.
+template <int KM>
 __device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const WakeKicks &kk, int nk,
                                               int S, double *part, int T, double *out) {
   const int tiles = (S + RW_TILE - 1) / RW_TILE;
@@ -813,12 +845,12 @@ __device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const Wake
   const int nf = h->n_filled;
   for (int ki = 0; ki < nk; ++ki) {
     const phx_wake_kick &w = kk.k[ki];
-    if (!is_rw_dot(w)) continue;
+    if (!is_rw_dot(w) || rw_is_mfma(w.precision) != (KM == 1)) continue;
     const int n_turns = nf < w.n_turns_wake ? nf : w.n_turns_wake;
     if (t >= n_turns) continue;
     bool first = true;
     for (int k = 0; k < ki; ++k) first = first && !same_shape(kk.k[k], w);
-    if (first) wake_rw_group(h, kk, nk, S, part, T, out, ki, tile, t);
+    if (first) wake_rw_group<KM>(h, kk, nk, S, part, T, out, ki, tile, t);
   }
 }
 

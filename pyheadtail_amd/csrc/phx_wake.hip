@@ -18,10 +18,11 @@ __global__ void __launch_bounds__(WK_THREADS)
   wake_conv_block(h, kk, S, part, T, out, gmax);
 }
 
+template <int KM>
 __global__ void __launch_bounds__(RW_THREADS)
     k_wake_rw(const phx_wake_hist *h, WakeKicks kk, int nk, int S, double *part, int T,
               double *out) {
-  wake_rw_block(h, kk, nk, S, part, T, out);
+  wake_rw_block<KM>(h, kk, nk, S, part, T, out);
 }
 
 __global__ void k_wake_conv_reduce(const phx_wake_hist *h, WakeKicks kk, int n_kicks, int S,
@@ -90,7 +91,7 @@ static int check_kicks(const phx_wake_kick *kicks, int32_t n_kicks, const char *
         (!k.table_t || !k.table_w || k.table_n < 2 || k.table_kind < PHX_TABLE_TRANS_UR ||
          k.table_kind > PHX_TABLE_LONG))
       return phx::fail(PHX_E_INVALID, "%s: kick %d bad wake table", who, i);
-    if (k.precision < PHX_PREC_FP64 || k.precision > PHX_PREC_MFMA_BF16X3)
+    if (k.precision < PHX_PREC_FP64 || k.precision > PHX_PREC_MFMA_BF16X6)
       return phx::fail(PHX_E_INVALID, "%s: kick %d bad precision %d", who, i, k.precision);
   }
   return PHX_OK;
@@ -115,8 +116,13 @@ int phx_wake_convolve_ws(const phx_wake_hist *hist, const phx_wake_kick *kicks, 
   if (wake_conv_lds_bytes(n_slices, 1) + WK_STATIC_LDS > phx_max_lds_bytes())
     return phx::fail(PHX_E_LIMIT, "phx_wake_convolve: n_slices=%d too large for LDS", n_slices);
   const int gmax = wake_conv_gmax(n_slices, phx_max_lds_bytes());
-  bool general = false, rw = false;
-  for (int i = 0; i < n_kicks; ++i) (is_rw_dot(kicks[i]) ? rw : general) = true;
+  bool general = false, rw_valu = false, rw_mfma = false;
+  for (int i = 0; i < n_kicks; ++i) {
+    if (!is_rw_dot(kicks[i])) general = true;
+    else if (rw_is_mfma(kicks[i].precision)) rw_mfma = true;
+    else rw_valu = true;
+  }
+  const bool rw = rw_valu || rw_mfma;
   if (rw && wake_rw_lds_bytes(n_slices) + RW_STATIC_LDS > phx_max_lds_bytes())
     return phx::fail(PHX_E_LIMIT, "phx_wake_convolve: n_slices=%d too large for LDS", n_slices);
   if (general) {
@@ -124,10 +130,17 @@ int phx_wake_convolve_ws(const phx_wake_hist *hist, const phx_wake_kick *kicks, 
         hist, kk, n_slices, part, T, kicks_out, gmax);
     PHX_CHECK_LAUNCH("phx_wake_convolve");
   }
-  if (rw) {
-    k_wake_rw<<<dim3((n_slices + RW_TILE - 1) / RW_TILE * T), RW_THREADS,
-                wake_rw_lds_bytes(n_slices), s>>>(hist, kk, n_kicks, n_slices, part, T, kicks_out);
+  if (rw_valu) {
+    k_wake_rw<0><<<dim3((n_slices + RW_TILE - 1) / RW_TILE * T), RW_THREADS,
+                   wake_rw_lds_bytes(n_slices), s>>>(hist, kk, n_kicks, n_slices, part, T,
+                                                     kicks_out);
     PHX_CHECK_LAUNCH("phx_wake_convolve: resistive wall");
+  }
+  if (rw_mfma) {
+    k_wake_rw<1><<<dim3((n_slices + RW_TILE - 1) / RW_TILE * T), RW_THREADS,
+                   wake_rw_lds_bytes(n_slices), s>>>(hist, kk, n_kicks, n_slices, part, T,
+                                                     kicks_out);
+    PHX_CHECK_LAUNCH("phx_wake_convolve: resistive wall (matrix cores)");
   }
   if (T > 1) {
     const int tot = n_kicks * n_slices;

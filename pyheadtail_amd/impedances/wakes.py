@@ -339,7 +339,7 @@ class WakeField(Element):
 
     PRECISIONS = {'fp64': _phx.PREC_FP64, 'fp32': _phx.PREC_FP32,
                   'fp64_direct': _phx.PREC_FP64_DIRECT, 'mfma_fp32': _phx.PREC_MFMA_F32,
-                  'mfma_bf16x3': _phx.PREC_MFMA_BF16X3}
+                  'mfma_bf16x3': _phx.PREC_MFMA_BF16X3, 'mfma_bf16x6': _phx.PREC_MFMA_BF16X6}
 
     def __init__(self, slicer, *wake_sources, precision='fp64'):
         """precision (not in PyHEADTAIL) of the multi-turn resistive-wall
@@ -352,8 +352,10 @@ class WakeField(Element):
                         runs of n_slices/8 sources, in FP64 beyond);
           'mfma_fp32'   every pair on the matrix cores, FP32 MFMA with FP32
                         accumulation over 256 sources (FP64 beyond);
-          'mfma_bf16x3' the same with bfloat16 MFMA on split (hi + lo)
-                        wake values and moments."""
+          'mfma_bf16x3' the same with bfloat16 MFMA on wake values and
+                        moments split in two bfloat16 parts (three
+                        products, about 16 significant bits);
+          'mfma_bf16x6' three parts each, six products."""
         if precision not in self.PRECISIONS:
             raise ValueError("precision must be one of %s" % sorted(self.PRECISIONS))
         self.slicer = slicer

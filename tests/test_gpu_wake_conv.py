@@ -148,13 +148,14 @@ def test_rw_kicks_of_one_source_share_sums():
 
 
 @pytest.mark.parametrize("prec,bound", [("fp64", 1e-13), ("fp32", 1e-6), ("mfma_fp32", 1e-6),
-                                        ("mfma_bf16x3", 1e-6)])
+                                        ("mfma_bf16x3", 2e-5), ("mfma_bf16x6", 1e-6)])
 def test_resistive_wall_kick_error_per_component(prec, bound):
     """Each precision of the multi-turn wall against FP64 pair by pair (the
     reference's sums), 1000 slices x 64 turns of a parallel-plates wall: max
     and RMS error per kick component, relative to the component's max and
     RMS.  FP64 takes the past turns through the separable expansion (bound
-    1e-13); the FP32 and matrix-core paths sum every pair (bound 1e-6)."""
+    1e-13); the FP32 and matrix-core paths sum every pair (bound 1e-6; the
+    two-part bfloat16 split carries about 16 bits, bound 2e-5)."""
     from pyheadtail_amd import _phx
     from pyheadtail_amd.impedances.wakes import WakeField
     print("\n[case] %s vs FP64 pair by pair, resistive wall, S=1000 T=64" % prec, flush=True)
