@@ -76,6 +76,14 @@ def parse():
     ap.add_argument("--no-sc", action="store_true",
                     help="served stream without the space-charge kicks of the largest jobs")
     ap.add_argument("--streams", type=int, default=12)
+    ap.add_argument("--tune-file", default=None,
+                    help="autotuner table (default ~/.cache/pyheadtail_amd/tune.json)")
+    ap.add_argument("--retune", action="store_true", help="measure again even if stored")
+    ap.add_argument("--traffic-from", metavar="DIR",
+                    help="only record profiles/pass_traffic.json from rocprofv3 --pmc output "
+                         "under DIR: subdirectories fetch_pass, write_pass, fetch_copy and "
+                         "write_copy (FETCH_SIZE / WRITE_SIZE of the pass at --n "
+                         "macroparticles and of --copy-calibration at the same --n)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of the pass's algorithmic bytes "
                          "(48 B read + 48 B written per macroparticle), for scaling the "
@@ -201,6 +209,49 @@ def pass_traffic(n):
         written = t["write_kib"] * t["copy_bytes"] / t["copy_write_kib"]
         return read + written
     return None
+
+
+def _counter_means(d, counter, match):
+    """Mean per dispatch of a rocprofv3 counter over the kernels whose name
+    contains `match`, from the counter_collection CSV files under d."""
+    import csv
+    import glob
+    vals = {}
+    for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter or match not in row.get("Kernel_Name", ""):
+                    continue
+                key = row.get("Dispatch_Id") or row.get("Correlation_Id") or len(vals)
+                vals[key] = vals.get(key, 0.0) + float(row["Counter_Value"])
+    if not vals:
+        raise SystemExit("no %s values for kernels matching %r under %s" % (counter, match, d))
+    return sum(vals.values()) / len(vals), len(vals)
+
+
+def record_traffic(d, n):
+    """profiles/pass_traffic.json: the pass's FETCH_SIZE and WRITE_SIZE (KiB
+    per dispatch) at n macroparticles and the same counters for a plain copy
+    of the six coordinate arrays, with the digest of the sources built."""
+    fetch, nd = _counter_means(os.path.join(d, "fetch_pass"), "FETCH_SIZE", "k_engine_pass")
+    write, _ = _counter_means(os.path.join(d, "write_pass"), "WRITE_SIZE", "k_engine_pass")
+    cfetch, _ = _counter_means(os.path.join(d, "fetch_copy"), "FETCH_SIZE", "opy")
+    cwrite, _ = _counter_means(os.path.join(d, "write_copy"), "WRITE_SIZE", "opy")
+    rec = {"kernel": "k_engine_pass (headline map)", "macroparticles": n,
+           "fetch_kib": fetch, "write_kib": write, "dispatches": nd,
+           "copy_bytes": 48 * n, "copy_fetch_kib": cfetch, "copy_write_kib": cwrite,
+           "sources_sha256": sources_sha256()}
+    path = os.path.join(ROOT, "profiles", "pass_traffic.json")
+    try:
+        with open(path) as f:
+            recs = json.load(f)
+        recs = recs if isinstance(recs, list) else [recs]
+    except (OSError, ValueError):
+        recs = []
+    recs = [r for r in recs if r.get("macroparticles") != n] + [rec]
+    with open(path, "w") as f:
+        json.dump(sorted(recs, key=lambda r: r["macroparticles"]), f, indent=1)
+    print(json.dumps(rec))
 
 
 def p0_of_gamma(gamma):
@@ -360,23 +411,43 @@ def scan_job(i, n_scale=1.0, seed_base=1000, bunch=True):
 
 
 def scan64(args):
-    """64 independent bunches: job by job against batched (one JSON line)."""
+    """64 independent bunches: one launch per stage for all of them (tuned
+    launch shape) against the plain GPU path job by job and against the
+    per-job graphs on several streams (one JSON line)."""
     import torch
     torch.cuda.set_device(0)
     from pyheadtail_amd.batch import BatchedScan
+    from pyheadtail_amd.tune import Tuner, tune_batch
     jobs = [scan_job(i) for i in range(64)]
-    scan = BatchedScan(jobs, n_streams=args.streams, chunk_turns=50,
-                       graph_turns=args.graph_turns,
-                       big_job_bytes=int(args.big_job_mib * (1 << 20)))
+    scan = BatchedScan(jobs, mode='fused', graph_turns=args.graph_turns)
     n_mp = scan.macroparticles
     steps = min(args.steps, 200)
-    scan.track_turns(40)                      # graphs, histories (T <= 32) filled
-    t0 = time.perf_counter()
-    scan.track_turns_job_by_job(steps)
-    t_seq = time.perf_counter() - t0
+    tuner = Tuner(args.tune_file) if args.tune_file else Tuner()
+    entry = tune_batch(scan, tuner=tuner, turns=20, force=args.retune)
+    print("[scan64] tuned: pass_grid %d, graph_turns %d (%.3f ms per turn)"
+          % (entry['pass_grid'], entry['graph_turns'], entry['ms_per_turn']), file=sys.stderr,
+          flush=True)
+    scan.track_turns(40)                      # histories (T <= 32) filled, graphs captured
     t0 = time.perf_counter()
     scan.track_turns(steps)
     t_bat = time.perf_counter() - t0
+    # the plain GPU path job by job: each job's own engine at its default
+    # pass grid, one after the other
+    tuned = scan.pass_grid
+    scan.configure(pass_grid=0)
+    scan.track_turns_job_by_job(40)
+    t0 = time.perf_counter()
+    scan.track_turns_job_by_job(steps)
+    t_seq = time.perf_counter() - t0
+    scan.configure(pass_grid=tuned)
+    # per-job graphs on 12 streams
+    streams = BatchedScan(jobs, mode='streams', n_streams=args.streams, chunk_turns=50,
+                          graph_turns=args.graph_turns,
+                          big_job_bytes=int(args.big_job_mib * (1 << 20)))
+    streams.track_turns(40)
+    t0 = time.perf_counter()
+    streams.track_turns(steps)
+    t_str = time.perf_counter() - t0
     out = {
         "metric": "aggregate macroparticle-turns/s, 64-bunch scan",
         "value": n_mp * steps / t_bat, "unit": "macroparticle-turns/s", "n_gpus": 1,
@@ -387,9 +458,13 @@ def scan64(args):
                                "[2e5, 4e6] (%d in total), slices in {100,250,500,1000}, "
                                "chromaticity in [-5, 15], n_turns_wake in {1,8,32}; LinearMap "
                                "ring, resonator or resistive-wall wake" % n_mp,
-                   "bunches": 64, "macroparticles": n_mp, "streams": args.streams},
+                   "bunches": 64, "macroparticles": n_mp},
+        "tuned": {k: entry[k] for k in ('pass_grid', 'graph_turns', 'ms_per_turn', 'candidates')},
         "job_by_job": {"value": n_mp * steps / t_seq, "ms_per_step": t_seq / steps * 1e3},
+        "streams": {"value": n_mp * steps / t_str, "ms_per_step": t_str / steps * 1e3,
+                    "streams": args.streams},
         "speedup_vs_job_by_job": t_seq / t_bat,
+        "speedup_streams_vs_job_by_job": t_seq / t_str,
     }
     print(json.dumps(out))
 
@@ -559,6 +634,9 @@ def main():
               "(independent bunches would shard across GPUs without communication). "
               "Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.traffic_from:
+        record_traffic(args.traffic_from, args.n)
+        return
     if args.copy_calibration:
         copy_calibration(args.n)
         return

@@ -631,6 +631,31 @@ enum { PHX_NOALLOC_RESET = 1, PHX_NOALLOC_FLUSH = 2 };
 int phx_engine_turns_noalloc(const phx_engine *e, int32_t n_turns, int32_t flags,
                              phx_stream_t stream);
 
+/* Batches (the batched scan): n engines (independent bunches of any
+   macroparticle count, slice count, wake memory and kick set; each with its
+   own buffers, wake field required) advanced together, each stage of a turn
+   one launch covering every engine (block offsets per engine in a device
+   descriptor table), each engine's results those of its run alone.  The
+   engines must share record_stats, program_shape and det_shape.  No
+   counterpart in the reference, whose scans run one bunch after another
+   (machines/synchrotron.py:294-298 per bunch).
+   phx_batch_create copies the descriptor table (phx_batch_desc_bytes(n),
+   caller-owned device memory) on `stream` and returns a host handle; the
+   engines' buffers must stay where they are while it is used.
+   phx_batch_turns: particle 0 of every engine, then n_turns turns.
+   phx_batch_capture: a HIP graph of n_turns turns (replayed with
+   phx_graph_launch, destroyed with phx_graph_destroy), without the
+   particle-0 record: phx_batch_snapshot before the first replay after
+   coordinates changed outside the engines.  The pending kick of each engine
+   is applied by phx_engine_flush of that engine. */
+int64_t phx_batch_desc_bytes(int32_t n);
+int phx_batch_create(const phx_engine *engines, int32_t n, void *desc, int64_t desc_bytes,
+                     phx_stream_t stream, void **batch);
+int phx_batch_turns(void *batch, int32_t n_turns, phx_stream_t stream);
+int phx_batch_snapshot(void *batch, phx_stream_t stream);
+int phx_batch_capture(void *batch, int32_t n_turns, void **graph);
+int phx_batch_destroy(void *batch);
+
 /* HIP graph of n_turns fused turns (captured on an internal stream);
    replay with phx_graph_launch.  Returns an opaque handle.  The graph does
    not record particle 0 (each turn refreshes it for the next): call

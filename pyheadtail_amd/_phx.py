@@ -232,6 +232,12 @@ SIGNATURES = {
     "phx_engine_stage": (C.c_int, [P, I32, P]),
     "phx_engine_snapshot": (C.c_int, [P, P]),
     "phx_engine_turns_noalloc": (C.c_int, [P, I32, I32, P]),
+    "phx_batch_desc_bytes": (I64, [I32]),
+    "phx_batch_create": (C.c_int, [P, I32, P, I64, P, C.POINTER(C.c_void_p)]),
+    "phx_batch_turns": (C.c_int, [P, I32, P]),
+    "phx_batch_snapshot": (C.c_int, [P, P]),
+    "phx_batch_capture": (C.c_int, [P, I32, C.POINTER(C.c_void_p)]),
+    "phx_batch_destroy": (C.c_int, [P]),
     "phx_graph_capture": (C.c_int, [P, I32, C.POINTER(C.c_void_p)]),
     "phx_graph_launch": (C.c_int, [P, P]),
     "phx_graph_destroy": (C.c_int, [P]),

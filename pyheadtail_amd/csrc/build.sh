@@ -16,7 +16,7 @@ bdir="${PHX_BUILD_DIR:-$here/_build}"
 objs=()
 pids=()
 mkdir -p "$bdir"
-for src in phx_api phx_stats phx_track phx_wake phx_engine phx_spacecharge phx_sort; do
+for src in phx_api phx_stats phx_track phx_wake phx_engine phx_batch phx_spacecharge phx_sort; do
   "$HIPCC" "${FLAGS[@]}" "${defs[@]}" -c "$here/$src.hip" -o "$bdir/$src.o" &
   pids+=($!)
   objs+=("$bdir/$src.o")

@@ -1,0 +1,484 @@
+// Device side of the fused one-turn map (the engine): workspace layout,
+// launch arguments and the bodies of the per-turn stages, shared by the
+// single-engine kernels (phx_engine.hip) and the batch kernels, which run
+// one stage of many engines in one launch (phx_batch.hip).
+#pragma once
+
+#include "phx_device.h"
+#include "phx_slicing.h"
+#include "phx_wake.h"
+
+#define EN_STAT_NV 33
+#define EN_NV (ZB_NV + EN_STAT_NV)
+#define EN_HIST_BLOCKS 256
+#define EN_HIST_THREADS 1024
+
+namespace phx_en {
+
+
+inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+struct EngineWs {
+  double *partials;   // [EN_NV][pass grid] (slot-major)
+  double *p0;         // particle 0 before the pass: the shifts of its sums
+  AccParts acc;       // H partial histograms, T = 2 (x, y)
+  double *conv_part;  // [n_kicks][T][S]
+};
+
+__host__ __device__ inline int hist_blocks(int64_t n) {
+  const int64_t nc = (n + PHX_CHUNK - 1) / PHX_CHUNK;
+  return (int)(nc < EN_HIST_BLOCKS ? nc : EN_HIST_BLOCKS);
+}
+
+// The partials region holds the largest pass grid, so the layout does not
+// depend on the grid chosen.
+__host__ __device__ inline EngineWs engine_ws(void *ws, int64_t n, int S) {
+  EngineWs w;
+  char *p = reinterpret_cast<char *>(ws);
+  const int64_t nc = phx_grid_for(n, PHX_MAX_GRID);
+  const int Se = S > 0 ? S : 1;
+  w.partials = reinterpret_cast<double *>(p);
+  p += ((size_t)nc * EN_NV * 8 + 255) / 256 * 256;
+  w.p0 = reinterpret_cast<double *>(p);
+  p += 256;
+  const int H = hist_blocks(n);
+  w.acc = acc_parts_layout(p, H, Se, 2);
+  p += ((size_t)H * Se * 4 + 255) / 256 * 256 + (size_t)H * 2 * Se * 16 + (size_t)2 * Se * 8;
+  w.conv_part = reinterpret_cast<double *>(p);
+  return w;
+}
+
+inline size_t engine_ws_bytes(int64_t n, int S, int n_kicks, int T) {
+  const int64_t nc = phx_grid_for(n, PHX_MAX_GRID);
+  const int Se = S > 0 ? S : 1;
+  size_t b = round256((size_t)nc * EN_NV * 8) + 256 + acc_parts_bytes(hist_blocks(n), Se, 2);
+  b += (size_t)(n_kicks > 0 ? n_kicks : 1) * T * Se * 8;
+  return round256(b);
+}
+
+struct EngineArgs {
+  phx_engine e;
+  int n_kicks;
+  int T;
+  int grid;  // blocks of the particle pass (its partials: [slot][grid])
+  int gmax;  // kicks one convolution block serves (LDS permitting)
+};
+
+__device__ __forceinline__ P6 load_p(const phx_engine &e, int64_t j) {
+  P6 p;
+  p.x = e.coords[0][j];
+  p.xp = e.coords[1][j];
+  p.y = e.coords[2][j];
+  p.yp = e.coords[3][j];
+  p.z = e.coords[4][j];
+  p.dp = e.coords[5][j];
+  return p;
+}
+
+__device__ __forceinline__ void store_p(const phx_engine &e, int64_t j, const P6 &p) {
+  e.coords[0][j] = p.x;
+  e.coords[1][j] = p.xp;
+  e.coords[2][j] = p.y;
+  e.coords[3][j] = p.yp;
+  e.coords[4][j] = p.z;
+  e.coords[5][j] = p.dp;
+}
+
+// nk = min(program kicks, engine kicks): the kick table holds the latter.
+template <int KS>
+__device__ __forceinline__ void kick_pending(P6 &p, const phx_turn_program &g, int nk,
+                                             double tail, double width, double inv_width,
+                                             int S, const double *kicks) {
+  const double f = slice_floor_r(p.z, tail, width, inv_width);
+  if (f >= 0.0 && f < (double)S) apply_kick_set<KS>(p, (int32_t)f, S, g.kick, nk, kicks);
+}
+
+// Particles k*PHX_BLOCK + t (k < PHX_PPT) of the group starting at the
+// six pointers c (uniform: scalar base plus the thread's 32-bit offset).
+// lim: the group's particle count if it is the partial last group, else
+// PHX_CHUNK (no guard needed).
+struct GroupPtr {
+  double *c[6];
+};
+
+__device__ __forceinline__ void group_load(const GroupPtr &gp, int lim, P6 (&p)[PHX_PPT]) {
+#pragma unroll
+  for (int k = 0; k < PHX_PPT; ++k) {
+    const int o = (int)threadIdx.x + k * PHX_BLOCK;
+    if (lim == PHX_CHUNK || o < lim)
+      p[k] = P6{gp.c[0][o], gp.c[1][o], gp.c[2][o], gp.c[3][o], gp.c[4][o], gp.c[5][o]};
+    else
+      p[k] = P6{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  }
+}
+
+__device__ __forceinline__ void group_store(const GroupPtr &gp, int lim,
+                                            const P6 (&p)[PHX_PPT]) {
+#pragma unroll
+  for (int k = 0; k < PHX_PPT; ++k) {
+    const int o = (int)threadIdx.x + k * PHX_BLOCK;
+    if (lim == PHX_CHUNK || o < lim) {
+      gp.c[0][o] = p[k].x; gp.c[1][o] = p[k].xp; gp.c[2][o] = p[k].y;
+      gp.c[3][o] = p[k].yp; gp.c[4][o] = p[k].z; gp.c[5][o] = p[k].dp;
+    }
+  }
+}
+
+// stats slots: [0..6) sum u, [6..12) sum (u-u0), [12..33) sum (u-u0)(v-v0)
+__device__ __forceinline__ void stats_accumulate(double *v, const P6 &p, const double *sh) {
+  const double u[6] = {p.x, p.xp, p.y, p.yp, p.z, p.dp};
+  double d[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    d[i] = u[i] - sh[i];
+    v[i] = v[i] + u[i];
+    v[6 + i] = v[6 + i] + d[i];
+  }
+  int q = 12;
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = i; j < 6; ++j) {
+      v[q] = v[q] + d[i] * d[j];
+      ++q;
+    }
+}
+
+template <bool REC, bool WAKE>
+struct PassLayout {
+  static constexpr int NV = (WAKE ? ZB_NV : 0) + (REC ? EN_STAT_NV : 0) + ((!WAKE && !REC) ? 1 : 0);
+  static constexpr unsigned long long MINM = WAKE ? ZB_MINM : 0ULL;
+  static constexpr unsigned long long MAXM = WAKE ? ZB_MAXM : 0ULL;
+  static constexpr int SOFF = WAKE ? ZB_NV : 0;
+};
+
+// MODE 0: a turn; MODE 1: flush (pending kick + final statistics only).
+// Persistent grid (a.grid blocks): each block walks its particle
+// groups with the next group's coordinates loaded while the current one is
+// computed, so the particle traffic overlaps the FP64 work.
+// No workgroup barrier or fence precedes the last read of the turn program
+// and the coordinate arrays are declared non-aliasing (__restrict__
+// arguments): otherwise the compiler must assume the program may have been
+// written and reloads it with vector loads that wait on the particle loads.
+// So particle 0 is computed by every lane, and the kick table is read from
+// global memory (it stays in the caches).  Dynamic LDS: the block reduction.
+#ifndef PHX_PASS_MIN_BLOCKS
+#define PHX_PASS_MIN_BLOCKS 1
+#endif
+#ifndef PHX_PASS_LAUNDER
+#define PHX_PASS_LAUNDER 0
+#endif
+template <int MODE, bool REC, bool WAKE, int SHAPE, int KS, int DS>
+__device__ __forceinline__ void pass_body(
+    const EngineArgs &a, double *__restrict__ x, double *__restrict__ xp,
+    double *__restrict__ y, double *__restrict__ yp, double *__restrict__ z,
+    double *__restrict__ dp, const phx_turn_program *__restrict__ programs,
+    const double *__restrict__ kicks, int bid, int nblk) {
+  using L = PassLayout<REC, WAKE>;
+  const phx_engine &e = a.e;
+  extern __shared__ double pass_lds[];
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
+  const int64_t n = e.n;
+  const int turn = *e.turn_counter;
+  const int prog_turn = MODE == 0 ? turn : (turn > 0 ? turn - 1 : 0);
+  const phx_turn_program &g = programs[e.turn_program[prog_turn]];
+  const int pend = *e.pending;
+  const double tail = e.state->tail, width = e.state->width;
+  const double inv_width = 1.0 / width;
+  const int S = e.n_slices;
+  const int nk = g.n_kicks < a.n_kicks ? g.n_kicks : a.n_kicks;
+  const int64_t ng = phx_n_groups(n);
+  // the group's base pointers and particle count (uniform)
+  auto group = [&](int64_t gg, GroupPtr &gp) {
+    const int64_t b = gg * PHX_CHUNK;
+    gp.c[0] = x + b; gp.c[1] = xp + b; gp.c[2] = y + b;
+    gp.c[3] = yp + b; gp.c[4] = z + b; gp.c[5] = dp + b;
+    const int64_t rest = n - b;
+    return rest >= PHX_CHUNK ? PHX_CHUNK : (int)rest;
+  };
+  int64_t gi = bid;
+  GroupPtr gc, gnx;
+  int lc = group(gi, gc);
+  P6 cur[PHX_PPT], nxt[PHX_PPT];
+  group_load(gc, lc, cur);
+  // Particle 0 sets the shifts of the shifted sums (stats.pyx:cov): its
+  // state after the pending kick (statistics), and its z after this turn's
+  // elements (cuts), which needs the kick only where a kick changes dp and
+  // only the longitudinal elements.  Every lane computes it (uniform values;
+  // once per block) from the snapshot taken before this pass: block 0
+  // rewrites particle 0 while later blocks may still be starting.
+  double src[7];
+  {
+    const double *p0 = w.p0;
+    P6 q = P6{p0[0], p0[1], p0[2], p0[3], p0[4], p0[5]};
+    bool kick_dp = false;
+    if constexpr (KS == PHX_KICKS_GENERIC) {
+      for (int k = 0; k < nk; ++k) kick_dp = kick_dp || g.kick[k].plane == PHX_PLANE_DP;
+    }
+    if (pend && (REC || kick_dp)) kick_pending<KS>(q, g, nk, tail, width, inv_width, S, kicks);
+    src[0] = q.x; src[1] = q.xp; src[2] = q.y; src[3] = q.yp; src[4] = q.z; src[5] = q.dp;
+    if (MODE == 0) apply_shape_longitudinal<SHAPE>(q, g);
+    src[6] = q.z;
+  }
+  double v[L::NV];
+#pragma unroll
+  for (int i = 0; i < L::NV; ++i) v[i] = red_ident<L::MINM, L::MAXM>(i);
+  double shs[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) shs[i] = src[i];
+  const double zshift = src[6];
+  for (; gi < ng; gi += nblk) {
+    const int64_t gn = gi + nblk;
+    int ln = 0;
+    if (gn < ng) {
+      ln = group(gn, gnx);
+      group_load(gnx, ln, nxt);
+    }
+    if (pend) {
+#pragma unroll
+      for (int k = 0; k < PHX_PPT; ++k)
+        kick_pending<KS>(cur[k], g, nk, tail, width, inv_width, S, kicks);
+    }
+    const bool full = lc == PHX_CHUNK;
+    if (REC) {
+#pragma unroll
+      for (int k = 0; k < PHX_PPT; ++k)
+        if (full || (int)threadIdx.x + k * PHX_BLOCK < lc)
+          stats_accumulate(&v[L::SOFF], cur[k], shs);
+    }
+    if (MODE == 0) {
+      // PHX_PASS_LAUNDER: read the element parameters from the program in
+      // every group (a pointer the compiler cannot follow across
+      // iterations) instead of holding them in scalar registers over the
+      // loop; measured slower (more vector registers, 2 waves per SIMD)
+      const phx_turn_program *gl = &g;
+#if PHX_PASS_LAUNDER
+      asm volatile("" : "+s"(gl));
+#endif
+      apply_shape_n<SHAPE, PHX_PPT, DS>(cur, *gl);
+    }
+    if (WAKE) {
+#pragma unroll
+      for (int k = 0; k < PHX_PPT; ++k) {
+        if (full || (int)threadIdx.x + k * PHX_BLOCK < lc) {
+          double zb[ZB_NV];
+#pragma unroll
+          for (int i = 0; i < ZB_NV; ++i) zb[i] = v[i];
+          zblock_accumulate<0x05u>(zb, cur[k], zshift);  // max |x|, |y|: the histogram's
+#pragma unroll
+          for (int i = 0; i < ZB_NV; ++i) v[i] = zb[i];
+        }
+      }
+    }
+    if (MODE == 0 || pend) group_store(gc, lc, cur);
+#pragma unroll
+    for (int k = 0; k < PHX_PPT; ++k) cur[k] = nxt[k];
+    gc = gnx;
+    lc = ln;
+  }
+  const double r = block_reduce_slots<L::NV, L::MINM, L::MAXM>(v, pass_lds);
+  if (threadIdx.x < L::NV) w.partials[(size_t)threadIdx.x * nblk + bid] = r;
+}
+
+// The cuts of this turn from the pass partials (one block): fold the z
+// block (and the statistics slots) in a fixed order, the slicing state, the
+// statistics row, particle 0 for the next pass, and the wake-history
+// bookkeeping (WakeField.track: ages += C/(beta c), then the new entry at
+// age 0, impedances/wakes.py:134-142).  FOLD_ALL: this block does every
+// part (the batch's cuts stage, block 0 of the standalone histogram);
+// otherwise only the fold into res and the shared cut values (the other
+// histogram blocks).  Returns the state through *stp (thread 0).
+template <bool REC, bool FOLD_ALL>
+__device__ __forceinline__ void cuts_body(const EngineArgs &a, double *res,
+                                          phx_slice_state *stp) {
+  using L = PassLayout<REC, true>;
+  const phx_engine &e = a.e;
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
+  const int turn = *e.turn_counter;
+  const phx_turn_program &g = e.programs[e.turn_program[turn]];
+  if (FOLD_ALL && threadIdx.x < 6) w.p0[threadIdx.x] = e.coords[threadIdx.x][0];
+  fold_partials<L::MINM, L::MAXM>(w.partials, a.grid, 0, ZB_NV, res);
+  if constexpr (REC && FOLD_ALL) fold_partials<L::MINM, L::MAXM>(w.partials, a.grid, ZB_NV, L::NV, res);
+  if (threadIdx.x == 0) {
+    phx_slice_state st;
+    compute_cuts(g.slicer, e.n, res[0], res[1], res[2], res[3], res[4], st);
+    for (int i = 0; i < 6; ++i) st.absmax[i] = res[5 + i];
+    st.turn = turn;
+    st.pad = 0;
+    *stp = st;
+    if (FOLD_ALL) {
+      *e.state = st;
+      phx_wake_hist *h = e.hist;
+      const int cap = h->capacity, nf = h->n_filled, head = h->head;
+      for (int i = 0; i < nf; ++i) {
+        const int sl = ring_slot(head, i, cap);
+        h->age[sl] = h->age[sl] + g.age_increment;
+      }
+      const int nh = (head + 1) % cap;
+      h->head = nh;
+      h->n_filled = nf + 1 < cap ? nf + 1 : cap;
+      h->age[nh] = 0.0;
+      h->beta[nh] = g.beta;
+    }
+  }
+  if (REC && FOLD_ALL && threadIdx.x < EN_STAT_NV)
+    e.stats[(size_t)turn * EN_STAT_NV + threadIdx.x] = res[L::SOFF + threadIdx.x];
+}
+
+// Slice histogram of the new state: counts, sum x and sum y per slice in an
+// LDS histogram, stored as partial bid of nblk.  FOLD: the block first folds
+// the pass partials into the cuts itself (every block, identical results;
+// block 0 also publishes them and does the turn's bookkeeping); otherwise
+// the cuts stage did that and the block reads the state.
+template <bool REC, bool FOLD>
+__device__ __forceinline__ void hist_body(const EngineArgs &a, int bid, int nblk) {
+  using L = PassLayout<REC, true>;
+  extern __shared__ unsigned long long acc_lds[];
+  __shared__ double res[L::NV];
+  __shared__ phx_slice_state sh_st;
+  const phx_engine &e = a.e;
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
+  const int S = e.n_slices;
+  constexpr int CH = EN_HIST_THREADS * PHX_PPT;
+  const int64_t nc = (e.n + CH - 1) / CH;
+  const int64_t n = e.n;
+  // Each chunk loads the next one before it is binned.  No particle load is
+  // in flight across a barrier: with the first chunk's loads issued ahead
+  // of the fold of the cuts, some blocks' folds came out wrong on MI355X
+  // (about 1 in 25 block folds on a new engine's first turns; none with
+  // the loads issued after the last barrier).
+  double zz[PHX_PPT], xx[PHX_PPT], yy[PHX_PPT];
+  double zn[PHX_PPT], xn[PHX_PPT], yn[PHX_PPT];
+  auto load_chunk = [&](int64_t c, double (&z)[PHX_PPT], double (&x)[PHX_PPT],
+                        double (&y)[PHX_PPT]) {
+#pragma unroll
+    for (int k = 0; k < PHX_PPT; ++k) {
+      const int64_t j = c * CH + threadIdx.x + (int64_t)k * EN_HIST_THREADS;
+      if (c < nc && j < n) {
+        z[k] = e.coords[4][j];
+        x[k] = e.coords[0][j];
+        y[k] = e.coords[2][j];
+      } else {
+        z[k] = 0.0;
+        x[k] = 0.0;
+        y[k] = 0.0;
+      }
+    }
+  };
+  if constexpr (FOLD) {
+    if (bid == 0) cuts_body<REC, true>(a, res, &sh_st);
+    else cuts_body<REC, false>(a, res, &sh_st);
+  } else {
+    if (threadIdx.x == 0) sh_st = *e.state;
+  }
+  AccLds A = acc_lds_layout(acc_lds, S, 2);
+  acc_lds_zero(A, S, 2);  // ends with a barrier: the shared cuts are visible
+  const double tail = sh_st.tail, width = sh_st.width;
+  const double inv_width = 1.0 / width;
+  const int ex = fx_exponent(sh_st.absmax[0]), ey = fx_exponent(sh_st.absmax[2]);
+  load_chunk(bid, zz, xx, yy);
+  for (int64_t c = bid; c < nc; c += nblk) {
+    load_chunk(c + nblk, zn, xn, yn);
+    const int64_t base = c * CH + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < PHX_PPT; ++k) {
+      const int64_t j = base + (int64_t)k * EN_HIST_THREADS;
+      if (j < n) {
+        const double f = slice_floor_r(zz[k], tail, width, inv_width);
+        if (f >= 0.0 && f < (double)S) {
+          const int s = (int)f;
+          atomicAdd(&A.counts[s], 1u);
+          acc_add_value(A, S, 0, s, xx[k], ex, w.acc.fallback);
+          acc_add_value(A, S, 1, s, yy[k], ey, w.acc.fallback);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PHX_PPT; ++k) {
+      zz[k] = zn[k];
+      xx[k] = xn[k];
+      yy[k] = yn[k];
+    }
+  }
+  acc_lds_store(A, S, 2, w.acc, bid);
+}
+
+// Merge the partial histograms; means and the new history entry:
+// times = z_centers / (beta c) (particles/slicing.py:124-126, 303-307),
+// moments n, n*mean_x, n*mean_y (impedances/wake_kicks.py:217, 235).
+template <int SL>
+__device__ __forceinline__ void merge_body(const EngineArgs &a, int bid) {
+  const phx_engine &e = a.e;
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
+  const int S = e.n_slices;
+  SliceTotals tot;
+  const int turn = e.state->turn;
+  if (bid == 0 && threadIdx.x == 0) *e.turn_counter = turn + 1;
+  if (!acc_merge<2, SL>(w.acc, S, tot, bid)) return;
+  const int s = bid * SL + threadIdx.x % SL;
+  // bin edges as np.arange(tail, head + 1e-7*width, width)
+  const double tail = e.state->tail, width = e.state->width;
+  e.z_bins[s] = arange_value(tail, width, s);
+  if (s == S - 1) e.z_bins[S] = arange_value(tail, width, S);
+  const phx_turn_program &g = e.programs[e.turn_program[turn]];
+  const int ex = fx_exponent(e.state->absmax[0]);
+  const int ey = fx_exponent(e.state->absmax[2]);
+  const double cnt = (double)tot.count;
+  const double sx = fx_value(tot.hi[0], tot.lo[0], ex) + w.acc.fallback[s];
+  const double sy = fx_value(tot.hi[1], tot.lo[1], ey) + w.acc.fallback[S + s];
+  w.acc.fallback[s] = 0.0;
+  w.acc.fallback[S + s] = 0.0;
+  const double mx = tot.count ? sx / cnt : 0.0;
+  const double my = tot.count ? sy / cnt : 0.0;
+  e.counts[s] = (int32_t)tot.count;
+  e.slice_means[s] = mx;
+  e.slice_means[S + s] = my;
+  phx_wake_hist *h = e.hist;
+  const int nh = h->head;
+  const double zl = arange_value(tail, width, s), zr = arange_value(tail, width, s + 1);
+  const double zc = zl + 0.5 * (zr - zl);
+  h->times[(size_t)nh * S + s] = zc / g.beta_c;
+  double *mm = h->mom + (size_t)nh * 3 * S;
+  mm[s] = cnt;
+  mm[S + s] = cnt * mx;
+  mm[2 * S + s] = cnt * my;
+}
+
+// Slices per merge block: 4 up to 512 slices, 8 above (5.2 / 6.3 us at 500
+// slices, 7.2 / 6.3 us at 1000); integer sums, the same results.
+__host__ __device__ inline int merge_sl(int S) { return S <= 512 ? 4 : 8; }
+
+// Wake convolution block (tile, ki, t) of the general kernel.
+__device__ __forceinline__ void conv_body(const EngineArgs &a, int tile, int ki, int t) {
+  const phx_engine &e = a.e;
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
+  const phx_turn_program &g = e.programs[e.turn_program[e.state->turn]];
+  const WakeKicks &kk = *reinterpret_cast<const WakeKicks *>(g.kick);
+  if (tile == 0 && ki == 0 && t == 0 && threadIdx.x == 0) *e.pending = 1;
+  wake_conv_block(e.hist, kk, e.n_slices, w.conv_part, a.T, e.kicks, a.gmax, tile, ki, t,
+                  a.n_kicks);
+}
+
+template <int KM>
+__device__ __forceinline__ void conv_rw_body(const EngineArgs &a, int bid) {
+  const phx_engine &e = a.e;
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
+  const phx_turn_program &g = e.programs[e.turn_program[e.state->turn]];
+  const WakeKicks &kk = *reinterpret_cast<const WakeKicks *>(g.kick);
+  if (bid == 0 && threadIdx.x == 0) *e.pending = 1;
+  wake_rw_block<KM>(e.hist, kk, a.n_kicks, e.n_slices, w.conv_part, a.T, e.kicks, bid);
+}
+
+__device__ __forceinline__ void conv_reduce_body(const EngineArgs &a, int bid) {
+  const phx_engine &e = a.e;
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
+  const phx_turn_program &g = e.programs[e.turn_program[e.state->turn]];
+  const WakeKicks &kk = *reinterpret_cast<const WakeKicks *>(g.kick);
+  wake_conv_reduce(e.hist, kk, a.n_kicks, e.n_slices, w.conv_part, a.T, e.kicks, bid);
+}
+
+// Launch arguments of engine e (checked): the record, its kick count and
+// memory depth, the pass grid and the convolution blocks (phx_engine.hip).
+int engine_args_for(const phx_engine *e, EngineArgs *out);
+
+}  // namespace phx_en

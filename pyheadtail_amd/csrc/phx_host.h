@@ -17,6 +17,11 @@ inline hipStream_t as_stream(phx_stream_t s) { return reinterpret_cast<hipStream
 
 inline int64_t n_chunks(int64_t n) { return phx_n_groups(n); }
 
+// The opaque handle of phx_graph_capture / phx_batch_capture.
+struct GraphHandle {
+  hipGraphExec_t exec;
+};
+
 // Byte layout of the shared workspace of the reduction kernels.
 struct WsLayout {
   static constexpr size_t kTicketBytes = 256;         // reserved head of the workspace

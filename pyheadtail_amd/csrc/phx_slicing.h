@@ -206,13 +206,14 @@ struct SliceTotals {
   unsigned long long hi[PHX_MAX_TERMS], lo[PHX_MAX_TERMS];
 };
 
+// bid: the tile of SL slices (blockIdx.x of the standalone launch)
 template <int T, int SL = RD_SL>
-__device__ __forceinline__ bool acc_merge(const AccParts &P, int S, SliceTotals &out) {
+__device__ __forceinline__ bool acc_merge(const AccParts &P, int S, SliceTotals &out, int bid) {
   constexpr int NVAL = 2 * T + 1;
   constexpr int NW = PHX_BLOCK / PHX_WAVE;
   __shared__ unsigned long long wsum[NW][NVAL][SL];
   const int sl = threadIdx.x % SL, g = threadIdx.x / SL;
-  const int s = blockIdx.x * SL + sl;
+  const int s = bid * SL + sl;
   // v[0] = count, v[1 + 2i] = hi of term i, v[2 + 2i] = lo of term i
   unsigned long long v[NVAL];
 #pragma unroll

@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(RD_SL * RD_G)
     k_slice_acc_merge(AccParts P, int S, AccTerms terms, const phx_slice_state *state,
                       int32_t *counts, double *sums, double *means) {
   SliceTotals tot;
-  if (!acc_merge<T>(P, S, tot)) return;
+  if (!acc_merge<T>(P, S, tot, blockIdx.x)) return;
   const int s = blockIdx.x * RD_SL + threadIdx.x % RD_SL;
   if (counts) counts[s] = (int32_t)tot.count;
 #pragma unroll

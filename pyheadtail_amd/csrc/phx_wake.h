@@ -223,15 +223,16 @@ __device__ __host__ __forceinline__ bool is_rw_dot(const phx_wake_kick &k) {
 
 #define WK_GMAX 4  // kicks one block serves
 
+// Block (tile, ki, t) of nk kick rows: blockIdx.(x, y, z) and gridDim.y of
+// the standalone launch, explicit in a batch.
 __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const WakeKicks &kk,
                                                 int S, double *part, int T, double *out,
-                                                int gmax) {
+                                                int gmax, int tile, int ki, int t, int nk) {
   // LDS, convolution form: [WK_TILE + S - 1] wake values, then [S] source
   // moments; dot-product form: [S] source times, then [gmax][S] moments.
   extern __shared__ double vbuf[];
   __shared__ double red[WK_GMAX][WK_THREADS];
   __shared__ double rwp[WK_GMAX][3];
-  const int tile = blockIdx.x, ki = blockIdx.y, t = blockIdx.z, nk = gridDim.y;
   const phx_wake_kick &w = kk.k[ki];
   if (is_rw_dot(w)) return;  // wake_rw_block's
   const int nf = h->n_filled, head = h->head, cap = h->capacity;
@@ -315,8 +316,8 @@ __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const Wa
 // Sum of the per-turn convolutions in turn order, times the wake factor.
 __device__ __forceinline__ void wake_conv_reduce(const phx_wake_hist *h, const WakeKicks &kk,
                                                  int n_kicks, int S, const double *part, int T,
-                                                 double *out) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+                                                 double *out, int bid) {
+  const int idx = bid * blockDim.x + threadIdx.x;
   if (idx >= n_kicks * S) return;
   const int ki = idx / S, j = idx % S;
   const phx_wake_kick &w = kk.k[ki];
@@ -839,9 +840,9 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const Wake
 // order) whose memory reaches turn t.
 template <int KM>
 __device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const WakeKicks &kk, int nk,
-                                              int S, double *part, int T, double *out) {
+                                              int S, double *part, int T, double *out, int bid) {
   const int tiles = (S + RW_TILE - 1) / RW_TILE;
-  const int tile = blockIdx.x % tiles, t = blockIdx.x / tiles;
+  const int tile = bid % tiles, t = bid / tiles;
   const int nf = h->n_filled;
   for (int ki = 0; ki < nk; ++ki) {
     const phx_wake_kick &w = kk.k[ki];

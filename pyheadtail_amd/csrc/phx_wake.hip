@@ -15,19 +15,19 @@ __global__ void k_hist_push(phx_wake_hist *h, const double *times, const int32_t
 __global__ void __launch_bounds__(WK_THREADS)
     k_wake_conv(const phx_wake_hist *h, WakeKicks kk, int S, double *part, int T, double *out,
                 int gmax) {
-  wake_conv_block(h, kk, S, part, T, out, gmax);
+  wake_conv_block(h, kk, S, part, T, out, gmax, blockIdx.x, blockIdx.y, blockIdx.z, gridDim.y);
 }
 
 template <int KM>
 __global__ void __launch_bounds__(RW_THREADS)
     k_wake_rw(const phx_wake_hist *h, WakeKicks kk, int nk, int S, double *part, int T,
               double *out) {
-  wake_rw_block<KM>(h, kk, nk, S, part, T, out);
+  wake_rw_block<KM>(h, kk, nk, S, part, T, out, blockIdx.x);
 }
 
 __global__ void k_wake_conv_reduce(const phx_wake_hist *h, WakeKicks kk, int n_kicks, int S,
                                    const double *part, int T, double *out) {
-  wake_conv_reduce(h, kk, n_kicks, S, part, T, out);
+  wake_conv_reduce(h, kk, n_kicks, S, part, T, out, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(PHX_BLOCK)
