@@ -27,7 +27,7 @@ def _check(scan, alone, ids):
     for j, b in enumerate(scan.bunches):
         for k in COORDS:
             np.testing.assert_array_equal(getattr(b, k).cpu().numpy(), alone[j][k],
-                                          err_msg="job %d %s" % (ids[j], k))
+                                          err_msg="job %s %s" % (ids[j], k))
 
 
 @pytest.mark.parametrize("mode,pass_grid", [("fused", 0), ("fused", 128), ("streams", 0)])
