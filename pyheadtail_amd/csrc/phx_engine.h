@@ -10,8 +10,17 @@
 
 #define EN_STAT_NV 33
 #define EN_NV (ZB_NV + EN_STAT_NV)
+// the slice histogram: at most EN_HIST_BLOCKS blocks (partial histograms)
+// of EN_HIST_THREADS threads, EN_HIST_PPT particles per thread and chunk
+#ifndef EN_HIST_BLOCKS
 #define EN_HIST_BLOCKS 256
+#endif
+#ifndef EN_HIST_THREADS
 #define EN_HIST_THREADS 1024
+#endif
+#ifndef EN_HIST_PPT
+#define EN_HIST_PPT PHX_PPT
+#endif
 
 namespace phx_en {
 
@@ -101,12 +110,45 @@ struct GroupPtr {
   double *c[6];
 };
 
+// PHX_PASS_NT: the momenta xp, yp, dp (which the slice histogram after the
+// pass does not read) are loaded and stored non-temporally, so that they do
+// not push x, y and z out of the last-level cache before the histogram;
+// with PHX_PASS_NT_ALL every coordinate's (old) value is loaded so.
+#ifndef PHX_PASS_NT
+#define PHX_PASS_NT 1
+#endif
+#ifndef PHX_PASS_NT_ALL
+#define PHX_PASS_NT_ALL 0
+#endif
+__device__ __forceinline__ double ld_mom(const double *p) {
+#if PHX_PASS_NT
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ double ld_pos(const double *p) {
+#if PHX_PASS_NT_ALL
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ void st_mom(double *p, double v) {
+#if PHX_PASS_NT
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
 __device__ __forceinline__ void group_load(const GroupPtr &gp, int lim, P6 (&p)[PHX_PPT]) {
 #pragma unroll
   for (int k = 0; k < PHX_PPT; ++k) {
     const int o = (int)threadIdx.x + k * PHX_BLOCK;
     if (lim == PHX_CHUNK || o < lim)
-      p[k] = P6{gp.c[0][o], gp.c[1][o], gp.c[2][o], gp.c[3][o], gp.c[4][o], gp.c[5][o]};
+      p[k] = P6{ld_pos(gp.c[0] + o), ld_mom(gp.c[1] + o), ld_pos(gp.c[2] + o),
+                ld_mom(gp.c[3] + o), ld_pos(gp.c[4] + o), ld_mom(gp.c[5] + o)};
     else
       p[k] = P6{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   }
@@ -118,8 +160,8 @@ __device__ __forceinline__ void group_store(const GroupPtr &gp, int lim,
   for (int k = 0; k < PHX_PPT; ++k) {
     const int o = (int)threadIdx.x + k * PHX_BLOCK;
     if (lim == PHX_CHUNK || o < lim) {
-      gp.c[0][o] = p[k].x; gp.c[1][o] = p[k].xp; gp.c[2][o] = p[k].y;
-      gp.c[3][o] = p[k].yp; gp.c[4][o] = p[k].z; gp.c[5][o] = p[k].dp;
+      gp.c[0][o] = p[k].x; st_mom(gp.c[1] + o, p[k].xp); gp.c[2][o] = p[k].y;
+      st_mom(gp.c[3] + o, p[k].yp); gp.c[4][o] = p[k].z; st_mom(gp.c[5] + o, p[k].dp);
     }
   }
 }
@@ -339,7 +381,7 @@ __device__ __forceinline__ void hist_body(const EngineArgs &a, int bid, int nblk
   const phx_engine &e = a.e;
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
   const int S = e.n_slices;
-  constexpr int CH = EN_HIST_THREADS * PHX_PPT;
+  constexpr int CH = EN_HIST_THREADS * EN_HIST_PPT;
   const int64_t nc = (e.n + CH - 1) / CH;
   const int64_t n = e.n;
   // Each chunk loads the next one before it is binned.  No particle load is
@@ -347,12 +389,12 @@ __device__ __forceinline__ void hist_body(const EngineArgs &a, int bid, int nblk
   // of the fold of the cuts, some blocks' folds came out wrong on MI355X
   // (about 1 in 25 block folds on a new engine's first turns; none with
   // the loads issued after the last barrier).
-  double zz[PHX_PPT], xx[PHX_PPT], yy[PHX_PPT];
-  double zn[PHX_PPT], xn[PHX_PPT], yn[PHX_PPT];
-  auto load_chunk = [&](int64_t c, double (&z)[PHX_PPT], double (&x)[PHX_PPT],
-                        double (&y)[PHX_PPT]) {
+  double zz[EN_HIST_PPT], xx[EN_HIST_PPT], yy[EN_HIST_PPT];
+  double zn[EN_HIST_PPT], xn[EN_HIST_PPT], yn[EN_HIST_PPT];
+  auto load_chunk = [&](int64_t c, double (&z)[EN_HIST_PPT], double (&x)[EN_HIST_PPT],
+                        double (&y)[EN_HIST_PPT]) {
 #pragma unroll
-    for (int k = 0; k < PHX_PPT; ++k) {
+    for (int k = 0; k < EN_HIST_PPT; ++k) {
       const int64_t j = c * CH + threadIdx.x + (int64_t)k * EN_HIST_THREADS;
       if (c < nc && j < n) {
         z[k] = e.coords[4][j];
@@ -381,7 +423,7 @@ __device__ __forceinline__ void hist_body(const EngineArgs &a, int bid, int nblk
     load_chunk(c + nblk, zn, xn, yn);
     const int64_t base = c * CH + threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < PHX_PPT; ++k) {
+    for (int k = 0; k < EN_HIST_PPT; ++k) {
       const int64_t j = base + (int64_t)k * EN_HIST_THREADS;
       if (j < n) {
         const double f = slice_floor_r(zz[k], tail, width, inv_width);
@@ -394,7 +436,7 @@ __device__ __forceinline__ void hist_body(const EngineArgs &a, int bid, int nblk
       }
     }
 #pragma unroll
-    for (int k = 0; k < PHX_PPT; ++k) {
+    for (int k = 0; k < EN_HIST_PPT; ++k) {
       zz[k] = zn[k];
       xx[k] = xn[k];
       yy[k] = yn[k];
