@@ -504,6 +504,15 @@ typedef struct {
     int32_t n_kicks;
     int32_t pad;
     phx_wake_kick kick[PHX_MAX_KICKS];
+    /* space charge ending this stage (turns split into stages at their
+       space-charge kicks, see phx_engine.sc_stages): has_sc = 1 when a
+       TransverseGaussianSpaceCharge kick follows this stage's elements;
+       sc_slicer its slicing, sc its parameters.  The kick itself is applied
+       at the start of the next stage's particle pass. */
+    int32_t has_sc;
+    int32_t pad_sc;
+    phx_slicer sc_slicer;
+    phx_spacecharge sc;
 } phx_turn_program;
 
 /* Per-turn bunch statistics recorded at the end of each turn. */
@@ -522,8 +531,11 @@ enum {
     PHX_SHAPE_GENERIC = 0,
     PHX_SHAPE_RF1_T = 1, PHX_SHAPE_RF2_T = 2, PHX_SHAPE_T_RF1 = 3, PHX_SHAPE_T_RF2 = 4,
     PHX_SHAPE_LIN_T = 5, PHX_SHAPE_T_LIN = 6, PHX_SHAPE_T = 7,
-    PHX_SHAPE_COUNT = 8
+    PHX_SHAPE_NONE = 8,       /* no elements (a space-charge stage of the kick alone) */
+    PHX_SHAPE_COUNT = 9
 };
+/* Space-charge kicks per turn of the fused engine. */
+#define PHX_MAX_SC_STAGES 16
 /* Wake kick sets with a compiled kick step (other sets: PHX_KICKS_GENERIC):
    DXDY is [dipole x (xp += k), dipole y (yp += k)], as a round chamber's
    resonator or resistive wall gives; PLATES is [dipole x, quadrupole x
@@ -565,7 +577,16 @@ typedef struct {
     int32_t det_shape;                  /* PHX_DETS_*: the detuners of every
                                            program's transverse segments
                                            (GENERIC is always valid) */
-    int32_t reserved;                   /* 0 */
+    int32_t sc_stages;                  /* space-charge kicks per turn (0: none).
+                                           With m > 0 every turn is m + 1
+                                           stages, each a program of its own:
+                                           turn_program[t] + k is stage k of
+                                           turn t (programs of stages 0..m-1
+                                           have has_sc = 1, stage m's 0);
+                                           program_shape is then stage 0's
+                                           element order, sc_stage_shape[k]
+                                           stage k's, det_shape that of every
+                                           stage's transverse segments */
     const phx_turn_program *programs;   /* [n_programs] */
     const int32_t *turn_program;        /* [n_turns_total] -> program index */
     double *stats;                      /* raw moment sums per turn, see below */
@@ -578,6 +599,14 @@ typedef struct {
     int32_t *counts;                    /* [n_slices] of the last slicing */
     double *slice_means;                /* [2][n_slices] mean_x, mean_y */
     void *ws;                           /* phx_engine_ws_bytes, zeroed once */
+    int32_t sc_slices;                  /* most slices of any space-charge
+                                           slicing (0 without space charge) */
+    int32_t pad_sc;
+    void *sc_ws;                        /* phx_engine_sc_ws_bytes (space charge
+                                           only), set up by phx_engine_reset */
+    int32_t sc_stage_shape[PHX_MAX_SC_STAGES + 1];  /* PHX_SHAPE_* of stages
+                                           1..sc_stages (entry 0 unused) */
+    int32_t pad_sc2;
 } phx_engine;
 
 /* stats row r (state after r turns, r = 0 .. turns run) holds 33 FP64 sums
@@ -588,6 +617,17 @@ typedef struct {
 /* Workspace bytes of the fused engine. */
 int64_t phx_engine_ws_bytes(int64_t n, int32_t n_slices, int32_t n_kicks,
                             int32_t max_turns);
+/* Space-charge workspace bytes of the fused engine: the space-charge cuts,
+   the per-slice field parameters and the partial histograms of the
+   space-charge slicing (sc_slices slices).  The space-charge stages of a
+   turn (TransverseGaussianSpaceCharge.track, spacecharge/spacecharge.py:
+   202-224, fused): stage k's particle pass applies the kick that ends stage
+   k - 1, then the stage's elements; after it a slice histogram (counts and
+   exact fixed-point sums of x, y, x^2, y^2 per slice on the cuts of the
+   current z) and a merge (slice means, standard deviations
+   sqrt((sum u^2 - (sum u)^2 / n) / (n - 1)) for n > 1, else 0, and the line
+   density) give the field of the next kick. */
+int64_t phx_engine_sc_ws_bytes(int64_t n, int32_t sc_slices);
 /* Device-side reset of the engine state (turn counter, pending flag,
    workspace accumulators, wake history).  No memset nodes. */
 int phx_engine_reset(const phx_engine *e, phx_stream_t stream);

@@ -23,6 +23,7 @@ MAX_LONG_OPS = 10
 MAX_KICKS = 16
 MAX_TERMS = 8
 MAX_PROGRAM = 16
+MAX_SC_STAGES = 16       # space-charge kicks per turn of the fused engine
 STANDALONE_GRID = 768
 MAX_GRID = 2048
 
@@ -42,7 +43,7 @@ MOM_N, MOM_X, MOM_Y = 0, 1, 2
 PLANE_XP, PLANE_YP, PLANE_DP = 0, 1, 2
 EL_TSEG, EL_LONG, EL_LINMAP = 1, 2, 3
 (SHAPE_GENERIC, SHAPE_RF1_T, SHAPE_RF2_T, SHAPE_T_RF1, SHAPE_T_RF2, SHAPE_LIN_T, SHAPE_T_LIN,
- SHAPE_T) = range(8)
+ SHAPE_T, SHAPE_NONE) = range(9)
 KICKS_GENERIC, KICKS_DXDY, KICKS_PLATES = 0, 1, 2
 DETS_GENERIC, DETS_NONE, DETS_C1, DETS_C1_AMP = 0, 1, 2, 3
 CONV_GENERAL, CONV_RW_DOT, CONV_RW_MFMA = 1, 2, 4
@@ -168,7 +169,8 @@ class TurnProgram(C.Structure):
                 ("tseg", TSeg * 4), ("lseq", LongSeq * 2), ("lin", LinMap * 2),
                 ("slicer", Slicer), ("beta", C.c_double), ("beta_c", C.c_double),
                 ("age_increment", C.c_double), ("n_kicks", C.c_int32), ("pad", C.c_int32),
-                ("kick", WakeKick * MAX_KICKS)]
+                ("kick", WakeKick * MAX_KICKS), ("has_sc", C.c_int32),
+                ("pad_sc", C.c_int32), ("sc_slicer", Slicer), ("sc", SpaceCharge)]
 
 
 class Engine(C.Structure):
@@ -177,12 +179,14 @@ class Engine(C.Structure):
                 ("record_stats", C.c_int32), ("program_shape", C.c_int32),
                 ("kick_shape", C.c_int32), ("pass_grid", C.c_int32),
                 ("conv_kinds", C.c_int32), ("det_shape", C.c_int32),
-                ("reserved", C.c_int32), ("programs", C.c_void_p),
+                ("sc_stages", C.c_int32), ("programs", C.c_void_p),
                 ("turn_program", C.c_void_p), ("stats", C.c_void_p),
                 ("turn_counter", C.c_void_p), ("pending", C.c_void_p),
                 ("state", C.c_void_p), ("hist", C.c_void_p), ("z_bins", C.c_void_p),
                 ("kicks", C.c_void_p), ("counts", C.c_void_p),
-                ("slice_means", C.c_void_p), ("ws", C.c_void_p)]
+                ("slice_means", C.c_void_p), ("ws", C.c_void_p),
+                ("sc_slices", C.c_int32), ("pad_sc", C.c_int32), ("sc_ws", C.c_void_p),
+                ("sc_stage_shape", C.c_int32 * (MAX_SC_STAGES + 1)), ("pad_sc2", C.c_int32)]
 
 
 RECORDS = {
@@ -226,6 +230,7 @@ SIGNATURES = {
     "phx_wake_convolve": (C.c_int, [P, P, I32, I32, I32, P, P, P]),
     "phx_wake_apply": (C.c_int, [P, I64, P, P, P, I32, I32, P, P]),
     "phx_engine_ws_bytes": (I64, [I64, I32, I32, I32]),
+    "phx_engine_sc_ws_bytes": (I64, [I64, I32]),
     "phx_engine_reset": (C.c_int, [P, P]),
     "phx_engine_turns": (C.c_int, [P, I32, P]),
     "phx_engine_flush": (C.c_int, [P, P]),

@@ -255,6 +255,11 @@ extern "C" int phx_batch_create(const phx_engine *engines, int32_t n, void *desc
       delete B;
       return phx::fail(PHX_E_INVALID, "phx_batch_create: engine %d has no wake field", i);
     }
+    if (e.sc_stages != 0) {
+      delete B;
+      return phx::fail(PHX_E_INVALID,
+                       "phx_batch_create: engine %d has space-charge stages (not batched)", i);
+    }
     if (i == 0) {
       B->rec = e.record_stats;
       B->shape = e.program_shape;

@@ -368,6 +368,7 @@ __device__ __forceinline__ void apply_shape_n(P6 (&p)[N], const phx_turn_program
     apply_linmap_n<N>(p, g.lin[0]);
   } else if constexpr (SHAPE == PHX_SHAPE_T) {
     apply_tseg_n<N, DS>(p, g.tseg[0]);
+  } else if constexpr (SHAPE == PHX_SHAPE_NONE) {
   } else {
     apply_program_n<N>(p, g);
   }

@@ -5,6 +5,7 @@
 #pragma once
 
 #include "phx_device.h"
+#include "phx_sc.h"
 #include "phx_slicing.h"
 #include "phx_wake.h"
 
@@ -72,6 +73,80 @@ struct EngineArgs {
   int grid;  // blocks of the particle pass (its partials: [slot][grid])
   int gmax;  // kicks one convolution block serves (LDS permitting)
 };
+
+// ---------------------------------------------------------------------------
+// Space-charge stages.  Their workspace (sc_ws): the Faddeeva coefficients,
+// the cuts of the last space-charge slicing, the field parameters of every
+// slice (one 64-byte row each) and the partial histograms of counts and
+// sums of x, y, x^2, y^2 (laid out for the slicing's own slice count).
+// ---------------------------------------------------------------------------
+#define SC_TERMS 4
+
+struct ScSlice {
+  double mx, my, sx, sy, nl, pad[3];
+};
+
+struct ScWs {
+  WofzCoeffs *wc;
+  phx_slice_state *state;
+  ScSlice *slices;
+  char *acc;  // acc_parts_layout(acc, hist_blocks(n), S, SC_TERMS)
+};
+
+__host__ __device__ inline size_t sc_r256(size_t b) { return (b + 255) / 256 * 256; }
+
+__host__ __device__ inline ScWs sc_ws(void *base, int S) {
+  ScWs w;
+  char *p = reinterpret_cast<char *>(base);
+  w.wc = reinterpret_cast<WofzCoeffs *>(p);
+  p += sc_r256(sizeof(WofzCoeffs));
+  w.state = reinterpret_cast<phx_slice_state *>(p);
+  p += sc_r256(sizeof(phx_slice_state));
+  w.slices = reinterpret_cast<ScSlice *>(p);
+  p += sc_r256((size_t)(S > 0 ? S : 1) * sizeof(ScSlice));
+  w.acc = p;
+  return w;
+}
+
+inline size_t sc_ws_bytes(int64_t n, int S) {
+  const int Se = S > 0 ? S : 1;
+  return sc_r256(sizeof(WofzCoeffs)) + sc_r256(sizeof(phx_slice_state)) +
+         sc_r256((size_t)Se * sizeof(ScSlice)) + sc_r256(acc_parts_bytes(hist_blocks(n), Se, SC_TERMS));
+}
+
+// What a space-charge kick of a pass reads: the cuts of the slicing after
+// the previous stage, its slices' field parameters, the coefficients.
+struct ScCtx {
+  const ScSlice *sl;
+  const WofzCoeffs *wc;
+  double tail, width, inv_width;
+  int S;
+};
+
+template <int N>
+__device__ __forceinline__ void sc_kick_n(P6 (&p)[N], const ScCtx &c, const phx_spacecharge &sp) {
+  double dx[N], dy[N], sx[N], sy[N], nl[N], ex[N], ey[N];
+  bool in[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const double f = slice_floor_r(p[k].z, c.tail, c.width, c.inv_width);
+    in[k] = f >= 0.0 && f < (double)c.S;
+    const ScSlice &q = c.sl[in[k] ? (int)f : 0];
+    dx[k] = p[k].x - q.mx;
+    dy[k] = p[k].y - q.my;
+    sx[k] = q.sx;
+    sy[k] = q.sy;
+    nl[k] = q.nl;
+  }
+  sc_field_n<N>(*c.wc, sp, dx, dy, sx, sy, ex, ey);
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    if (in[k]) {
+      p[k].xp = p[k].xp + (ex[k] * nl[k]) * sp.prefactor;
+      p[k].yp = p[k].yp + (ey[k] * nl[k]) * sp.prefactor;
+    }
+  }
+}
 
 __device__ __forceinline__ P6 load_p(const phx_engine &e, int64_t j) {
   P6 p;
@@ -210,12 +285,16 @@ struct PassLayout {
 #ifndef PHX_PASS_LAUNDER
 #define PHX_PASS_LAUNDER 0
 #endif
+// Space-charge engines (sc_stages > 0) run this pass for stage 0 of a
+// turn (stage: the offset of the stage's program from the turn's first;
+// the later stages run sc_pass_body).  The pending wake kick and the
+// statistics belong to stage 0.
 template <int MODE, bool REC, bool WAKE, int SHAPE, int KS, int DS>
 __device__ __forceinline__ void pass_body(
     const EngineArgs &a, double *__restrict__ x, double *__restrict__ xp,
     double *__restrict__ y, double *__restrict__ yp, double *__restrict__ z,
     double *__restrict__ dp, const phx_turn_program *__restrict__ programs,
-    const double *__restrict__ kicks, int bid, int nblk) {
+    const double *__restrict__ kicks, int bid, int nblk, int stage = 0) {
   using L = PassLayout<REC, WAKE>;
   const phx_engine &e = a.e;
   extern __shared__ double pass_lds[];
@@ -223,8 +302,8 @@ __device__ __forceinline__ void pass_body(
   const int64_t n = e.n;
   const int turn = *e.turn_counter;
   const int prog_turn = MODE == 0 ? turn : (turn > 0 ? turn - 1 : 0);
-  const phx_turn_program &g = programs[e.turn_program[prog_turn]];
-  const int pend = *e.pending;
+  const phx_turn_program &g = programs[e.turn_program[prog_turn] + stage];
+  const int pend = stage == 0 ? *e.pending : 0;
   const double tail = e.state->tail, width = e.state->width;
   const double inv_width = 1.0 / width;
   const int S = e.n_slices;
@@ -320,6 +399,81 @@ __device__ __forceinline__ void pass_body(
   }
   const double r = block_reduce_slots<L::NV, L::MINM, L::MAXM>(v, pass_lds);
   if (threadIdx.x < L::NV) w.partials[(size_t)threadIdx.x * nblk + bid] = r;
+}
+
+// The pass of space-charge stage `stage` > 0: the space-charge kick that
+// ended the previous stage (the slicing after it is in the space-charge
+// workspace), then the stage's elements, the coordinates written back and
+// the z block of the result (11 partials per block, as pass_body without
+// statistics).  The field evaluation needs many registers, so a thread
+// holds one particle at a time, taking the particles of pass_body's groups
+// in pass_body's order: the z-block sums, and so the cuts, are those
+// pass_body would give.
+template <int SHAPE, int DS>
+__device__ __forceinline__ void sc_pass_body(
+    const EngineArgs &a, double *__restrict__ x, double *__restrict__ xp,
+    double *__restrict__ y, double *__restrict__ yp, double *__restrict__ z,
+    double *__restrict__ dp, const phx_turn_program *__restrict__ programs, int bid, int nblk,
+    int stage) {
+  const phx_engine &e = a.e;
+  extern __shared__ double pass_lds[];
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
+  const int64_t n = e.n;
+  const int turn = *e.turn_counter;
+  const int first = e.turn_program[turn];
+  const phx_turn_program &g = programs[first + stage];
+  const phx_spacecharge &sp = programs[first + stage - 1].sc;
+  const ScWs sw = sc_ws(e.sc_ws, e.sc_slices);
+  ScCtx scc;
+  scc.sl = sw.slices;
+  scc.wc = sw.wc;
+  scc.tail = sw.state->tail;
+  scc.width = sw.state->width;
+  scc.inv_width = 1.0 / scc.width;
+  scc.S = sw.state->n_slices;
+  // particle 0's z after this stage's elements: the shift of the z sums
+  double zshift;
+  {
+    P6 q = P6{w.p0[0], w.p0[1], w.p0[2], w.p0[3], w.p0[4], w.p0[5]};
+    apply_shape_longitudinal<SHAPE>(q, g);
+    zshift = q.z;
+  }
+  // transverse-only stages leave z and dp as they are
+  const bool store_long = SHAPE == PHX_SHAPE_T || SHAPE == PHX_SHAPE_NONE
+                              ? false
+                              : (SHAPE != PHX_SHAPE_GENERIC || g.n_long > 0 || g.n_lin > 0);
+  double v[ZB_NV];
+#pragma unroll
+  for (int i = 0; i < ZB_NV; ++i) v[i] = red_ident<ZB_MINM, ZB_MAXM>(i);
+  const int64_t ng = phx_n_groups(n);
+  for (int64_t gi = bid; gi < ng; gi += nblk) {
+    const int64_t b = gi * PHX_CHUNK;
+    const int64_t rest = n - b;
+    const int lim = rest >= PHX_CHUNK ? PHX_CHUNK : (int)rest;
+#pragma unroll 1
+    for (int k = 0; k < PHX_PPT; ++k) {
+      const int o = (int)threadIdx.x + k * PHX_BLOCK;
+      if (o < lim) {
+        const int64_t j = b + o;
+        P6 p[1];
+        p[0] = P6{ld_pos(x + j), ld_mom(xp + j), ld_pos(y + j), ld_mom(yp + j), ld_pos(z + j),
+                  ld_mom(dp + j)};
+        sc_kick_n<1>(p, scc, sp);
+        apply_shape_n<SHAPE, 1, DS>(p, g);
+        x[j] = p[0].x;
+        st_mom(xp + j, p[0].xp);
+        y[j] = p[0].y;
+        st_mom(yp + j, p[0].yp);
+        if (store_long) {
+          z[j] = p[0].z;
+          st_mom(dp + j, p[0].dp);
+        }
+        zblock_accumulate<0x05u>(v, p[0], zshift);
+      }
+    }
+  }
+  const double r = block_reduce_slots<ZB_NV, ZB_MINM, ZB_MAXM>(v, pass_lds);
+  if (threadIdx.x < ZB_NV) w.partials[(size_t)threadIdx.x * nblk + bid] = r;
 }
 
 // The cuts of this turn from the pass partials (one block): fold the z
@@ -484,6 +638,179 @@ __device__ __forceinline__ void merge_body(const EngineArgs &a, int bid) {
   mm[s] = cnt;
   mm[S + s] = cnt * mx;
   mm[2 * S + s] = cnt * my;
+}
+
+// Space-charge slicing of the state after stage `stage` (the slicing that
+// TransverseGaussianSpaceCharge.track asks for with mean_x, mean_y, sigma_x,
+// sigma_y, spacecharge/spacecharge.py:207-208): every block folds the
+// stage pass's z block into the cuts of the stage program's sc_slicer
+// (block 0 publishes them and records particle 0 for the next pass), then
+// keeps an LDS histogram of counts and fixed-point sums of x, y, x^2 and
+// y^2, stored as its partial.  The scales come from max |x| and max |y| of
+// the same state, so every term fits.
+__device__ __forceinline__ void sc_hist_body(const EngineArgs &a, int stage, int bid, int nblk) {
+  extern __shared__ unsigned long long acc_lds[];
+  __shared__ double res[ZB_NV];
+  __shared__ phx_slice_state sh_st;
+  const phx_engine &e = a.e;
+  EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
+  const ScWs sw = sc_ws(e.sc_ws, e.sc_slices);
+  const int turn = *e.turn_counter;
+  const phx_turn_program &g = e.programs[e.turn_program[turn] + stage];
+  const int S = g.sc_slicer.n_slices;
+  const int64_t n = e.n;
+  const AccParts P = acc_parts_layout(sw.acc, hist_blocks(n), S, SC_TERMS);
+  fold_partials<ZB_MINM, ZB_MAXM>(w.partials, a.grid, 0, ZB_NV, res);
+  if (threadIdx.x == 0) {
+    phx_slice_state st;
+    compute_cuts(g.sc_slicer, n, res[0], res[1], res[2], res[3], res[4], st);
+    for (int i = 0; i < 6; ++i) st.absmax[i] = res[5 + i];
+    st.turn = turn;
+    st.pad = 0;
+    sh_st = st;
+    if (bid == 0) *sw.state = st;
+  }
+  if (bid == 0 && threadIdx.x < 6) w.p0[threadIdx.x] = e.coords[threadIdx.x][0];
+  AccLds A = acc_lds_layout(acc_lds, S, SC_TERMS);
+  acc_lds_zero(A, S, SC_TERMS);  // ends with a barrier: the shared cuts are visible
+  const double tail = sh_st.tail, width = sh_st.width;
+  const double inv_width = 1.0 / width;
+  const double bx = sh_st.absmax[0], by = sh_st.absmax[2];
+  const int ex = fx_exponent(bx), ey = fx_exponent(by);
+  const int exx = fx_exponent(bx * bx), eyy = fx_exponent(by * by);
+  constexpr int CH = EN_HIST_THREADS * EN_HIST_PPT;
+  const int64_t nc = (n + CH - 1) / CH;
+  double zz[EN_HIST_PPT], xx[EN_HIST_PPT], yy[EN_HIST_PPT];
+  double zn[EN_HIST_PPT], xn[EN_HIST_PPT], yn[EN_HIST_PPT];
+  auto load_chunk = [&](int64_t c, double (&zv)[EN_HIST_PPT], double (&xv)[EN_HIST_PPT],
+                        double (&yv)[EN_HIST_PPT]) {
+#pragma unroll
+    for (int k = 0; k < EN_HIST_PPT; ++k) {
+      const int64_t j = c * CH + threadIdx.x + (int64_t)k * EN_HIST_THREADS;
+      if (c < nc && j < n) {
+        zv[k] = e.coords[4][j];
+        xv[k] = e.coords[0][j];
+        yv[k] = e.coords[2][j];
+      } else {
+        zv[k] = 0.0;
+        xv[k] = 0.0;
+        yv[k] = 0.0;
+      }
+    }
+  };
+  load_chunk(bid, zz, xx, yy);
+  for (int64_t c = bid; c < nc; c += nblk) {
+    load_chunk(c + nblk, zn, xn, yn);
+    const int64_t base = c * CH + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < EN_HIST_PPT; ++k) {
+      const int64_t j = base + (int64_t)k * EN_HIST_THREADS;
+      if (j < n) {
+        const double f = slice_floor_r(zz[k], tail, width, inv_width);
+        if (f >= 0.0 && f < (double)S) {
+          const int s = (int)f;
+          atomicAdd(&A.counts[s], 1u);
+          acc_add_value(A, S, 0, s, xx[k], ex, P.fallback);
+          acc_add_value(A, S, 1, s, yy[k], ey, P.fallback);
+          acc_add_value(A, S, 2, s, xx[k] * xx[k], exx, P.fallback);
+          acc_add_value(A, S, 3, s, yy[k] * yy[k], eyy, P.fallback);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < EN_HIST_PPT; ++k) {
+      zz[k] = zn[k];
+      xx[k] = xn[k];
+      yy[k] = yn[k];
+    }
+  }
+  acc_lds_store(A, S, SC_TERMS, P, bid);
+}
+
+// A fixed-point total as an unevaluated double-double hi + lo (exact):
+// the integer h*2^32 + l, h = hi_sum + (lo_sum >> 32), l = lo_sum mod 2^32,
+// times 2^-e.
+__device__ __forceinline__ void fx_value_dd(unsigned long long hi_sum, unsigned long long lo_sum,
+                                            int e, double &hi, double &lo) {
+  const long long h = (long long)hi_sum + (long long)(lo_sum >> 32);
+  const double a = (double)h;
+  const long long r = h - (long long)a;  // |r| <= 2^10
+  const double A = ldexp(a, 32);
+  const double B = (double)(r * 4294967296LL + (long long)(lo_sum & 0xffffffffULL));
+  const double s = A + B;
+  const double bb = s - A;
+  const double t = (A - (s - bb)) + (B - bb);
+  hi = ldexp(s, -e);
+  lo = ldexp(t, -e);
+}
+
+// (q - x^2/n) / (n - 1) from double-double totals x = sum u, q = sum u^2:
+// the variance with the single rounding of each u^2 and of the result.
+__device__ __forceinline__ double var_from_sums(double xh, double xl, double qh, double ql,
+                                                double n) {
+  // x^2 as double-double
+  const double ph = xh * xh;
+  const double pl = fma(xh, xh, -ph) + 2.0 * xh * xl;
+  // (ph + pl) / n
+  const double q1 = ph / n;
+  const double r = fma(-q1, n, ph);
+  const double q2 = (r + pl) / n;
+  // (qh + ql) - (q1 + q2)
+  const double s = qh - q1;
+  const double bb = s - qh;
+  const double err = (qh - (s - bb)) + (-q1 - bb);
+  const double d = s + ((err + ql) - q2);
+  const double v = d / (n - 1.0);
+  return v > 0.0 ? v : 0.0;
+}
+
+// Merge of the space-charge partial histograms: per slice the count, means
+// (mean_per_slice: sum / n, 0 for empty slices), standard deviations
+// (std_per_slice: 0 for n <= 1) and the line density, into the field rows
+// the next stage's kick reads.
+template <int SL>
+__device__ __forceinline__ void sc_merge_body(const EngineArgs &a, int stage, int bid) {
+  const phx_engine &e = a.e;
+  const ScWs sw = sc_ws(e.sc_ws, e.sc_slices);
+  const int turn = *e.turn_counter;
+  const phx_turn_program &g = e.programs[e.turn_program[turn] + stage];
+  const int S = g.sc_slicer.n_slices;
+  const AccParts P = acc_parts_layout(sw.acc, hist_blocks(e.n), S, SC_TERMS);
+  SliceTotals tot;
+  if (!acc_merge<SC_TERMS, SL>(P, S, tot, bid)) return;
+  const int s = bid * SL + threadIdx.x % SL;
+  const phx_slice_state &st = *sw.state;
+  const double tail = st.tail, width = st.width;
+  const double bx = st.absmax[0], by = st.absmax[2];
+  const int ex = fx_exponent(bx), ey = fx_exponent(by);
+  const int exx = fx_exponent(bx * bx), eyy = fx_exponent(by * by);
+  const double cnt = (double)tot.count;
+  const double fx = P.fallback[s], fy = P.fallback[S + s];
+  const double fxx = P.fallback[2 * S + s], fyy = P.fallback[3 * S + s];
+  P.fallback[s] = 0.0;
+  P.fallback[S + s] = 0.0;
+  P.fallback[2 * S + s] = 0.0;
+  P.fallback[3 * S + s] = 0.0;
+  const double sx = fx_value(tot.hi[0], tot.lo[0], ex) + fx;
+  const double sy = fx_value(tot.hi[1], tot.lo[1], ey) + fy;
+  ScSlice row;
+  row.mx = tot.count ? sx / cnt : 0.0;
+  row.my = tot.count ? sy / cnt : 0.0;
+  row.sx = 0.0;
+  row.sy = 0.0;
+  if (tot.count > 1) {
+    double xh, xl, yh, yl, qxh, qxl, qyh, qyl;
+    fx_value_dd(tot.hi[0], tot.lo[0], ex, xh, xl);
+    fx_value_dd(tot.hi[1], tot.lo[1], ey, yh, yl);
+    fx_value_dd(tot.hi[2], tot.lo[2], exx, qxh, qxl);
+    fx_value_dd(tot.hi[3], tot.lo[3], eyy, qyh, qyl);
+    row.sx = sqrt(var_from_sums(xh, xl + fx, qxh, qxl + fxx, cnt));
+    row.sy = sqrt(var_from_sums(yh, yl + fy, qyh, qyl + fyy, cnt));
+  }
+  row.nl = sc_line_density((int32_t)tot.count, g.sc.charge_per_mp, arange_value(tail, width, s),
+                           arange_value(tail, width, s + 1));
+  row.pad[0] = row.pad[1] = row.pad[2] = 0.0;
+  sw.slices[s] = row;
 }
 
 // Slices per merge block: 4 up to 512 slices, 8 above (5.2 / 6.3 us at 500

@@ -33,16 +33,36 @@ __global__ void __launch_bounds__(PHX_BLOCK, PHX_PASS_MIN_BLOCKS)
     k_engine_pass(EngineArgs a, double *__restrict__ x, double *__restrict__ xp,
                   double *__restrict__ y, double *__restrict__ yp, double *__restrict__ z,
                   double *__restrict__ dp, const phx_turn_program *__restrict__ programs,
-                  const double *__restrict__ kicks) {
+                  const double *__restrict__ kicks, int stage) {
   pass_body<MODE, REC, WAKE, SHAPE, KS, DS>(a, x, xp, y, yp, z, dp, programs, kicks, blockIdx.x,
-                                            gridDim.x);
+                                            gridDim.x, stage);
+}
+
+// PHX_SCPASS_WAVES > 0: ask the compiler for that many waves per SIMD (it
+// then keeps the registers within the budget, spilling if it must).
+#ifndef PHX_SCPASS_WAVES
+#define PHX_SCPASS_WAVES 0
+#endif
+#if PHX_SCPASS_WAVES > 0
+#define PHX_SCPASS_ATTR __attribute__((amdgpu_waves_per_eu(PHX_SCPASS_WAVES)))
+#else
+#define PHX_SCPASS_ATTR
+#endif
+template <int SHAPE, int DS>
+__global__ void __launch_bounds__(PHX_BLOCK) PHX_SCPASS_ATTR
+    k_engine_scpass(EngineArgs a, double *__restrict__ x, double *__restrict__ xp,
+                    double *__restrict__ y, double *__restrict__ yp, double *__restrict__ z,
+                    double *__restrict__ dp, const phx_turn_program *__restrict__ programs,
+                    int stage) {
+  sc_pass_body<SHAPE, DS>(a, x, xp, y, yp, z, dp, programs, blockIdx.x, gridDim.x, stage);
 }
 
 // One block: fold the chunk partials, then the per-turn bookkeeping (for
-// maps without a wake field, and for the flush).
-template <int MODE, bool REC>
+// maps without a wake field, and for the flush).  ZBL: the partials have
+// the z-block layout (the passes of space-charge engines).
+template <int MODE, bool REC, bool ZBL = false>
 __global__ void __launch_bounds__(PHX_BLOCK) k_engine_fin(EngineArgs a) {
-  using L = PassLayout<REC, false>;
+  using L = PassLayout<REC, ZBL>;
   const phx_engine &e = a.e;
   __shared__ double res[L::NV];
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
@@ -120,6 +140,28 @@ __global__ void __launch_bounds__(RW_THREADS) k_engine_conv_rw(EngineArgs a) {
 
 __global__ void k_engine_conv_reduce(EngineArgs a) { conv_reduce_body(a, blockIdx.x); }
 
+__global__ void __launch_bounds__(EN_HIST_THREADS) k_engine_schist(EngineArgs a, int stage) {
+  sc_hist_body(a, stage, blockIdx.x, gridDim.x);
+}
+
+template <int SL>
+__global__ void __launch_bounds__(PHX_BLOCK) k_engine_smerge(EngineArgs a, int stage) {
+  sc_merge_body<SL>(a, stage, blockIdx.x);
+}
+
+// The space-charge workspace at reset: zeroed, then the Faddeeva
+// coefficients.
+__global__ void k_engine_sc_init(EngineArgs a, WofzCoeffs c, size_t clear_bytes) {
+  const phx_engine &e = a.e;
+  unsigned long long *p = reinterpret_cast<unsigned long long *>(e.sc_ws);
+  const size_t nw = clear_bytes / 8;
+  const size_t first = sc_r256(sizeof(WofzCoeffs)) / 8;
+  for (size_t i = first + blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nw;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0ULL;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *reinterpret_cast<WofzCoeffs *>(e.sc_ws) = c;
+}
+
 __global__ void k_engine_reset(EngineArgs a, size_t clear_bytes) {
   const phx_engine &e = a.e;
   unsigned long long *p = reinterpret_cast<unsigned long long *>(e.ws);
@@ -170,6 +212,17 @@ int check_engine(const phx_engine *e, int *n_kicks, int *T) {
     return phx::fail(PHX_E_INVALID, "phx_engine: det_shape %d", e->det_shape);
   if (e->n_slices > 0 && (!e->hist || !e->z_bins || !e->kicks || !e->counts || !e->slice_means))
     return phx::fail(PHX_E_INVALID, "phx_engine: wake buffers null");
+  if (e->sc_stages < 0 || e->sc_stages > PHX_MAX_SC_STAGES)
+    return phx::fail(PHX_E_LIMIT, "phx_engine: sc_stages %d not in 0..%d", e->sc_stages,
+                     PHX_MAX_SC_STAGES);
+  if (e->sc_stages > 0) {
+    if (!e->sc_ws || e->sc_slices < 1)
+      return phx::fail(PHX_E_INVALID, "phx_engine: space-charge stages need sc_ws and sc_slices");
+    for (int k = 1; k <= e->sc_stages; ++k)
+      if (e->sc_stage_shape[k] < 0 || e->sc_stage_shape[k] >= PHX_SHAPE_COUNT)
+        return phx::fail(PHX_E_INVALID, "phx_engine: sc_stage_shape[%d] = %d", k,
+                         e->sc_stage_shape[k]);
+  }
   *n_kicks = e->n_kicks;
   *T = e->hist_capacity > 0 ? e->hist_capacity : 1;
   return PHX_OK;
@@ -182,12 +235,12 @@ size_t pass_lds_bytes() {
 }
 
 template <int MODE, bool R, bool W, int SH, int KS, int DS>
-void launch_pass_kernel(const EngineArgs &a, hipStream_t s) {
+void launch_pass_kernel(const EngineArgs &a, int stage, hipStream_t s) {
   const phx_engine &e = a.e;
   double *const *c = e.coords;
   k_engine_pass<MODE, R, W, SH, KS, DS>
       <<<dim3((unsigned)a.grid), PHX_BLOCK, pass_lds_bytes<R, W>(), s>>>(
-          a, c[0], c[1], c[2], c[3], c[4], c[5], e.programs, e.kicks);
+          a, c[0], c[1], c[2], c[3], c[4], c[5], e.programs, e.kicks, stage);
 }
 
 // Blocks of this pass that stay resident on one CU (queried once).
@@ -196,9 +249,9 @@ int pass_blocks_per_cu() {
   static int cached = 0;
   if (cached == 0) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_engine_pass<MODE, R, W, SH, KS, DS>,
-                                                     PHX_BLOCK, pass_lds_bytes<R, W>()) !=
-            hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, k_engine_pass<MODE, R, W, SH, KS, DS>, PHX_BLOCK,
+            pass_lds_bytes<R, W>()) != hipSuccess ||
         nb < 1)
       nb = 1;
     cached = nb;
@@ -207,7 +260,7 @@ int pass_blocks_per_cu() {
 }
 
 struct PassKernel {
-  void (*launch)(const EngineArgs &, hipStream_t);
+  void (*launch)(const EngineArgs &, int, hipStream_t);
   int (*blocks_per_cu)();
 };
 
@@ -283,21 +336,128 @@ int pass_grid(const phx_engine &e, const PassKernel &k) {
   return phx_grid_for(e.n, g);
 }
 
+// The space-charge stage pass compiled for a stage's element order and the
+// detuner set (the orders the served rings have; others: the generic pass).
+struct ScPassKernel {
+  void (*kernel)(EngineArgs, double *, double *, double *, double *, double *, double *,
+                 const phx_turn_program *, int);
+  int (*blocks_per_cu)();
+};
+
+template <int SH, int DS>
+int scpass_blocks_per_cu() {
+  static int cached = 0;
+  if (cached == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_engine_scpass<SH, DS>, PHX_BLOCK,
+                                                     BR_LDS(ZB_NV) * sizeof(double)) !=
+            hipSuccess ||
+        nb < 1)
+      nb = 1;
+    cached = nb;
+  }
+  return cached;
+}
+
+template <int SH, int DS>
+ScPassKernel scpass_kernel() {
+  return ScPassKernel{k_engine_scpass<SH, DS>, scpass_blocks_per_cu<SH, DS>};
+}
+
+template <int SH>
+ScPassKernel scpass_dets(int ds) {
+  switch (ds) {
+    case PHX_DETS_NONE: return scpass_kernel<SH, PHX_DETS_NONE>();
+    case PHX_DETS_C1: return scpass_kernel<SH, PHX_DETS_C1>();
+    case PHX_DETS_C1_AMP: return scpass_kernel<SH, PHX_DETS_C1_AMP>();
+    default: return scpass_kernel<SH, PHX_DETS_GENERIC>();
+  }
+}
+
+ScPassKernel pick_scpass(int shape, int ds) {
+  switch (shape) {
+    case PHX_SHAPE_T: return scpass_dets<PHX_SHAPE_T>(ds);
+    case PHX_SHAPE_LIN_T: return scpass_dets<PHX_SHAPE_LIN_T>(ds);
+    case PHX_SHAPE_T_LIN: return scpass_dets<PHX_SHAPE_T_LIN>(ds);
+    case PHX_SHAPE_NONE: return scpass_kernel<PHX_SHAPE_NONE, PHX_DETS_GENERIC>();
+    default: return scpass_kernel<PHX_SHAPE_GENERIC, PHX_DETS_GENERIC>();
+  }
+}
+
+// Resident blocks per CU of the least occupying stage pass of an engine.
+int scpass_blocks_per_cu(const phx_engine &e) {
+  int nb = 1 << 20;
+  for (int k = 1; k <= e.sc_stages; ++k) {
+    const int b = pick_scpass(e.sc_stage_shape[k], e.det_shape).blocks_per_cu();
+    nb = b < nb ? b : nb;
+  }
+  return nb;
+}
+
 EngineArgs make_args(const phx_engine *e, int nk, int T) {
   const bool has_wake = e->n_slices > 0 && nk > 0;
   EngineArgs a{*e, nk, T, 1, 1};
-  a.grid = pass_grid(*e, pick_pass<0>(*e, has_wake));
+  if (e->sc_stages > 0 && e->pass_grid == 0) {
+    // every pass of a turn shares the grid (the partials' layout); space-
+    // charge engines take the one that fills the device with their stage
+    // passes, which carry most of the work
+    int64_t g = (int64_t)scpass_blocks_per_cu(*e) * (int64_t)phx_num_cus();
+    if (g > PHX_MAX_GRID) g = PHX_MAX_GRID;
+    a.grid = phx_grid_for(e->n, g);
+  } else {
+    a.grid = pass_grid(*e, pick_pass<0>(*e, has_wake || e->sc_stages > 0));
+  }
   if (e->n_slices > 0) a.gmax = wake_conv_gmax(e->n_slices, phx_max_lds_bytes());
   return a;
 }
 
+// The space-charge stages of a turn (sc_stages = m): for k = 0 .. m, the
+// pass of stage k (stage 0: the pending wake kick, statistics and the
+// turn's first elements; stage k > 0: the space-charge kick ending stage
+// k-1, then stage k's elements), each but the last followed by the
+// space-charge slicing of its result (histogram and merge).  The z block of
+// the last pass feeds the wake field's cuts, or the finalize without one.
+int launch_sc_stages(const EngineArgs &a, bool has_wake, hipStream_t s) {
+  const phx_engine &e = a.e;
+  const int m = e.sc_stages;
+  const int S = e.sc_slices;
+  const unsigned hb = (unsigned)hist_blocks(e.n);
+  const size_t lds = acc_lds_bytes(S, SC_TERMS);
+  for (int k = 0; k <= m; ++k) {
+    if (k == 0) {
+      pick_pass<0>(e, true).launch(a, 0, s);
+    } else {
+      double *const *c = e.coords;
+      const ScPassKernel kk = pick_scpass(e.sc_stage_shape[k], e.det_shape);
+      hipLaunchKernelGGL(kk.kernel, dim3((unsigned)a.grid), dim3(PHX_BLOCK),
+                         BR_LDS(ZB_NV) * sizeof(double), s, a, c[0], c[1], c[2], c[3], c[4],
+                         c[5], e.programs, k);
+    }
+    PHX_CHECK_LAUNCH("phx_engine: space-charge stage pass");
+    if (k == m) break;
+    k_engine_schist<<<dim3(hb), EN_HIST_THREADS, lds, s>>>(a, k);
+    PHX_CHECK_LAUNCH("phx_engine: space-charge histogram");
+    if (merge_sl(S) == 4)
+      k_engine_smerge<4><<<dim3((S + 3) / 4), PHX_BLOCK, 0, s>>>(a, k);
+    else
+      k_engine_smerge<8><<<dim3((S + 7) / 8), PHX_BLOCK, 0, s>>>(a, k);
+    PHX_CHECK_LAUNCH("phx_engine: space-charge merge");
+  }
+  if (has_wake) return PHX_OK;
+  if (e.record_stats) k_engine_fin<0, true, true><<<1, PHX_BLOCK, 0, s>>>(a);
+  else k_engine_fin<0, false, true><<<1, PHX_BLOCK, 0, s>>>(a);
+  PHX_CHECK_LAUNCH("phx_engine: finalize");
+  return PHX_OK;
+}
+
 template <int MODE>
 int launch_pass(const EngineArgs &args, bool has_wake, hipStream_t s) {
+  if (MODE == 0 && args.e.sc_stages > 0) return launch_sc_stages(args, has_wake, s);
   const bool wake_pass = MODE == 0 && has_wake;
   const PassKernel k = pick_pass<MODE>(args.e, wake_pass);
   EngineArgs a = args;
   if (MODE != 0) a.grid = pass_grid(a.e, k);
-  k.launch(a, s);
+  k.launch(a, 0, s);
   PHX_CHECK_LAUNCH("phx_engine: pass");
   if (wake_pass) return PHX_OK;
   if (a.e.record_stats) k_engine_fin<MODE, true><<<1, PHX_BLOCK, 0, s>>>(a);
@@ -385,6 +545,8 @@ int enqueue_turn(const EngineArgs &a, bool has_wake, hipStream_t s) {
 }
 
 int check_lds(const phx_engine *e) {
+  if (e->sc_stages > 0 && acc_lds_bytes(e->sc_slices, SC_TERMS) > phx_max_lds_bytes())
+    return phx::fail(PHX_E_LIMIT, "phx_engine: sc_slices=%d exceeds LDS", e->sc_slices);
   if (e->n_slices > 0 && (acc_lds_bytes(e->n_slices, 2) > phx_max_lds_bytes() ||
                           wake_conv_lds_bytes(e->n_slices, 1) + WK_STATIC_LDS > phx_max_lds_bytes()))
     return phx::fail(PHX_E_LIMIT, "phx_engine: n_slices=%d exceeds LDS", e->n_slices);
@@ -415,6 +577,11 @@ extern "C" int64_t phx_engine_ws_bytes(int64_t n, int32_t n_slices, int32_t n_ki
   return (int64_t)engine_ws_bytes(n, n_slices, n_kicks, max_turns);
 }
 
+extern "C" int64_t phx_engine_sc_ws_bytes(int64_t n, int32_t sc_slices) {
+  if (n < 1 || sc_slices < 1) return -1;
+  return (int64_t)sc_ws_bytes(n, sc_slices);
+}
+
 extern "C" int phx_engine_reset(const phx_engine *e, phx_stream_t stream) {
   int nk, T;
   int rc = check_engine(e, &nk, &T);
@@ -423,6 +590,11 @@ extern "C" int phx_engine_reset(const phx_engine *e, phx_stream_t stream) {
   const size_t bytes = engine_ws_bytes(e->n, e->n_slices, nk, T);
   k_engine_reset<<<256, 256, 0, phx::as_stream(stream)>>>(a, bytes);
   PHX_CHECK_LAUNCH("phx_engine_reset");
+  if (e->sc_stages > 0) {
+    k_engine_sc_init<<<256, 256, 0, phx::as_stream(stream)>>>(
+        a, phx_wofz_host_coeffs(), sc_ws_bytes(e->n, e->sc_slices));
+    PHX_CHECK_LAUNCH("phx_engine_reset: space charge");
+  }
   return PHX_OK;
 }
 
@@ -529,6 +701,10 @@ extern "C" int phx_engine_turns_noalloc(const phx_engine *e, int32_t n_turns, in
   PHX_CHECK_HIP(hipGetDevice(&dev), "phx_engine_turns_noalloc");
   if (dev < 0 || dev >= PHX_MAX_DEVICES || !g_scratch[dev])
     return phx::fail(PHX_E_NODEVICE, "phx_engine_turns_noalloc: phx_init(%d) not called", dev);
+  if (e->sc_stages != 0)
+    return phx::fail(PHX_E_INVALID,
+                     "phx_engine_turns_noalloc: space-charge stages need their own workspace "
+                     "(phx_engine_turns)");
   phx_engine c = *e;
   c.ws = g_scratch[dev];
   int nk, T;

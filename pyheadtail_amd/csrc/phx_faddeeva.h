@@ -44,28 +44,56 @@ inline WofzCoeffs wofz_coefficients() {
   return c;
 }
 
+// w(x_k + i y_k), y_k >= 0, for N points: real and imaginary parts.  The N
+// evaluations run in lockstep (N independent recurrences for the compiler
+// to interleave); each point's arithmetic is that of N = 1.
+template <int N>
+__device__ __forceinline__ void phx_wofz_n(const WofzCoeffs &c, const double (&x)[N],
+                                           const double (&y)[N], double (&wr)[N],
+                                           double (&wi)[N]) {
+  const double L = c.L;
+  double qr[N], qi[N], Zr[N], Zi[N], pr[N], pi[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    // L - i z = (L + y) - i x,  L + i z = (L - y) + i x
+    const double dr = L + y[k], di = -x[k];
+    const double den = dr * dr + di * di;
+    // q = 1 / (L - i z) = (dr - i di) / den
+    qr[k] = dr / den;
+    qi[k] = -di / den;
+    // Z = (L + i z) q
+    const double nr = L - y[k], ni = x[k];
+    Zr[k] = nr * qr[k] - ni * qi[k];
+    Zi[k] = nr * qi[k] + ni * qr[k];
+    pr[k] = c.a[0];
+    pi[k] = 0.0;
+  }
+#pragma unroll 4
+  for (int n = 1; n < PHX_WOFZ_N; ++n) {
+    const double an = c.a[n];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const double tr = fma(pr[k], Zr[k], fma(-pi[k], Zi[k], an));
+      pi[k] = fma(pr[k], Zi[k], pi[k] * Zr[k]);
+      pr[k] = tr;
+    }
+  }
+  // w = 2 p q^2 + q / sqrt(pi)
+  const double isp = 0.56418958354775628695;  // 1/sqrt(pi)
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const double q2r = qr[k] * qr[k] - qi[k] * qi[k], q2i = 2.0 * qr[k] * qi[k];
+    wr[k] = 2.0 * (pr[k] * q2r - pi[k] * q2i) + isp * qr[k];
+    wi[k] = 2.0 * (pr[k] * q2i + pi[k] * q2r) + isp * qi[k];
+  }
+}
+
 // w(x + i y), y >= 0: real and imaginary parts.
 __device__ __forceinline__ void phx_wofz(const WofzCoeffs &c, double x, double y, double &wr,
                                          double &wi) {
-  const double L = c.L;
-  // L - i z = (L + y) - i x,  L + i z = (L - y) + i x
-  const double dr = L + y, di = -x;
-  const double den = dr * dr + di * di;
-  // q = 1 / (L - i z) = (dr - i di) / den
-  const double qr = dr / den, qi = -di / den;
-  // Z = (L + i z) q
-  const double nr = L - y, ni = x;
-  const double Zr = nr * qr - ni * qi, Zi = nr * qi + ni * qr;
-  double pr = c.a[0], pi = 0.0;
-#pragma unroll 8
-  for (int n = 1; n < PHX_WOFZ_N; ++n) {
-    const double tr = fma(pr, Zr, fma(-pi, Zi, c.a[n]));
-    pi = fma(pr, Zi, pi * Zr);
-    pr = tr;
-  }
-  // w = 2 p q^2 + q / sqrt(pi)
-  const double q2r = qr * qr - qi * qi, q2i = 2.0 * qr * qi;
-  const double isp = 0.56418958354775628695;  // 1/sqrt(pi)
-  wr = 2.0 * (pr * q2r - pi * q2i) + isp * qr;
-  wi = 2.0 * (pr * q2i + pi * q2r) + isp * qi;
+  const double xs[1] = {x}, ys[1] = {y};
+  double a[1], b[1];
+  phx_wofz_n<1>(c, xs, ys, a, b);
+  wr = a[0];
+  wi = b[0];
 }

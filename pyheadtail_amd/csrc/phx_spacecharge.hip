@@ -1,15 +1,16 @@
 // 2.5-D transverse Gaussian space charge (spacecharge/spacecharge.py:159-417)
 // and the device Faddeeva function behind it.
-#include "phx_device.h"
-#include "phx_faddeeva.h"
 #include "phx_host.h"
+#include "phx_sc.h"
 
-namespace {
-
-const WofzCoeffs &wofz_host_coeffs() {
+const WofzCoeffs &phx_wofz_host_coeffs() {
   static const WofzCoeffs c = wofz_coefficients();
   return c;
 }
+
+namespace {
+
+const WofzCoeffs &wofz_host_coeffs() { return phx_wofz_host_coeffs(); }
 
 __global__ void __launch_bounds__(PHX_BLOCK)
     k_math_wofz(const double *__restrict__ x, const double *__restrict__ y, int64_t n,
@@ -20,34 +21,6 @@ __global__ void __launch_bounds__(PHX_BLOCK)
   phx_wofz(c, x[i], y[i], a, b);
   wr[i] = a;
   wi[i] = b;
-}
-
-// TransverseGaussianSpaceCharge._efieldn_mit for x, y >= 0, sig_x > sig_y
-// (:275-300): the operation order of the reference's expression.
-__device__ __forceinline__ void efieldn_mit(const WofzCoeffs &c, double x, double y, double sx,
-                                            double sy, double denom_pre, double &ex_out,
-                                            double &ey_out) {
-  const double sig_sqrt = sqrt(2.0 * (sx * sx - sy * sy));
-  double w1r, w1i, w2r, w2i;
-  phx_wofz(c, x / sig_sqrt, y / sig_sqrt, w1r, w1i);
-  const double ex = exp(-x * x / (2.0 * sx * sx) + -y * y / (2.0 * sy * sy));
-  phx_wofz(c, x * sy / (sx * sig_sqrt), y * sx / (sy * sig_sqrt), w2r, w2i);
-  const double denom = denom_pre * sig_sqrt;  // 2 eps0 sqrt(pi) sig_sqrt
-  ex_out = (w1i - ex * w2i) / denom;
-  ey_out = (w1r - ex * w2r) / denom;
-}
-
-// _efieldn_round (:392-399), x, y >= 0.
-__device__ __forceinline__ void efieldn_round(double x, double y, double sr, double round_pre,
-                                              double &ex_out, double &ey_out) {
-  const double r2 = x * x + y * y;
-  const double amplitude = (1.0 - exp(-r2 / (2.0 * sr * sr))) / (round_pre * r2);
-  ex_out = x * amplitude;
-  ey_out = y * amplitude;
-}
-
-__device__ __forceinline__ double sign_of(double v) {
-  return v > 0.0 ? 1.0 : (v < 0.0 ? -1.0 : (v == 0.0 ? 0.0 : v));
 }
 
 __global__ void __launch_bounds__(PHX_BLOCK)
@@ -67,31 +40,13 @@ __global__ void __launch_bounds__(PHX_BLOCK)
       s = (f >= 0.0 && f < (double)S) ? (int32_t)f : -1;
     }
     if (s < 0 || s >= S) continue;
-    // Nlambda_i = lambda_bins(smoothen=False) / slice_widths
-    const double nl = ((double)sl.counts[s] * p.charge_per_mp) / (sl.z_bins[s + 1] - sl.z_bins[s]);
-    const double sx = sl.sigma_x[s], sy = sl.sigma_y[s];
-    const double dx = x[j] - sl.mean_x[s], dy = y[j] - sl.mean_y[s];
-    const double ax = fabs(dx), ay = fabs(dy);
-    double ex, ey;
-    // add_sigma_check (:401-417): np.allclose(sig_y, sig_x, rtol, atol),
-    // np.allclose(sig_y, 0, rtol, atol), sig_x < sig_y
-    const bool round = p.sig_check && fabs(sy - sx) <= p.atol + p.rtol * fabs(sx);
-    if (round) {
-      if (fabs(sy - 0.0) <= p.atol + p.rtol * 0.0) {
-        ex = ey = 0.0;
-      } else {
-        efieldn_round(ax, ay, sx, p.round_pre, ex, ey);
-      }
-    } else if (p.sig_check && sx < sy) {
-      efieldn_mit(c, ay, ax, sy, sx, p.denom_pre, ey, ex);
-    } else {
-      efieldn_mit(c, ax, ay, sx, sy, p.denom_pre, ex, ey);
-    }
-    // get_efieldn (:233-246), then the kick (:209-224)
-    ex = fabs(ex) * sign_of(dx);
-    ey = fabs(ey) * sign_of(dy);
-    xp[j] = xp[j] + (ex * nl) * p.prefactor;
-    yp[j] = yp[j] + (ey * nl) * p.prefactor;
+    const double nl = sc_line_density(sl.counts[s], p.charge_per_mp, sl.z_bins[s],
+                                      sl.z_bins[s + 1]);
+    double xpj = xp[j], ypj = yp[j];
+    sc_kick_particle(c, p, sl.mean_x[s], sl.mean_y[s], sl.sigma_x[s], sl.sigma_y[s], nl, x[j],
+                     y[j], xpj, ypj);
+    xp[j] = xpj;
+    yp[j] = ypj;
   }
 }
 

@@ -5,8 +5,16 @@ FusedOneTurnMap(one_turn_map) accepts the element list that a user would
 otherwise track one by one (as Synchrotron.one_turn_map,
 machines/synchrotron.py:294-298, or the loop of
 tests/simulation_tests/test_bb_resonator.py:91-96): TransverseSegmentMap,
-LinearMap, RFSystems, Drift and Kick elements, optionally ended by one
-WakeField.  It is itself an Element, so it can replace that list.
+LinearMap, RFSystems, Drift and Kick elements, TransverseGaussianSpaceCharge
+kicks between them (as Synchrotron.install_after_each_transverse_segment
+places them), optionally ended by one WakeField.  It is itself an Element,
+so it can replace that list.
+
+A map with m space-charge kicks runs each turn as m + 1 stages: stage 0
+is the elements before the first kick, stage k the kick k, then the
+elements up to the next kick; after each of the first m stages the
+space-charge slicing of the current state (counts, means, standard
+deviations per slice) is computed on the device for the kick that follows.
 
 Host scalars (p0 updates of the RF kicks, wake ages, wake factors) are
 advanced on the host for every turn with the elements' own arithmetic
@@ -24,6 +32,7 @@ from . import _phx
 from .general.element import Element
 from .impedances.wakes import WakeField
 from .particles.particles import ScalarState, COORDS
+from .spacecharge.spacecharge import TransverseGaussianSpaceCharge
 from .trackers.longitudinal_tracking import Drift, Kick, LinearMap, RFSystems
 from .trackers.transverse_tracking import TransverseSegmentMap
 
@@ -80,15 +89,34 @@ class FusedOneTurnMap(Element):
         if elements and isinstance(elements[-1], WakeField):
             self.wake = elements.pop()
         for el in elements:
-            if not isinstance(el, self.LOCAL):
+            if not isinstance(el, self.LOCAL + (TransverseGaussianSpaceCharge,)):
                 raise TypeError("FusedOneTurnMap cannot fuse %s: supported are "
-                                "TransverseSegmentMap, LinearMap, RFSystems, Drift, Kick "
-                                "and one final WakeField" % type(el).__name__)
-        n_t = sum(isinstance(el, TransverseSegmentMap) for el in elements)
-        n_l = sum(isinstance(el, (RFSystems, Drift, Kick)) for el in elements)
-        n_m = sum(isinstance(el, LinearMap) for el in elements)
-        if n_t > 4 or n_l > 2 or n_m > 2 or len(elements) > _phx.MAX_PROGRAM:
-            raise ValueError("too many elements for one fused pass")
+                                "TransverseSegmentMap, LinearMap, RFSystems, Drift, Kick, "
+                                "TransverseGaussianSpaceCharge and one final WakeField"
+                                % type(el).__name__)
+        # stages: the elements before the first space-charge kick, then
+        # each kick with the elements up to the next one
+        self.sc = [el for el in elements if isinstance(el, TransverseGaussianSpaceCharge)]
+        self.stages = [[]]
+        for el in elements:
+            if isinstance(el, TransverseGaussianSpaceCharge):
+                self.stages.append([])
+            else:
+                self.stages[-1].append(el)
+        if len(self.sc) > _phx.MAX_SC_STAGES:
+            raise ValueError("the fused engine takes at most %d space-charge kicks per turn"
+                             % _phx.MAX_SC_STAGES)
+        for sc in self.sc:
+            if sc.slicer.mode != 'uniform_bin':
+                raise ValueError("the fused engine slices with uniform bins; track a "
+                                 "space-charge kick with a %s slicer as an element"
+                                 % sc.slicer.mode)
+        for st in self.stages:
+            n_t = sum(isinstance(el, TransverseSegmentMap) for el in st)
+            n_l = sum(isinstance(el, (RFSystems, Drift, Kick)) for el in st)
+            n_m = sum(isinstance(el, LinearMap) for el in st)
+            if n_t > 4 or n_l > 2 or n_m > 2 or len(st) > _phx.MAX_PROGRAM:
+                raise ValueError("too many elements for one fused pass")
         if self.wake is not None and not self.wake.wake_kicks:
             raise ValueError("WakeField without kicks")
         if self.wake is not None and self.wake.slicer.mode != 'uniform_bin':
@@ -97,7 +125,7 @@ class FusedOneTurnMap(Element):
         if self.wake is not None and len(self.wake.wake_kicks) > _phx.MAX_KICKS:
             raise ValueError("the fused engine takes at most %d wake kicks; track this "
                              "WakeField as an element" % _phx.MAX_KICKS)
-        self.local = elements
+        self.local = [el for el in elements if not isinstance(el, TransverseGaussianSpaceCharge)]
         self.record_stats = bool(record_stats)
         self.graph_turns = int(graph_turns)
         self._bufs = None
@@ -108,6 +136,7 @@ class FusedOneTurnMap(Element):
         self._pending_beam = None
         self.last_stats = None
         self._shape = (_phx.SHAPE_GENERIC, _phx.KICKS_GENERIC, _phx.DETS_GENERIC)
+        self._sc_shapes = [_phx.SHAPE_GENERIC] * len(self.sc)
         self._conv_kinds = 0
         # False: always the generic pass (the same arithmetic, for checks)
         self.specialise = bool(specialise)
@@ -118,9 +147,38 @@ class FusedOneTurnMap(Element):
 
     # -- per-turn programs -------------------------------------------------------
     def _program(self, state):
+        """The programs of one turn's stages (one without space charge),
+        advancing the host scalars as the elements do, in map order."""
+        progs = []
+        for k, st in enumerate(self.stages):
+            if k > 0:
+                # the kick that ends the previous stage: its parameters at
+                # this point of the turn (spacecharge.py:209-210)
+                prev = progs[-1]
+                sc = self.sc[k - 1]
+                prev.has_sc = 1
+                prev.sc_slicer = sc.slicer.phx_config()
+                prev.sc = sc._params(state)
+            progs.append(self._stage_program(st, state))
+        if self.wake is not None:
+            w = self.wake
+            for prog in progs:
+                prog.has_wake = 1
+                prog.age_increment = state.circumference / (state.beta * c)
+                prog.slicer = w.slicer.phx_config()
+                prog.beta = state.beta
+                prog.beta_c = state.beta * c
+                prog.n_kicks = len(w.wake_kicks)
+            for i, k in enumerate(w.wake_kicks):
+                rec = k.record(state)
+                for prog in progs:
+                    prog.kick[i] = rec
+        return progs
+
+    def _stage_program(self, elements, state):
         prog = _phx.TurnProgram()
         n_t = n_l = n_m = 0
-        for i, el in enumerate(self.local):
+        for i, el in enumerate(elements):
             if isinstance(el, TransverseSegmentMap):
                 prog.el_type[i], prog.el_index[i] = _phx.EL_TSEG, n_t
                 prog.tseg[n_t] = el._advance(state)
@@ -139,18 +197,8 @@ class FusedOneTurnMap(Element):
                     seq.op[0] = el._op(state)
                     prog.lseq[n_l] = seq
                 n_l += 1
-        prog.n_el = len(self.local)
+        prog.n_el = len(elements)
         prog.n_tseg, prog.n_long, prog.n_lin = n_t, n_l, n_m
-        if self.wake is not None:
-            w = self.wake
-            prog.has_wake = 1
-            prog.age_increment = state.circumference / (state.beta * c)
-            prog.slicer = w.slicer.phx_config()
-            prog.beta = state.beta
-            prog.beta_c = state.beta * c
-            prog.n_kicks = len(w.wake_kicks)
-            for i, k in enumerate(w.wake_kicks):
-                prog.kick[i] = k.record(state)
         return prog
 
     @staticmethod
@@ -194,6 +242,21 @@ class FusedOneTurnMap(Element):
                     }.get(FusedOneTurnMap._det_key(prog.tseg[0]), _phx.DETS_GENERIC)
         return shape, kicks, dets
 
+    @classmethod
+    def _stage_shapes(cls, progs, wake):
+        """Space-charge engines: ((stage 0's PHX_SHAPE_*, the kick set, the
+        detuner set shared by every stage's segments), [PHX_SHAPE_* of
+        stages 1..m]); a stage without elements is PHX_SHAPE_NONE."""
+        shapes, dets = [], set()
+        for q in progs:
+            sh, _, ds = cls._shapes(q, None)
+            shapes.append(_phx.SHAPE_NONE if q.n_el == 0 else sh)
+            if q.n_tseg > 0:
+                dets.add(ds)
+        kicks = cls._shapes(progs[0], wake)[1]
+        ds = dets.pop() if len(dets) == 1 else _phx.DETS_GENERIC
+        return (shapes[0], kicks, ds), shapes[1:]
+
     @staticmethod
     def _det_key(tseg):
         """(type, order) of each detuner of a segment record (order 0 for
@@ -210,22 +273,27 @@ class FusedOneTurnMap(Element):
         """Per-turn programs from the host-scalar recurrence.  Once a turn
         leaves the scalars unchanged, every later turn has the same program
         and the same scalars (the recurrence is deterministic), so the rest
-        of the table is filled without evaluating it."""
+        of the table is filled without evaluating it.  A turn's stage
+        programs are stored together; the table holds the first's index."""
         state = ScalarState(beam)
         progs, table, index = [], np.empty(n_turns, dtype=np.int32), {}
+        n_st = len(self.stages)
         t = 0
         while t < n_turns:
             before = self._state_key(state)
-            p = self._program(state)
+            ps = self._program(state)
+            p = ps[0]
             if t == 0 and self.specialise:
                 self._shape = self._shapes(p, self.wake)
+                if n_st > 1:
+                    self._shape, self._sc_shapes = self._stage_shapes(ps, self.wake)
             for k in range(p.n_kicks):
                 self._conv_kinds |= _phx.conv_kind(p.kick[k])
-            key = bytes(memoryview(p).cast("B"))
+            key = b"".join(bytes(memoryview(q).cast("B")) for q in ps)
             if key not in index:
                 index[key] = len(progs)
                 progs.append(key)
-            table[t] = index[key]
+            table[t] = index[key] * n_st
             t += 1
             if self._state_key(state) == before:
                 table[t:] = index[key]
@@ -240,10 +308,14 @@ class FusedOneTurnMap(Element):
         b = self._bufs
         if b is None or b['n'] != n:
             self._drop_graphs()
+            S_sc = max([int(sc.slicer.n_slices) for sc in self.sc] + [0])
             S = int(self.wake.slicer.n_slices) if self.wake is not None else 0
             n_kicks = len(self.wake.wake_kicks) if self.wake is not None else 0
             T = self.wake.n_turns_wake_max if self.wake is not None else 1
-            b = dict(n=n, S=S, n_kicks=n_kicks, T=T, turn_cap=0, prog_cap=0)
+            b = dict(n=n, S=S, n_kicks=n_kicks, T=T, turn_cap=0, prog_cap=0, S_sc=S_sc)
+            if S_sc > 0:
+                nsc = _phx.lib().phx_engine_sc_ws_bytes(n, S_sc)
+                b['sc_ws'] = torch.zeros(nsc, dtype=torch.uint8, device=dev)
             b['counter'] = torch.zeros(1, dtype=torch.int32, device=dev)
             b['pending'] = torch.zeros(1, dtype=torch.int32, device=dev)
             b['state'] = torch.zeros(C.sizeof(_phx.SliceState), dtype=torch.uint8, device=dev)
@@ -266,7 +338,7 @@ class FusedOneTurnMap(Element):
             self._drop_graphs()
             b['turn_cap'] = max(n_turns, 2 * b['turn_cap'], 64)
             b['prog_cap'] = max(n_programs, 2 * b['prog_cap'], 4)
-            psz = C.sizeof(_phx.TurnProgram)
+            psz = C.sizeof(_phx.TurnProgram) * len(self.stages)
             b['programs'] = torch.zeros(b['prog_cap'] * psz, dtype=torch.uint8, device=dev)
             b['table'] = torch.zeros(b['turn_cap'], dtype=torch.int32, device=dev)
             # host mirrors of the device programs and table: a call uploads
@@ -301,6 +373,11 @@ class FusedOneTurnMap(Element):
         eng.counts = b['counts'].data_ptr()
         eng.slice_means = b['means'].data_ptr()
         eng.ws = b['ws'].data_ptr()
+        eng.sc_stages = len(self.sc)
+        for k, sh in enumerate(self._sc_shapes):
+            eng.sc_stage_shape[k + 1] = sh
+        eng.sc_slices = b['S_sc']
+        eng.sc_ws = b['sc_ws'].data_ptr() if b['S_sc'] > 0 else None
         return eng
 
     def _drop_graphs(self):

@@ -6,9 +6,9 @@ them; an admitted job then advances by chunks of turns, each chunk enqueued
 as soon as the job's previous one has completed, the jobs spread over a few
 HIP streams so that their kernels overlap, and a job leaves as soon as its
 turns are done, releasing its share of the budget to
-the queue.  A one-turn map the fused engine takes runs fused; one it does not
-take (space-charge kicks, for one) runs element by element, in smaller
-chunks.  Jobs share no device state: each result is the one the job gets
+the queue.  A one-turn map the fused engine takes (space-charge kicks
+included) runs fused; one it does not take runs element by element, in
+smaller chunks.  Jobs share no device state: each result is the one the job gets
 alone, bit for bit.
 """
 import collections
@@ -36,9 +36,11 @@ class ElementLoop(object):
                 el.track(beam)
 
 
-def make_runner(one_turn_map, graph_turns):
-    """The fused engine for one_turn_map if it takes every element, else an
-    element loop."""
+def make_runner(one_turn_map, graph_turns, fuse=True):
+    """The fused engine for one_turn_map if it takes every element (and
+    fuse is set), else an element loop."""
+    if not fuse:
+        return ElementLoop(one_turn_map)
     try:
         eng = FusedOneTurnMap(one_turn_map, graph_turns=graph_turns)
     except (TypeError, ValueError):
@@ -49,7 +51,7 @@ def make_runner(one_turn_map, graph_turns):
 
 class Job(object):
     __slots__ = ('job_id', 'one_turn_map', 'bunch', 'n_turns', 'remaining', 'bytes', 'engine',
-                 'stream', 'event', 't_submit', 't_admit', 't_done')
+                 'stream', 'event', 't_submit', 't_admit', 't_done', 'fuse')
 
     @property
     def time_to_result(self):
@@ -65,6 +67,11 @@ def job_bytes(one_turn_map, bunch):
     nk = len(wake.wake_kicks) if wake is not None else 0
     T = wake.n_turns_wake_max if wake is not None else 1
     ws = max(0, _phx.lib().phx_engine_ws_bytes(n, S, nk, T))
+    from .spacecharge.spacecharge import TransverseGaussianSpaceCharge
+    S_sc = max([int(el.slicer.n_slices) for el in one_turn_map
+                if isinstance(el, TransverseGaussianSpaceCharge)] + [0])
+    if S_sc > 0:
+        ws += max(0, _phx.lib().phx_engine_sc_ws_bytes(n, S_sc))
     return 6 * 8 * n + 4 * n + ws + (S + 1) * 8 * (3 + 4 * T + nk)
 
 
@@ -91,9 +98,12 @@ class ScanServer(object):
     def busy(self):
         return bool(self.queue or self.active)
 
-    def submit(self, one_turn_map, bunch, n_turns, job_id=None):
-        """Queue a job; bunch must already be on the device."""
+    def submit(self, one_turn_map, bunch, n_turns, job_id=None, fuse=True):
+        """Queue a job; bunch must already be on the device.  fuse=False
+        runs it element by element even where the fused engine would take
+        its map."""
         j = Job()
+        j.fuse = bool(fuse)
         j.job_id = self._next_id if job_id is None else job_id
         self._next_id += 1
         j.one_turn_map, j.bunch, j.n_turns = one_turn_map, bunch, int(n_turns)
@@ -124,7 +134,7 @@ class ScanServer(object):
         caller = torch.cuda.current_stream()
         while self.queue and self.used + self.queue[0].bytes <= self.budget:
             j = self.queue.popleft()
-            j.engine = make_runner(j.one_turn_map, self.graph_turns)
+            j.engine = make_runner(j.one_turn_map, self.graph_turns, j.fuse)
             j.stream = self._stream_for(j)
             j.stream.wait_stream(caller)      # the bunch was uploaded on the caller's stream
             j.t_admit = time.perf_counter()

@@ -82,7 +82,7 @@ def test_server_two_element_jobs_on_two_streams():
     srv = ScanServer(n_streams=3, chunk_turns=16, graph_turns=4, element_chunk_turns=3,
                      big_job_bytes=1 << 40)
     jobs = [_sc_job(seed, n=30000 + seed) for seed in (81, 82)]
-    handles = [srv.submit(m, b, n) for (m, b), n in zip(jobs, turns)]
+    handles = [srv.submit(m, b, n, fuse=False) for (m, b), n in zip(jobs, turns)]
     assert handles[0].bytes < srv.big_job_bytes
     side = _sc_job(90, n=50000)[1]
     while srv.busy:
@@ -96,10 +96,13 @@ def test_server_two_element_jobs_on_two_streams():
                                           err_msg="job %d %s" % (j, k))
 
 
-def test_server_mixes_fused_and_element_jobs():
-    """A job with space-charge kicks (not fusable) runs element by element
-    beside fused jobs; every job ends as it does alone."""
-    print("\n[case] scan server: space-charge job (elements) beside two fused jobs", flush=True)
+@pytest.mark.parametrize("fuse_sc", [False, True])
+def test_server_mixes_fused_and_element_jobs(fuse_sc):
+    """A job with space-charge kicks, run element by element (fuse_sc
+    False) or by the fused engine's space-charge stages, beside fused jobs;
+    every job ends as it does alone."""
+    print("\n[case] scan server: space-charge job (%s) beside two fused jobs"
+          % ("fused" if fuse_sc else "elements"), flush=True)
     import numpy as np
     from scipy.constants import e, m_p
     import bench
@@ -124,7 +127,10 @@ def test_server_mixes_fused_and_element_jobs():
 
     turns = [23, 19, 30]
     m, b = sc_job()
-    ElementLoop(m).track_turns(b, turns[0])
+    if fuse_sc:
+        FusedOneTurnMap(m, graph_turns=8).track_turns(b, turns[0])
+    else:
+        ElementLoop(m).track_turns(b, turns[0])
     alone = [{k: getattr(b, k).cpu().numpy() for k in COORDS}]
     for i, n in zip((1, 3), turns[1:]):
         m, b = bench.scan_job(i, n_scale=0.01)
@@ -132,8 +138,10 @@ def test_server_mixes_fused_and_element_jobs():
         alone.append({k: getattr(b, k).cpu().numpy() for k in COORDS})
     srv = ScanServer(n_streams=2, chunk_turns=16, graph_turns=8, element_chunk_turns=5)
     jobs = [sc_job(), bench.scan_job(1, n_scale=0.01), bench.scan_job(3, n_scale=0.01)]
-    handles = [srv.submit(m, b, n) for (m, b), n in zip(jobs, turns)]
+    handles = [srv.submit(m, b, n, fuse=(fuse_sc or i > 0))
+               for i, ((m, b), n) in enumerate(zip(jobs, turns))]
     srv.drain()
+    assert handles[0].fuse == fuse_sc
     assert [h.engine is None for h in handles] == [True] * 3
     for j, h in enumerate(handles):
         for k in COORDS:

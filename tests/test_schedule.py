@@ -23,7 +23,8 @@ def _beam(ob):
 def _full(eng, beam, n):
     from pyheadtail_amd.particles.particles import ScalarState
     st = ScalarState(beam)
-    keys = [bytes(memoryview(eng._program(st)).cast("B")) for _ in range(n)]
+    keys = [b"".join(bytes(memoryview(q).cast("B")) for q in eng._program(st))
+            for _ in range(n)]
     return keys, st
 
 
