@@ -476,8 +476,8 @@ def serve_jobs(n_jobs, seed=2000, space_charge=True):
     batched scan; the LinearMap ring with the broadband resonator.  Jobs of
     5e6 macroparticles and more (the upper end of the range) also get a
     2.5D transverse Gaussian space-charge kick after each of 8 transverse
-    segments (their own slicing, the job's slice count); those run element
-    by element.  Host-side bunches, generated up front."""
+    segments (their own slicing, the job's slice count), run by the fused
+    engine's space-charge stages.  Host-side bunches, generated up front."""
     import numpy as np
     from scipy.constants import e, m_p
     from pyheadtail_amd.machines.synchrotron import Synchrotron

@@ -83,7 +83,8 @@ struct EngineArgs {
 #define SC_TERMS 4
 
 struct ScSlice {
-  double mx, my, sx, sy, nl, pad[3];
+  double mx, my, nl, pad;
+  ScField f;
 };
 
 struct ScWs {
@@ -125,20 +126,20 @@ struct ScCtx {
 
 template <int N>
 __device__ __forceinline__ void sc_kick_n(P6 (&p)[N], const ScCtx &c, const phx_spacecharge &sp) {
-  double dx[N], dy[N], sx[N], sy[N], nl[N], ex[N], ey[N];
+  double dx[N], dy[N], nl[N], ex[N], ey[N];
+  ScField f[N];
   bool in[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    const double f = slice_floor_r(p[k].z, c.tail, c.width, c.inv_width);
-    in[k] = f >= 0.0 && f < (double)c.S;
-    const ScSlice &q = c.sl[in[k] ? (int)f : 0];
+    const double fl = slice_floor_r(p[k].z, c.tail, c.width, c.inv_width);
+    in[k] = fl >= 0.0 && fl < (double)c.S;
+    const ScSlice &q = c.sl[in[k] ? (int)fl : 0];
     dx[k] = p[k].x - q.mx;
     dy[k] = p[k].y - q.my;
-    sx[k] = q.sx;
-    sy[k] = q.sy;
     nl[k] = q.nl;
+    f[k] = q.f;
   }
-  sc_field_n<N>(*c.wc, sp, dx, dy, sx, sy, ex, ey);
+  sc_field_n<N>(*c.wc, sp, dx, dy, f, ex, ey);
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     if (in[k]) {
@@ -796,20 +797,20 @@ __device__ __forceinline__ void sc_merge_body(const EngineArgs &a, int stage, in
   ScSlice row;
   row.mx = tot.count ? sx / cnt : 0.0;
   row.my = tot.count ? sy / cnt : 0.0;
-  row.sx = 0.0;
-  row.sy = 0.0;
+  double sgx = 0.0, sgy = 0.0;
   if (tot.count > 1) {
     double xh, xl, yh, yl, qxh, qxl, qyh, qyl;
     fx_value_dd(tot.hi[0], tot.lo[0], ex, xh, xl);
     fx_value_dd(tot.hi[1], tot.lo[1], ey, yh, yl);
     fx_value_dd(tot.hi[2], tot.lo[2], exx, qxh, qxl);
     fx_value_dd(tot.hi[3], tot.lo[3], eyy, qyh, qyl);
-    row.sx = sqrt(var_from_sums(xh, xl + fx, qxh, qxl + fxx, cnt));
-    row.sy = sqrt(var_from_sums(yh, yl + fy, qyh, qyl + fyy, cnt));
+    sgx = sqrt(var_from_sums(xh, xl + fx, qxh, qxl + fxx, cnt));
+    sgy = sqrt(var_from_sums(yh, yl + fy, qyh, qyl + fyy, cnt));
   }
   row.nl = sc_line_density((int32_t)tot.count, g.sc.charge_per_mp, arange_value(tail, width, s),
                            arange_value(tail, width, s + 1));
-  row.pad[0] = row.pad[1] = row.pad[2] = 0.0;
+  row.pad = 0.0;
+  row.f = sc_field_params(sgx, sgy, g.sc);
   sw.slices[s] = row;
 }
 

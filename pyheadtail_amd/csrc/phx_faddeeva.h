@@ -12,6 +12,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "phx_device.h"
+
 #include <cmath>
 
 #define PHX_WOFZ_N 40
@@ -58,9 +60,11 @@ __device__ __forceinline__ void phx_wofz_n(const WofzCoeffs &c, const double (&x
     // L - i z = (L + y) - i x,  L + i z = (L - y) + i x
     const double dr = L + y[k], di = -x[k];
     const double den = dr * dr + di * di;
-    // q = 1 / (L - i z) = (dr - i di) / den
-    qr[k] = dr / den;
-    qi[k] = -di / den;
+    // q = 1 / (L - i z) = (dr - i di) / den: both quotients from one
+    // correctly rounded reciprocal (the division's bits, div_rcp)
+    const double rden = 1.0 / den;
+    qr[k] = div_rcp(dr, den, rden);
+    qi[k] = div_rcp(-di, den, rden);
     // Z = (L + i z) q
     const double nr = L - y[k], ni = x[k];
     Zr[k] = nr * qr[k] - ni * qi[k];

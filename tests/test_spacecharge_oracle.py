@@ -5,7 +5,8 @@ of a line charge, lambda / (2 pi eps0 r); the elliptic field tends to the
 round one as sigma_y -> sigma_x; exchanging the planes exchanges the
 components; the round field is the closed form (1 - exp(-r^2/2s^2)) /
 (2 pi eps0 r^2) (x, y) and is linear near the axis with slope 1/(4 pi eps0
-s^2)."""
+s^2); the assembled kick (prefactor q L / (p0 beta gamma gamma c), line
+density N q_mp / width) is the textbook small-amplitude space-charge kick."""
 import numpy as np
 from scipy.constants import epsilon_0
 
@@ -50,3 +51,47 @@ def test_round_field_closed_form_and_axis_slope():
     # zero sigma in both planes: no field
     ex0, ey0 = R.sc_efieldn_checked(x, x, 0., 0.)
     assert np.all(ex0 == 0) and np.all(ey0 == 0)
+
+
+def test_kick_assembly_small_amplitude_closed_form():
+    """The whole kick of transverse_gaussian_space_charge (prefactor, line
+    density, field) against the textbook small-amplitude kick of a Gaussian
+    beam: inside an elliptic Gaussian slice of line charge density lambda,
+    E_x = lambda x / (2 pi eps0 sig_x (sig_x + sig_y)) (E_y likewise) near
+    the axis, and a particle of charge q and momentum p0 = gamma m beta c
+    crossing the length L receives dx' = q E_x L (1 - beta^2) / (p0 beta c)
+    (the beam's magnetic field cancels the fraction beta^2 of the electric
+    force).  lambda = N q_mp / slice width, here one slice of width
+    6 sigma_z.  Probe particles at 1e-3 sigma from the centre: the cubic
+    correction is 1e-6 of the kick."""
+    from scipy.constants import c, e, m_p
+    rng = np.random.default_rng(7)
+    n, sx_b, sy_b = 400000, 2.0e-3, 1.3e-3
+    gamma, ppm, L = 27.7, 1e11 / 400000, 1727.875
+    x = rng.normal(0, sx_b, n)
+    y = rng.normal(0, sy_b, n)
+    z = rng.uniform(-0.2, 0.2, n)
+    # probes near the centre (they belong to the slice like any particle)
+    k = 8
+    x[:k] = np.mean(x[k:]) + 1e-3 * sx_b * np.linspace(-1, 1, k)
+    y[:k] = np.mean(y[k:]) + 1e-3 * sy_b * np.linspace(1, -1, k)
+    coords = dict(x=x, xp=np.zeros(n), y=y, yp=np.zeros(n), z=z, dp=np.zeros(n))
+    b = R.Bunch(n, ppm, e, m_p, 6911.5, gamma, coords)
+    R.transverse_gaussian_space_charge(b, 1, L, n_sigma_z=3)
+    # the slice's statistics and line density from first principles
+    mx, my = np.mean(x), np.mean(y)
+    sx, sy = np.std(x, ddof=1), np.std(y, ddof=1)
+    width = 6 * np.std(z, ddof=1)            # one slice over mean_z +- 3 sigma_z
+    lam = n * ppm * e / width
+    beta = np.sqrt(1 - gamma**-2)
+    p0 = gamma * m_p * beta * c
+    ex = lam * (x[:k] - mx) / (2 * np.pi * epsilon_0 * sx * (sx + sy))
+    ey = lam * (y[:k] - my) / (2 * np.pi * epsilon_0 * sy * (sx + sy))
+    want_xp = e * ex * L * (1 - beta**2) / (p0 * beta * c)
+    want_yp = e * ey * L * (1 - beta**2) / (p0 * beta * c)
+    np.testing.assert_allclose(b.xp[:k], want_xp, rtol=1e-5, atol=0)
+    np.testing.assert_allclose(b.yp[:k], want_yp, rtol=1e-5, atol=0)
+    # the plausible slips (beta gamma^2 -> beta gamma, a factor 2 in the
+    # field, the slice width counted in bins) fall far outside that bound
+    for slip in (gamma, 2.0, 0.5, 1.0 / 6):
+        assert not np.allclose(b.xp[:k], want_xp * slip, rtol=1e-5, atol=0)
