@@ -68,8 +68,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-turns", type=int, default=100)
     ap.add_argument("--graph-turns", type=int, default=10)
-    ap.add_argument("--workload", choices=("resonator", "rw64", "scan64", "serve"),
+    ap.add_argument("--workload", choices=("resonator", "rw64", "scan64", "serve", "spacecharge"),
                     default="resonator")
+    ap.add_argument("--n-sc", type=int, default=8000000,
+                    help="macroparticles of the space-charge workload")
+    ap.add_argument("--steps-sc", type=int, default=50)
     ap.add_argument("--jobs", type=int, default=48)
     ap.add_argument("--big-job-mib", type=float, default=128.,
                     help="batched and served jobs of this many MiB or more share one stream")
@@ -469,6 +472,77 @@ def scan64(args):
     print(json.dumps(out))
 
 
+def sc_machine(qp, n_slices, n_seg=8):
+    """The served ring with space charge: smooth optics in n_seg segments,
+    chromaticity qp, the LinearMap (Q_s = 0.017), a 2.5D transverse
+    Gaussian space-charge kick of length C/n_seg after each segment on its
+    own n_slices-slice uniform slicing (+-3 sigma_z)."""
+    from scipy.constants import e, m_p
+    from pyheadtail_amd.machines.synchrotron import Synchrotron
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.spacecharge.spacecharge import TransverseGaussianSpaceCharge
+    mach = Synchrotron(optics_mode='smooth', charge=e, mass=m_p, p0=p0_of_gamma(27.7),
+                       circumference=6911.5, n_segments=n_seg, beta_x=54.6, beta_y=54.6,
+                       D_x=0., D_y=0., accQ_x=20.13, accQ_y=20.18, Qp_x=qp, Qp_y=qp,
+                       alpha_mom_compaction=1.9e-3, longitudinal_mode='linear', Q_s=0.017)
+    mach.install_after_each_transverse_segment(
+        TransverseGaussianSpaceCharge(UniformBinSlicer(n_slices, n_sigma_z=3), 6911.5 / n_seg))
+    return mach
+
+
+def spacecharge(args):
+    """A served job with space charge at full size: n_sc macroparticles,
+    700 slices, 8 segments with a space-charge kick after each, the
+    broadband resonator; turns of the fused engine (space-charge stages)
+    against the same turns element by element."""
+    import numpy as np
+    import torch
+    torch.cuda.set_device(0)
+    from pyheadtail_amd.engine import FusedOneTurnMap
+    from pyheadtail_amd.serve import ElementLoop
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    from pyheadtail_amd.impedances.wakes import WakeField, CircularResonator
+    n, S = args.n_sc, 700
+    mach = sc_machine(10., S)
+    one_turn = list(mach.one_turn_map) + [WakeField(UniformBinSlicer(S, n_sigma_z=3),
+                                                    CircularResonator(1e6, 1e9, 1))]
+    np.random.seed(SEED)
+    b = mach.generate_6D_Gaussian_bunch(n, 2e11, EPSN, EPSN, SIGMA_Z)
+    eng = FusedOneTurnMap(one_turn, graph_turns=args.graph_turns)
+    eng.reserve(b, args.steps_sc)
+    eng.track_turns(b, max(args.warmup // 10, args.graph_turns))
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    eng.track_turns(b, args.steps_sc)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / args.steps_sc
+    b._run_pending()
+    el = ElementLoop(one_turn)
+    el.track_turns(b, 1)
+    torch.cuda.synchronize()
+    k = 3
+    tt = time.perf_counter()
+    el.track_turns(b, k)
+    torch.cuda.synchronize()
+    ms_el = (time.perf_counter() - tt) / k * 1e3
+    out = {
+        "metric": "space-charge job throughput (macroparticle-turns per second)",
+        "value": n / ms * 1e3, "unit": "MP-turns/s", "n_gpus": 1, "steps": args.steps_sc,
+        "warmup": max(args.warmup // 10, args.graph_turns), "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Gaussian bunch)",
+        "config": {"workload": "served job with space charge: %d macroparticles, 8 smooth "
+                               "segments with a 2.5D Gaussian space-charge kick after each "
+                               "(700 slices), LinearMap, broadband resonator (700 slices)" % n,
+                   "macroparticles": n, "slices": S, "space_charge_kicks": 8},
+        "element_by_element": {"value": n / ms_el * 1e3, "ms_per_step": ms_el},
+        "speedup_vs_element_by_element": ms_el / ms,
+    }
+    print(json.dumps(out))
+
+
 def serve_jobs(n_jobs, seed=2000, space_charge=True):
     """Served instability-scan stream: Poisson arrivals at 4 jobs/s;
     macroparticles log-uniform in [1e5, 1e7], slices uniform in [100, 1000],
@@ -481,8 +555,6 @@ def serve_jobs(n_jobs, seed=2000, space_charge=True):
     import numpy as np
     from scipy.constants import e, m_p
     from pyheadtail_amd.machines.synchrotron import Synchrotron
-    from pyheadtail_amd.particles.slicing import UniformBinSlicer
-    from pyheadtail_amd.spacecharge.spacecharge import TransverseGaussianSpaceCharge
     rng = np.random.default_rng(seed)
     t, out = 0.0, []
     for j in range(n_jobs):
@@ -492,14 +564,14 @@ def serve_jobs(n_jobs, seed=2000, space_charge=True):
         turns = int(np.exp(rng.uniform(np.log(256), np.log(4096))))
         intensity, qp = rng.uniform(1e10, 3e11), rng.uniform(-5., 15.)
         sc = space_charge and n >= 5e6
-        mach = Synchrotron(optics_mode='smooth', charge=e, mass=m_p, p0=p0_of_gamma(27.7),
-                           circumference=6911.5, n_segments=8 if sc else 1, beta_x=54.6,
-                           beta_y=54.6, D_x=0., D_y=0., accQ_x=20.13, accQ_y=20.18, Qp_x=qp,
-                           Qp_y=qp, alpha_mom_compaction=1.9e-3, longitudinal_mode='linear',
-                           Q_s=0.017)
         if sc:
-            mach.install_after_each_transverse_segment(
-                TransverseGaussianSpaceCharge(UniformBinSlicer(S, n_sigma_z=3), 6911.5 / 8))
+            mach = sc_machine(qp, S)
+        else:
+            mach = Synchrotron(optics_mode='smooth', charge=e, mass=m_p, p0=p0_of_gamma(27.7),
+                               circumference=6911.5, n_segments=1, beta_x=54.6, beta_y=54.6,
+                               D_x=0., D_y=0., accQ_x=20.13, accQ_y=20.18, Qp_x=qp, Qp_y=qp,
+                               alpha_mom_compaction=1.9e-3, longitudinal_mode='linear',
+                               Q_s=0.017)
         np.random.seed(seed + j)
         b = mach.generate_6D_Gaussian_bunch(n, intensity, EPSN, EPSN, SIGMA_Z)
         host = {k: getattr(b, k).cpu().numpy() for k in ('x', 'xp', 'y', 'yp', 'z', 'dp')}
@@ -648,6 +720,9 @@ def main():
         return
     if args.workload == "serve":
         serve(args)
+        return
+    if args.workload == "spacecharge":
+        spacecharge(args)
         return
     cpu = cpu_np = None
     if not args.no_cpu:
