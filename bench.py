@@ -104,7 +104,7 @@ def copy_calibration(n, reps=5):
     for _ in range(reps):
         dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration": {"bytes_read": src.numel() * 8,
+    emit(json.dumps({"copy_calibration": {"bytes_read": src.numel() * 8,
                                            "bytes_written": dst.numel() * 8, "reps": reps}}))
 
 
@@ -254,7 +254,7 @@ def record_traffic(d, n):
     recs = [r for r in recs if r.get("macroparticles") != n] + [rec]
     with open(path, "w") as f:
         json.dump(sorted(recs, key=lambda r: r["macroparticles"]), f, indent=1)
-    print(json.dumps(rec))
+    emit(json.dumps(rec))
 
 
 def p0_of_gamma(gamma):
@@ -367,7 +367,7 @@ def rw64(args):
                                for p in RW64_PRECISIONS},
         "kick_error_vs_fp64_direct": errs,
     }
-    print(json.dumps(out))
+    emit(json.dumps(out))
 
 
 def scan_params(i, seed_base=1000):
@@ -469,7 +469,7 @@ def scan64(args):
         "speedup_vs_job_by_job": t_seq / t_bat,
         "speedup_streams_vs_job_by_job": t_seq / t_str,
     }
-    print(json.dumps(out))
+    emit(json.dumps(out))
 
 
 def sc_machine(qp, n_slices, n_seg=8):
@@ -540,7 +540,7 @@ def spacecharge(args):
         "element_by_element": {"value": n / ms_el * 1e3, "ms_per_step": ms_el},
         "speedup_vs_element_by_element": ms_el / ms,
     }
-    print(json.dumps(out))
+    emit(json.dumps(out))
 
 
 def serve_jobs(n_jobs, seed=2000, space_charge=True):
@@ -640,7 +640,7 @@ def serve(args):
         "p50_s": p[50], "p99_s": p[99],
         "aggregate_mp_turns_per_s": work / makespan, "makespan_s": makespan,
     }
-    print(json.dumps(out))
+    emit(json.dumps(out))
 
 
 def headline_turns(n, args, steps, stage_turns):
@@ -699,7 +699,18 @@ def roofline(n, kernel_ms, kernel):
             "algorithmic_bytes_per_launch": BYTES_PER_MP_PASS * n}
 
 
+# The one result line goes to the process's standard output; everything
+# else (the elements' warnings and notes, which the reference's classes
+# print) goes to standard error while the workloads run.
+_RESULT_OUT = sys.stdout
+
+
+def emit(line):
+    print(line, file=_RESULT_OUT, flush=True)
+
+
 def main():
+    sys.stdout = sys.stderr
     args = parse()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -772,7 +783,7 @@ def main():
             "turn_roofline_frac": (BYTES_PER_MP_PASS * args.n_large / (sum(st_l) * 1e-3) / 1e9 /
                                    HBM_PEAK_GBS),
         }
-    print(json.dumps(out))
+    emit(json.dumps(out))
 
 
 if __name__ == "__main__":
