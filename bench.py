@@ -427,22 +427,21 @@ def scan64(args):
     steps = min(args.steps, 200)
     tuner = Tuner(args.tune_file) if args.tune_file else Tuner()
     entry = tune_batch(scan, tuner=tuner, turns=20, force=args.retune)
-    print("[scan64] tuned: pass_grid %d, graph_turns %d (%.3f ms per turn)"
-          % (entry['pass_grid'], entry['graph_turns'], entry['ms_per_turn']), file=sys.stderr,
-          flush=True)
+    print("[scan64] tuned: pass_grid %d, groups_per_block %d, graph_turns %d (%.3f ms per turn)"
+          % (entry['pass_grid'], entry.get('groups_per_block', 0), entry['graph_turns'],
+             entry['ms_per_turn']), file=sys.stderr, flush=True)
     scan.track_turns(40)                      # histories (T <= 32) filled, graphs captured
     t0 = time.perf_counter()
     scan.track_turns(steps)
     t_bat = time.perf_counter() - t0
     # the plain GPU path job by job: each job's own engine at its default
     # pass grid, one after the other
-    tuned = scan.pass_grid
-    scan.configure(pass_grid=0)
+    scan.configure(pass_grid=0, groups_per_block=0)
     scan.track_turns_job_by_job(40)
     t0 = time.perf_counter()
     scan.track_turns_job_by_job(steps)
     t_seq = time.perf_counter() - t0
-    scan.configure(pass_grid=tuned)
+    scan.configure(pass_grid=entry['pass_grid'], groups_per_block=entry.get('groups_per_block', 0))
     # per-job graphs on 12 streams
     streams = BatchedScan(jobs, mode='streams', n_streams=args.streams, chunk_turns=50,
                           graph_turns=args.graph_turns,
@@ -462,7 +461,8 @@ def scan64(args):
                                "chromaticity in [-5, 15], n_turns_wake in {1,8,32}; LinearMap "
                                "ring, resonator or resistive-wall wake" % n_mp,
                    "bunches": 64, "macroparticles": n_mp},
-        "tuned": {k: entry[k] for k in ('pass_grid', 'graph_turns', 'ms_per_turn', 'candidates')},
+        "tuned": {k: entry.get(k, 0) for k in ('pass_grid', 'groups_per_block', 'graph_turns',
+                                                 'ms_per_turn', 'candidates')},
         "job_by_job": {"value": n_mp * steps / t_seq, "ms_per_step": t_seq / steps * 1e3},
         "streams": {"value": n_mp * steps / t_str, "ms_per_step": t_str / steps * 1e3,
                     "streams": args.streams},

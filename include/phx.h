@@ -66,6 +66,11 @@ int phx_init(int device);
 /* sizeof of a public record by name ("phx_tseg", "phx_turn_program", ...),
    -1 if unknown: lets bindings check their struct layouts. */
 int64_t phx_sizeof(const char *type_name);
+/* Macroparticles per group of the fused engine's particle pass: a pass of
+   G blocks gives block b the groups b, b + G, ... (the last one partial),
+   so the pass grid that gives every block k groups is
+   ceil(ceil(n / group) / k).  No reference counterpart. */
+int32_t phx_pass_group_size(void);
 /* Bytes of workspace that the reduction and slicing calls need for n
    particles and up to n_slices slices.  The workspace must be zero-filled
    once when allocated; every call leaves it zero-filled again. */

@@ -208,6 +208,7 @@ SIGNATURES = {
     "phx_abi_version": (C.c_int, []),
     "phx_init": (C.c_int, [C.c_int]),
     "phx_sizeof": (I64, [C.c_char_p]),
+    "phx_pass_group_size": (I32, []),
     "phx_workspace_bytes": (I64, [I64, I32]),
     "phx_bunch_sums_len": (I64, [I32]),
     "phx_bunch_sums": (C.c_int, [P, I32, I64, P, P, P]),

@@ -17,8 +17,12 @@ one of two ways:
 Either way every job's arithmetic, reduction order included, is the one it
 has alone with the same pass grid: results are bit-identical to running
 the job by itself.  The pass grid (blocks of the particle pass per job;
-0: as many as stay resident) sets the bunch sums' fold order; the autotuner
-(tune.py) picks it, and the number of turns per graph, by measurement.
+0: as many as stay resident) sets the bunch sums' fold order.  It is either
+one value for every job (pass_grid) or sized per job so that every block
+walks about the same number of particle groups (groups_per_block: jobs of
+different sizes then end their pass together instead of the largest
+job's blocks running on alone).  The autotuner (tune.py) picks between
+these, and the number of turns per graph, by measurement.
 """
 import ctypes as C
 
@@ -32,7 +36,7 @@ from .serve import job_bytes
 class BatchedScan(object):
 
     def __init__(self, jobs, n_streams=8, chunk_turns=100, graph_turns=10, record_stats=False,
-                 big_job_bytes=128 << 20, mode='fused', pass_grid=0):
+                 big_job_bytes=128 << 20, mode='fused', pass_grid=0, groups_per_block=0):
         """jobs: list of (one_turn_map, bunch); the one-turn maps must be
         distinct objects (each keeps its wake history)."""
         if mode not in ('fused', 'streams'):
@@ -44,6 +48,10 @@ class BatchedScan(object):
         self.graph_turns = int(graph_turns)
         self.chunk_turns = int(chunk_turns)
         self._groups = None
+        self._pass_grid = int(pass_grid)
+        self.groups_per_block = 0
+        if groups_per_block:
+            self.configure(groups_per_block=groups_per_block)
         if mode == 'streams':
             self.streams = [torch.cuda.Stream() for _ in range(max(1, n_streams))]
             ns, k = len(self.streams), 0
@@ -58,14 +66,30 @@ class BatchedScan(object):
     # -- settings the tuner changes ------------------------------------------
     @property
     def pass_grid(self):
-        return self.engines[0].pass_grid
+        """The pass grid of every job (0: as many blocks as stay resident);
+        unused while groups_per_block is set."""
+        return self._pass_grid
 
-    def configure(self, pass_grid=None, graph_turns=None):
-        """Change the pass grid (every job) or the turns per graph; the
-        batches and graphs are built again at the next call."""
+    @staticmethod
+    def grid_for(n, groups_per_block):
+        """The pass grid that gives each block about groups_per_block of an
+        n-macroparticle bunch's particle groups."""
+        group = _phx.lib().phx_pass_group_size()
+        groups = -(-int(n) // group)
+        return max(1, min(_phx.MAX_GRID, -(-groups // int(groups_per_block))))
+
+    def configure(self, pass_grid=None, graph_turns=None, groups_per_block=None):
+        """Change the pass grids (one value for every job, or sized per job
+        by groups_per_block > 0) or the turns per graph; the batches and
+        graphs are built again at the next call."""
         if pass_grid is not None:
-            for eng in self.engines:
-                eng.pass_grid = int(pass_grid)
+            self._pass_grid = int(pass_grid)
+        if groups_per_block is not None:
+            self.groups_per_block = int(groups_per_block)
+        if pass_grid is not None or groups_per_block is not None:
+            for eng, b in zip(self.engines, self.bunches):
+                eng.pass_grid = (self.grid_for(b.macroparticlenumber, self.groups_per_block)
+                                 if self.groups_per_block > 0 else self._pass_grid)
         if graph_turns is not None:
             self.graph_turns = int(graph_turns)
             for eng in self.engines:

@@ -64,6 +64,8 @@ extern "C" int64_t phx_workspace_bytes(int64_t n, int32_t n_slices) {
                    acc_parts_bytes(H > 0 ? H : 1, n_slices, PHX_MAX_TERMS));
 }
 
+extern "C" int32_t phx_pass_group_size(void) { return PHX_CHUNK; }
+
 extern "C" int64_t phx_sizeof(const char *name) {
   if (!name) return -1;
 #define PHX_SZ(T) \

@@ -67,6 +67,28 @@ def test_batch_groups_of_different_passes():
     _check(scan, alone, [1, 3, 'rf0', 'rf1'])
 
 
+def test_per_job_grids_equal_each_job_alone():
+    """Pass grids sized per job (groups_per_block): every job bit-identical
+    to its run alone with its own grid."""
+    print("\n[case] 5 bunches, pass grids sized per job (2 groups per block), 21 turns",
+          flush=True)
+    import bench
+    from pyheadtail_amd.batch import BatchedScan
+    from pyheadtail_amd.engine import FusedOneTurnMap
+    jobs = [bench.scan_job(i, n_scale=0.02) for i in IDS]
+    grids = [BatchedScan.grid_for(b.macroparticlenumber, 2) for _, b in jobs]
+    assert len(set(grids)) > 1
+    alone = []
+    for i, g in zip(IDS, grids):
+        m, b = bench.scan_job(i, n_scale=0.02)
+        FusedOneTurnMap(m, graph_turns=4, pass_grid=g).track_turns(b, 21)
+        alone.append({k: getattr(b, k).cpu().numpy() for k in COORDS})
+    scan = BatchedScan(jobs, graph_turns=4, groups_per_block=2)
+    assert [e.pass_grid for e in scan.engines] == grids
+    scan.track_turns(21)
+    _check(scan, alone, IDS)
+
+
 def test_autotuner_stores_and_reuses(tmp_path):
     print("\n[case] autotuner: measured choice stored, reused, results as alone", flush=True)
     import bench
@@ -75,14 +97,22 @@ def test_autotuner_stores_and_reuses(tmp_path):
     path = str(tmp_path / "tune.json")
     scan = BatchedScan([bench.scan_job(i, n_scale=0.02) for i in IDS], graph_turns=4)
     entry = tune_batch(scan, tuner=Tuner(path), turns=4, pass_grids=(0, 128),
-                       graph_turns=(4, 8))
-    assert len(entry['candidates']) == 4
+                       groups_per_block=(3,), graph_turns=(4, 8))
+    assert len(entry['candidates']) == 6
     assert scan.pass_grid == entry['pass_grid']
+    assert scan.groups_per_block == entry['groups_per_block']
     again = tune_batch(scan, tuner=Tuner(path))
     assert again == entry
-    # a fresh scan with the stored choice, against the jobs alone at that grid
+    # a fresh scan with the stored choice, against the jobs alone at its grids
     fresh = BatchedScan([bench.scan_job(i, n_scale=0.02) for i in IDS], graph_turns=4)
     tune_batch(fresh, tuner=Tuner(path))
     assert fresh.pass_grid == entry['pass_grid']
+    grids = [e.pass_grid for e in fresh.engines]
     fresh.track_turns(13)
-    _check(fresh, _alone(IDS, 13, 0.02, entry['pass_grid']), IDS)
+    alone = []
+    from pyheadtail_amd.engine import FusedOneTurnMap
+    for i, g in zip(IDS, grids):
+        m, b = bench.scan_job(i, n_scale=0.02)
+        FusedOneTurnMap(m, graph_turns=4, pass_grid=g).track_turns(b, 13)
+        alone.append({k: getattr(b, k).cpu().numpy() for k in COORDS})
+    _check(fresh, alone, IDS)
