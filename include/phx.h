@@ -50,6 +50,9 @@ enum {
                                     standalone calls (phx_bunch_sums,
                                     phx_slice_cuts): their fold order */
 #define PHX_MAX_GRID 2048        /* largest engine pass grid */
+#define PHX_ENGINE_MAX_N 536870911LL  /* macroparticles of one bunch in the fused
+                                         engine (coordinates addressed by 32-bit
+                                         byte offsets) */
 
 typedef void *phx_stream_t;
 
@@ -552,6 +555,12 @@ enum { PHX_KICKS_GENERIC = 0, PHX_KICKS_DXDY = 1, PHX_KICKS_PLATES = 2, PHX_KICK
    chromaticity then amplitude detuning (Synchrotron's order). */
 enum { PHX_DETS_GENERIC = 0, PHX_DETS_NONE = 1, PHX_DETS_C1 = 2, PHX_DETS_C1_AMP = 3,
        PHX_DETS_COUNT = 4 };
+/* Flag added to PHX_DETS_C1 / PHX_DETS_C1_AMP: every segment is smooth,
+   I = identity, J00 = J11 = J22 = J33 = 0 exactly and no dispersion (equal
+   optics at both ends with alpha = 0, as smooth-optics machines have), so
+   M00 = I00 cos + J00 sin is cos bit for bit and the segment needs only
+   J01, J10, J23, J32. */
+#define PHX_DETS_SMOOTH 8
 /* Wake convolution kernels: resistive-wall kicks in the dot-product form
    (multi-turn memory) have their own, RW_DOT for the VALU precisions and
    RW_MFMA for the matrix-core ones; every other kick is GENERAL. */

@@ -151,28 +151,41 @@ void stage_blocks(const EngineArgs &a, int (&nb)[BS_COUNT]) {
 }
 
 // The compiled batch passes: first-order chromaticity (with or without
-// amplitude detuning) after an RF system or a linear map, and the generic
-// interpreter for every other map (and for statistics records).
+// amplitude detuning; general or smooth segments) after an RF system or a
+// linear map, and the generic interpreter for every other map (and for
+// statistics records).
 template <bool REC, int SH, int DS>
 void launch_batch_pass(const Batch &B, hipStream_t s) {
   k_batch_pass<REC, SH, DS><<<dim3((unsigned)B.total[BS_PASS]), PHX_BLOCK, B.lds_pass, s>>>(
       B.args());
 }
 
+template <int SH>
+bool batch_pass_dets(const Batch &B, hipStream_t s) {
+  switch (B.dets) {
+    case PHX_DETS_C1: launch_batch_pass<false, SH, PHX_DETS_C1>(B, s); return true;
+    case PHX_DETS_C1_AMP: launch_batch_pass<false, SH, PHX_DETS_C1_AMP>(B, s); return true;
+    case PHX_DETS_C1 | PHX_DETS_SMOOTH:
+      launch_batch_pass<false, SH, PHX_DETS_C1 | PHX_DETS_SMOOTH>(B, s);
+      return true;
+    case PHX_DETS_C1_AMP | PHX_DETS_SMOOTH:
+      launch_batch_pass<false, SH, PHX_DETS_C1_AMP | PHX_DETS_SMOOTH>(B, s);
+      return true;
+    default: return false;
+  }
+}
+
 int batch_pass(const Batch &B, hipStream_t s) {
+  bool done = false;
   if (B.rec) {
     launch_batch_pass<true, PHX_SHAPE_GENERIC, PHX_DETS_GENERIC>(B, s);
-  } else if (B.shape == PHX_SHAPE_RF1_T && B.dets == PHX_DETS_C1_AMP) {
-    launch_batch_pass<false, PHX_SHAPE_RF1_T, PHX_DETS_C1_AMP>(B, s);
-  } else if (B.shape == PHX_SHAPE_RF1_T && B.dets == PHX_DETS_C1) {
-    launch_batch_pass<false, PHX_SHAPE_RF1_T, PHX_DETS_C1>(B, s);
-  } else if (B.shape == PHX_SHAPE_LIN_T && B.dets == PHX_DETS_C1) {
-    launch_batch_pass<false, PHX_SHAPE_LIN_T, PHX_DETS_C1>(B, s);
-  } else if (B.shape == PHX_SHAPE_LIN_T && B.dets == PHX_DETS_C1_AMP) {
-    launch_batch_pass<false, PHX_SHAPE_LIN_T, PHX_DETS_C1_AMP>(B, s);
-  } else {
-    launch_batch_pass<false, PHX_SHAPE_GENERIC, PHX_DETS_GENERIC>(B, s);
+    done = true;
+  } else if (B.shape == PHX_SHAPE_RF1_T) {
+    done = batch_pass_dets<PHX_SHAPE_RF1_T>(B, s);
+  } else if (B.shape == PHX_SHAPE_LIN_T) {
+    done = batch_pass_dets<PHX_SHAPE_LIN_T>(B, s);
   }
+  if (!done) launch_batch_pass<false, PHX_SHAPE_GENERIC, PHX_DETS_GENERIC>(B, s);
   PHX_CHECK_LAUNCH("phx_batch: pass");
   return PHX_OK;
 }

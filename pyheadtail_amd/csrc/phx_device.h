@@ -147,7 +147,7 @@ __device__ __forceinline__ void detune_set(const phx_tseg &t, const P6 &p, doubl
   const phx_detuner &c = t.det[0];
   dx = dx + c.cx[0] * p.dp;
   dy = dy + c.cy[0] * p.dp;
-  if constexpr (DS == PHX_DETS_C1_AMP) {
+  if constexpr ((DS & ~PHX_DETS_SMOOTH) == PHX_DETS_C1_AMP) {
     double qx, qy;
     detune_amplitude(t.det[1], p, qx, qy);
     dx = dx + qx;
@@ -162,6 +162,7 @@ __device__ __forceinline__ void detune_set(const phx_tseg &t, const P6 &p, doubl
 template <int N, int DS = PHX_DETS_GENERIC>
 __device__ __forceinline__ void apply_tseg_n(P6 (&p)[N], const phx_tseg &t) {
   double M00[N], M01[N], M10[N], M11[N], M22[N], M23[N], M32[N], M33[N];
+  constexpr bool SMOOTH = (DS & PHX_DETS_SMOOTH) != 0 && (DS & ~PHX_DETS_SMOOTH) >= PHX_DETS_C1;
   const int nd = DS == PHX_DETS_GENERIC ? t.n_detuners : (DS == PHX_DETS_NONE ? 0 : 1);
   if (nd == 0) {
 #pragma unroll
@@ -202,6 +203,19 @@ __device__ __forceinline__ void apply_tseg_n(P6 (&p)[N], const phx_tseg &t) {
       double sx, cx, sy, cy;
       phx_sincos(ax, &sx, &cx);
       phx_sincos(ay, &sy, &cy);
+      if constexpr (SMOOTH) {
+        // 1 * cos + 0 * sin and 0 * cos + J * sin, each rounded once: the
+        // same values
+        M00[k] = cx;
+        M01[k] = J01 * sx;
+        M10[k] = J10 * sx;
+        M11[k] = cx;
+        M22[k] = cy;
+        M23[k] = J23 * sy;
+        M32[k] = J32 * sy;
+        M33[k] = cy;
+        continue;
+      }
       M00[k] = I00 * cx + J00 * sx;
       M01[k] = I01 * cx + J01 * sx;
       M10[k] = I10 * cx + J10 * sx;
@@ -212,7 +226,7 @@ __device__ __forceinline__ void apply_tseg_n(P6 (&p)[N], const phx_tseg &t) {
       M33[k] = I33 * cy + J33 * sy;
     }
   }
-  const bool disp = t.with_dispersion != 0;
+  const bool disp = !SMOOTH && t.with_dispersion != 0;
   const double dx0 = t.D_x_s0, dy0 = t.D_y_s0, dx1 = t.D_x_s1, dy1 = t.D_y_s1;
 #pragma unroll
   for (int k = 0; k < N; ++k) {

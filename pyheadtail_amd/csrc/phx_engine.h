@@ -178,10 +178,7 @@ __device__ __forceinline__ void kick_pending(P6 &p, const phx_turn_program &g, i
   if (f >= 0.0 && f < (double)S) apply_kick_set<KS>(p, (int32_t)f, S, g.kick, nk, kicks);
 }
 
-// Particles k*PHX_BLOCK + t (k < PHX_PPT) of the group starting at the
-// six pointers c (uniform: scalar base plus the thread's 32-bit offset).
-// lim: the group's particle count if it is the partial last group, else
-// PHX_CHUNK (no guard needed).
+// The six coordinate arrays of a pass.
 struct GroupPtr {
   double *c[6];
 };
@@ -218,26 +215,46 @@ __device__ __forceinline__ void st_mom(double *p, double v) {
 #endif
 }
 
-__device__ __forceinline__ void group_load(const GroupPtr &gp, int lim, P6 (&p)[PHX_PPT]) {
+// Coordinates addressed as the array's base (the kernel arguments, kept in
+// scalar registers) plus one 32-bit byte offset per particle, shared by the
+// six arrays: per-group pointers to six arrays, current and next group,
+// would take 24 scalar registers across the particle loop (bunches up to
+// PHX_ENGINE_MAX_N keep every offset below 2^32).
+__device__ __forceinline__ const double *at_off(const double *b, uint32_t off) {
+  return reinterpret_cast<const double *>(reinterpret_cast<const char *>(b) + off);
+}
+__device__ __forceinline__ double *at_off(double *b, uint32_t off) {
+  return reinterpret_cast<double *>(reinterpret_cast<char *>(b) + off);
+}
+
+// The PHX_PPT particles of this thread in the group starting at byte offset
+// goff; lim: the group's particle count if it is the partial last group,
+// else PHX_CHUNK (no guard needed).
+__device__ __forceinline__ void group_load(const GroupPtr &gb, uint32_t goff, int lim,
+                                           P6 (&p)[PHX_PPT]) {
 #pragma unroll
   for (int k = 0; k < PHX_PPT; ++k) {
     const int o = (int)threadIdx.x + k * PHX_BLOCK;
+    const uint32_t off = goff + (uint32_t)o * 8u;
     if (lim == PHX_CHUNK || o < lim)
-      p[k] = P6{ld_pos(gp.c[0] + o), ld_mom(gp.c[1] + o), ld_pos(gp.c[2] + o),
-                ld_mom(gp.c[3] + o), ld_pos(gp.c[4] + o), ld_mom(gp.c[5] + o)};
+      p[k] = P6{ld_pos(at_off(gb.c[0], off)), ld_mom(at_off(gb.c[1], off)),
+                ld_pos(at_off(gb.c[2], off)), ld_mom(at_off(gb.c[3], off)),
+                ld_pos(at_off(gb.c[4], off)), ld_mom(at_off(gb.c[5], off))};
     else
       p[k] = P6{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   }
 }
 
-__device__ __forceinline__ void group_store(const GroupPtr &gp, int lim,
+__device__ __forceinline__ void group_store(const GroupPtr &gb, uint32_t goff, int lim,
                                             const P6 (&p)[PHX_PPT]) {
 #pragma unroll
   for (int k = 0; k < PHX_PPT; ++k) {
     const int o = (int)threadIdx.x + k * PHX_BLOCK;
+    const uint32_t off = goff + (uint32_t)o * 8u;
     if (lim == PHX_CHUNK || o < lim) {
-      gp.c[0][o] = p[k].x; st_mom(gp.c[1] + o, p[k].xp); gp.c[2][o] = p[k].y;
-      st_mom(gp.c[3] + o, p[k].yp); gp.c[4][o] = p[k].z; st_mom(gp.c[5] + o, p[k].dp);
+      *at_off(gb.c[0], off) = p[k].x; st_mom(at_off(gb.c[1], off), p[k].xp);
+      *at_off(gb.c[2], off) = p[k].y; st_mom(at_off(gb.c[3], off), p[k].yp);
+      *at_off(gb.c[4], off) = p[k].z; st_mom(at_off(gb.c[5], off), p[k].dp);
     }
   }
 }
@@ -311,18 +328,19 @@ __device__ __forceinline__ void pass_body(
   const int nk = g.n_kicks < a.n_kicks ? g.n_kicks : a.n_kicks;
   const int64_t ng = phx_n_groups(n);
   // the group's base pointers and particle count (uniform)
-  auto group = [&](int64_t gg, GroupPtr &gp) {
+  const GroupPtr gb{{x, xp, y, yp, z, dp}};
+  // the group's byte offset and particle count (uniform)
+  auto group = [&](int64_t gg, uint32_t &off) {
     const int64_t b = gg * PHX_CHUNK;
-    gp.c[0] = x + b; gp.c[1] = xp + b; gp.c[2] = y + b;
-    gp.c[3] = yp + b; gp.c[4] = z + b; gp.c[5] = dp + b;
+    off = (uint32_t)b * 8u;
     const int64_t rest = n - b;
     return rest >= PHX_CHUNK ? PHX_CHUNK : (int)rest;
   };
   int64_t gi = bid;
-  GroupPtr gc, gnx;
+  uint32_t gc, gnx = 0;
   int lc = group(gi, gc);
   P6 cur[PHX_PPT], nxt[PHX_PPT];
-  group_load(gc, lc, cur);
+  group_load(gb, gc, lc, cur);
   // Particle 0 sets the shifts of the shifted sums (stats.pyx:cov): its
   // state after the pending kick (statistics), and its z after this turn's
   // elements (cuts), which needs the kick only where a kick changes dp and
@@ -354,7 +372,7 @@ __device__ __forceinline__ void pass_body(
     int ln = 0;
     if (gn < ng) {
       ln = group(gn, gnx);
-      group_load(gnx, ln, nxt);
+      group_load(gb, gnx, ln, nxt);
     }
     if (pend) {
 #pragma unroll
@@ -392,7 +410,7 @@ __device__ __forceinline__ void pass_body(
         }
       }
     }
-    if (MODE == 0 || pend) group_store(gc, lc, cur);
+    if (MODE == 0 || pend) group_store(gb, gc, lc, cur);
 #pragma unroll
     for (int k = 0; k < PHX_PPT; ++k) cur[k] = nxt[k];
     gc = gnx;

@@ -208,8 +208,15 @@ int check_engine(const phx_engine *e, int *n_kicks, int *T) {
   if (e->kick_shape == PHX_KICKS_PLATES && e->n_kicks != 4)
     return phx::fail(PHX_E_INVALID, "phx_engine: kick_shape PLATES needs 4 kicks, not %d",
                      e->n_kicks);
-  if (e->det_shape < 0 || e->det_shape >= PHX_DETS_COUNT)
-    return phx::fail(PHX_E_INVALID, "phx_engine: det_shape %d", e->det_shape);
+  {
+    const int ds = e->det_shape & ~PHX_DETS_SMOOTH;
+    if (ds < 0 || ds >= PHX_DETS_COUNT || (ds < PHX_DETS_C1 && ds != e->det_shape))
+      return phx::fail(PHX_E_INVALID, "phx_engine: det_shape %d", e->det_shape);
+  }
+  // the particle pass addresses a coordinate by a 32-bit byte offset
+  if (e->n > PHX_ENGINE_MAX_N)
+    return phx::fail(PHX_E_LIMIT, "phx_engine: %lld macroparticles, the fused engine takes at "
+                     "most %lld per bunch", (long long)e->n, (long long)PHX_ENGINE_MAX_N);
   if (e->n_slices > 0 && (!e->hist || !e->z_bins || !e->kicks || !e->counts || !e->slice_means))
     return phx::fail(PHX_E_INVALID, "phx_engine: wake buffers null");
   if (e->sc_stages < 0 || e->sc_stages > PHX_MAX_SC_STAGES)
@@ -280,6 +287,10 @@ PassKernel pick_dets(int ds) {
       case PHX_DETS_NONE: return pass_kernel<MODE, R, W, SH, KS, PHX_DETS_NONE>();
       case PHX_DETS_C1: return pass_kernel<MODE, R, W, SH, KS, PHX_DETS_C1>();
       case PHX_DETS_C1_AMP: return pass_kernel<MODE, R, W, SH, KS, PHX_DETS_C1_AMP>();
+      case PHX_DETS_C1 | PHX_DETS_SMOOTH:
+        return pass_kernel<MODE, R, W, SH, KS, PHX_DETS_C1 | PHX_DETS_SMOOTH>();
+      case PHX_DETS_C1_AMP | PHX_DETS_SMOOTH:
+        return pass_kernel<MODE, R, W, SH, KS, PHX_DETS_C1_AMP | PHX_DETS_SMOOTH>();
       default: break;
     }
   }
@@ -370,6 +381,9 @@ ScPassKernel scpass_dets(int ds) {
     case PHX_DETS_NONE: return scpass_kernel<SH, PHX_DETS_NONE>();
     case PHX_DETS_C1: return scpass_kernel<SH, PHX_DETS_C1>();
     case PHX_DETS_C1_AMP: return scpass_kernel<SH, PHX_DETS_C1_AMP>();
+    case PHX_DETS_C1 | PHX_DETS_SMOOTH: return scpass_kernel<SH, PHX_DETS_C1 | PHX_DETS_SMOOTH>();
+    case PHX_DETS_C1_AMP | PHX_DETS_SMOOTH:
+      return scpass_kernel<SH, PHX_DETS_C1_AMP | PHX_DETS_SMOOTH>();
     default: return scpass_kernel<SH, PHX_DETS_GENERIC>();
   }
 }

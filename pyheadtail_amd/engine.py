@@ -240,7 +240,20 @@ class FusedOneTurnMap(Element):
             dets = {(): _phx.DETS_NONE, ((_phx.DET_CHROMA, 1),): _phx.DETS_C1,
                     ((_phx.DET_CHROMA, 1), (_phx.DET_AMPLITUDE, 0)): _phx.DETS_C1_AMP
                     }.get(FusedOneTurnMap._det_key(prog.tseg[0]), _phx.DETS_GENERIC)
+            if dets in (_phx.DETS_C1, _phx.DETS_C1_AMP) and all(
+                    FusedOneTurnMap._smooth(prog.tseg[i]) for i in range(prog.n_tseg)):
+                dets |= _phx.DETS_SMOOTH
         return shape, kicks, dets
+
+    @staticmethod
+    def _smooth(tseg):
+        """A segment record whose matrices give M00 = cos etc. exactly: I the
+        identity, J00 = J11 = J22 = J33 = 0 and no dispersion (alpha = 0 and
+        equal beta at both ends, as in smooth optics)."""
+        return (tseg.I00 == 1.0 and tseg.I11 == 1.0 and tseg.I22 == 1.0 and tseg.I33 == 1.0 and
+                tseg.I01 == 0.0 and tseg.I10 == 0.0 and tseg.I23 == 0.0 and tseg.I32 == 0.0 and
+                tseg.J00 == 0.0 and tseg.J11 == 0.0 and tseg.J22 == 0.0 and tseg.J33 == 0.0 and
+                tseg.with_dispersion == 0)
 
     @classmethod
     def _stage_shapes(cls, progs, wake):

@@ -243,6 +243,29 @@ def _shape_case_machines():
     }
 
 
+@pytest.mark.parametrize("case", ['RF1_T', 'LIN_T', 'T_RF1_c1'])
+def test_smooth_segments_equal_general_segments(case, monkeypatch):
+    """The smooth-segment form (M00 = cos, M01 = J01 sin, ...) against the
+    general one (I cos + J sin) on the same smooth machine: bit-identical."""
+    _print_case("smooth vs general segment form: %s" % case)
+    from pyheadtail_amd import _phx
+    from pyheadtail_amd.engine import FusedOneTurnMap
+    make = _shape_case_machines()[case]
+    ob = W.nonlinear_bunch(20000, seed=22)
+    outs, dets = [], []
+    for smooth in (True, False):
+        if not smooth:
+            monkeypatch.setattr(FusedOneTurnMap, '_smooth', staticmethod(lambda t: False))
+        pb = to_product(ob)
+        eng = _fused(make(), graph_turns=0, pass_grid=_standalone_grid())
+        eng.track_turns(pb, 6)
+        outs.append(to_numpy(pb))
+        dets.append(eng._shape[2])
+    assert dets[0] & _phx.DETS_SMOOTH and not dets[1] & _phx.DETS_SMOOTH
+    for k in outs[0]:
+        np.testing.assert_array_equal(outs[0][k], outs[1][k], err_msg=k)
+
+
 @pytest.mark.parametrize("case", sorted(_shape_case_machines()))
 @pytest.mark.parametrize("rec", [False, True])
 def test_specialised_pass_equals_generic(case, rec):
@@ -268,6 +291,8 @@ def test_specialised_pass_equals_generic(case, rec):
     assert shapes[0][0] == getattr(_phx, 'SHAPE_' + base)
     want_dets = (_phx.DETS_NONE if 'nodet' in case else
                  _phx.DETS_C1 if '_c1' in case else _phx.DETS_C1_AMP)
+    if want_dets != _phx.DETS_NONE:
+        want_dets |= _phx.DETS_SMOOTH        # smooth optics: I = 1, J00 = J11 = 0
     assert shapes[0][2] == want_dets
     if case in ('RF2_T', 'T_RF1', 'T_LIN', 'LIN_T_nodet'):
         assert shapes[0][1] == _phx.KICKS_PLATES
