@@ -148,6 +148,48 @@ def test_sliceset_derived_data():
     assert ss.z_cut_tail == -0.8 and ss.z_cut_head == zb[-1]
 
 
+def _gaussian_nearest(a, sigma, truncate=4.0):
+    """A Gaussian filter written out from its definition: weights
+    exp(-k^2 / (2 sigma^2)) for |k| <= int(truncate * sigma + 0.5), divided by
+    their sum, the input extended at both ends by its edge values."""
+    r = int(truncate * sigma + 0.5)
+    k = np.arange(-r, r + 1, dtype=np.float64)
+    w = np.exp(-0.5 * (k / sigma) ** 2)
+    w /= w.sum()
+    ext = np.concatenate([np.full(r, a[0]), a, np.full(r, a[-1])])
+    return np.array([np.dot(w, ext[i:i + 2 * r + 1]) for i in range(len(a))])
+
+
+def test_lambda_bins_against_gaussian_filter_definition():
+    """SliceSet.lambda_bins (particles/slicing.py:229-237 of the reference):
+    charge per slice, smoothed by a nearest-mode Gaussian filter of width
+    0.02 * n_slices bins by default (:133-134)."""
+    _case("SliceSet.lambda_bins: raw and smoothed line charge")
+    from pyheadtail_amd.particles.slicing import UniformBinSlicer
+    b = uniform_bunch(20000)
+    for S in (7, 100, 400):
+        ss = b.get_slices(UniformBinSlicer(S, z_cuts=(-0.8, 0.6)))
+        counts = host(ss.n_macroparticles_per_slice).astype(np.float64)
+        raw = ss.lambda_bins(smoothen=False)
+        assert np.array_equal(raw, counts * b.charge_per_mp)
+        assert ss.smoothing_sigma == 0.02 * S
+        for sigma in (None, 1.5, 6.0):
+            sg = 0.02 * S if sigma is None else sigma
+            want = _gaussian_nearest(counts * b.charge_per_mp, sg)
+            got = ss.lambda_bins(sigma=sigma)
+            np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
+    # the written-out filter itself: a constant profile stays constant, and an
+    # impulse far from the ends keeps its total and spreads with variance
+    # sigma^2 (less the part beyond the 4-sigma truncation, 5e-4 of it)
+    flat = np.full(50, 3.0)
+    np.testing.assert_allclose(_gaussian_nearest(flat, 2.0), flat, rtol=1e-15)
+    imp = np.zeros(101)
+    imp[50] = 1.0
+    out = _gaussian_nearest(imp, 3.0)
+    k = np.arange(101) - 50.0
+    assert abs(out.sum() - 1.0) < 1e-15 and abs((out * k * k).sum() - 9.0) < 1e-3 * 9.0
+
+
 def test_slicer_statistics_emittance_without_dispersion():
     """tests/test_slicing.py:112-139 of the reference: with no dispersion the
     per-slice emittance equals the effective one; all statistics exist."""
