@@ -8,6 +8,17 @@
 // t = L tan(theta/2), theta = k pi / M, |k| < M = 2N.  With N = 40 it agrees
 // with scipy.special.wofz to about 2e-14 relative in the first quadrant
 // (the space-charge field evaluates w at |x| + i |y| only).
+//
+// p has real coefficients, so p(Z) is evaluated in real arithmetic as the
+// remainder of p on division by the real quadratic Z^2 - r Z + s
+// (r = 2 Re Z, s = |Z|^2), whose root Z is.  With u_j the coefficient of
+// Z^j (a[N-1-j] below) and d = N - 1: c_d = u_d, c_(d-1) = u_(d-1) + r c_d,
+// c_j = u_j + r c_(j+1) - s c_(j+2) down to j = 1, and p(Z) = c_1 Z +
+// (u_0 - s c_2) (D. E. Knuth, The Art of Computer Programming, vol. 2,
+// 4.6.4).  Two FMAs per coefficient instead of a complex multiply-add's
+// four operations; its error against scipy.special.wofz is that of the
+// complex Horner form (the approximation's own, 2.1e-14 at most on the
+// accuracy test's sample).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -54,7 +65,7 @@ __device__ __forceinline__ void phx_wofz_n(const WofzCoeffs &c, const double (&x
                                            const double (&y)[N], double (&wr)[N],
                                            double (&wi)[N]) {
   const double L = c.L;
-  double qr[N], qi[N], Zr[N], Zi[N], pr[N], pi[N];
+  double qr[N], qi[N], Zr[N], Zi[N], r[N], s[N], c1[N], c2[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     // L - i z = (L + y) - i x,  L + i z = (L - y) + i x
@@ -69,26 +80,32 @@ __device__ __forceinline__ void phx_wofz_n(const WofzCoeffs &c, const double (&x
     const double nr = L - y[k], ni = x[k];
     Zr[k] = nr * qr[k] - ni * qi[k];
     Zi[k] = nr * qi[k] + ni * qr[k];
-    pr[k] = c.a[0];
-    pi[k] = 0.0;
+    // the quadratic with root Z; the two highest coefficients
+    r[k] = 2.0 * Zr[k];
+    s[k] = fma(Zr[k], Zr[k], Zi[k] * Zi[k]);
+    c2[k] = c.a[0];
+    c1[k] = fma(r[k], c2[k], c.a[1]);
   }
-#pragma unroll 4
-  for (int n = 1; n < PHX_WOFZ_N; ++n) {
+#pragma unroll
+  for (int n = 2; n < PHX_WOFZ_N - 1; ++n) {
     const double an = c.a[n];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-      const double tr = fma(pr[k], Zr[k], fma(-pi[k], Zi[k], an));
-      pi[k] = fma(pr[k], Zi[k], pi[k] * Zr[k]);
-      pr[k] = tr;
+      const double c0 = fma(r[k], c1[k], fma(-s[k], c2[k], an));
+      c2[k] = c1[k];
+      c1[k] = c0;
     }
   }
   // w = 2 p q^2 + q / sqrt(pi)
   const double isp = 0.56418958354775628695;  // 1/sqrt(pi)
 #pragma unroll
   for (int k = 0; k < N; ++k) {
+    // p(Z) = c_1 Z + (a_0 - s c_2)
+    const double pr = fma(c1[k], Zr[k], fma(-s[k], c2[k], c.a[PHX_WOFZ_N - 1]));
+    const double pi = c1[k] * Zi[k];
     const double q2r = qr[k] * qr[k] - qi[k] * qi[k], q2i = 2.0 * qr[k] * qi[k];
-    wr[k] = 2.0 * (pr[k] * q2r - pi[k] * q2i) + isp * qr[k];
-    wi[k] = 2.0 * (pr[k] * q2i + pi[k] * q2r) + isp * qi[k];
+    wr[k] = 2.0 * (pr * q2r - pi * q2i) + isp * qr[k];
+    wi[k] = 2.0 * (pr * q2i + pi * q2r) + isp * qi[k];
   }
 }
 

@@ -50,33 +50,32 @@ __device__ __forceinline__ ScField sc_field_params(double sx, double sy,
 
 // _efieldn_mit (:275-300) at x, y >= 0 with sig_x = s1 > sig_y = s2, for N
 // points in lockstep: the operation order of the reference's expression,
-// its divisions by the slice constants as div_rcp.
+// its divisions by the slice constants as div_rcp.  The two Faddeeva
+// arguments of every point are formed first and the 2N evaluations run in
+// lockstep (independent chains; each evaluation's arithmetic is unchanged).
 template <int N>
 __device__ __forceinline__ void sc_efieldn_mit_n(const WofzCoeffs &c, const double (&x)[N],
                                                  const double (&y)[N], const ScField (&f)[N],
                                                  double denom_pre, double (&ex_out)[N],
                                                  double (&ey_out)[N]) {
-  double u[N], v[N], w1r[N], w1i[N], w2r[N], w2i[N], ex[N];
-#pragma unroll
-  for (int k = 0; k < N; ++k) {
-    u[k] = div_rcp(x[k], f[k].ss, f[k].r_ss);
-    v[k] = div_rcp(y[k], f[k].ss, f[k].r_ss);
-  }
-  phx_wofz_n<N>(c, u, v, w1r, w1i);
+  // [0, N): x / sig_sqrt + i y / sig_sqrt;  [N, 2N): the scaled arguments
+  double u[2 * N], v[2 * N], wr[2 * N], wi[2 * N], ex[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     const double s1 = f[k].s1, s2 = f[k].s2, ss = f[k].ss;
+    u[k] = div_rcp(x[k], ss, f[k].r_ss);
+    v[k] = div_rcp(y[k], ss, f[k].r_ss);
     ex[k] = exp(div_rcp(-x[k] * x[k], 2.0 * s1 * s1, f[k].r_d1) +
                 div_rcp(-y[k] * y[k], 2.0 * s2 * s2, f[k].r_d2));
-    u[k] = div_rcp(x[k] * s2, s1 * ss, f[k].r_m1);
-    v[k] = div_rcp(y[k] * s1, s2 * ss, f[k].r_m2);
+    u[N + k] = div_rcp(x[k] * s2, s1 * ss, f[k].r_m1);
+    v[N + k] = div_rcp(y[k] * s1, s2 * ss, f[k].r_m2);
   }
-  phx_wofz_n<N>(c, u, v, w2r, w2i);
+  phx_wofz_n<2 * N>(c, u, v, wr, wi);
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     const double denom = denom_pre * f[k].ss;  // 2 eps0 sqrt(pi) sig_sqrt
-    ex_out[k] = div_rcp(w1i[k] - ex[k] * w2i[k], denom, f[k].r_den);
-    ey_out[k] = div_rcp(w1r[k] - ex[k] * w2r[k], denom, f[k].r_den);
+    ex_out[k] = div_rcp(wi[k] - ex[k] * wi[N + k], denom, f[k].r_den);
+    ey_out[k] = div_rcp(wr[k] - ex[k] * wr[N + k], denom, f[k].r_den);
   }
 }
 
