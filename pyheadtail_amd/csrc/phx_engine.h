@@ -178,6 +178,47 @@ __device__ __forceinline__ void kick_pending(P6 &p, const phx_turn_program &g, i
   if (f >= 0.0 && f < (double)S) apply_kick_set<KS>(p, (int32_t)f, S, g.kick, nk, kicks);
 }
 
+// The pending kick's table values for the PHX_PPT particles of a group,
+// read one group ahead of their use (the loads are otherwise a dependent
+// gather in the middle of each group's arithmetic).  For the kick sets
+// with straight-line code: DXDY (dipole x, y: 2 values per particle) and
+// PLATES (dipole and quadrupole x, y: 4).  The kicks do not move z, so the
+// slice index from the group's loaded z is the one kick_pending takes, and
+// apply() adds in apply_kick_set's order: the same arithmetic.
+template <int KS>
+struct KickAhead {
+  static constexpr int NV = KS == PHX_KICKS_DXDY ? 2 : (KS == PHX_KICKS_PLATES ? 4 : 0);
+  static constexpr bool ON = NV > 0;
+  double k[PHX_PPT][NV > 0 ? NV : 1];
+  bool in[PHX_PPT];
+  __device__ __forceinline__ void fetch(const P6 (&q)[PHX_PPT], double tail, double width,
+                                        double inv_width, int S, const double *kicks) {
+#pragma unroll
+    for (int j = 0; j < PHX_PPT; ++j) {
+      const double f = slice_floor_r(q[j].z, tail, width, inv_width);
+      in[j] = f >= 0.0 && f < (double)S;
+      const int s = in[j] ? (int)f : 0;  // slot 0 is read and not used outside
+#pragma unroll
+      for (int i = 0; i < NV; ++i) k[j][i] = kicks[(size_t)i * S + s];
+    }
+  }
+  __device__ __forceinline__ void apply(P6 (&q)[PHX_PPT]) const {
+#pragma unroll
+    for (int j = 0; j < PHX_PPT; ++j) {
+      if (!in[j]) continue;
+      if constexpr (KS == PHX_KICKS_DXDY) {
+        q[j].xp = q[j].xp + k[j][0];
+        q[j].yp = q[j].yp + k[j][1];
+      } else if constexpr (KS == PHX_KICKS_PLATES) {
+        q[j].xp = q[j].xp + k[j][0];
+        q[j].xp = q[j].xp + k[j][1] * q[j].x;
+        q[j].yp = q[j].yp + k[j][2];
+        q[j].yp = q[j].yp + k[j][3] * q[j].y;
+      }
+    }
+  }
+};
+
 // The six coordinate arrays of a pass.
 struct GroupPtr {
   double *c[6];
@@ -367,6 +408,8 @@ __device__ __forceinline__ void pass_body(
 #pragma unroll
   for (int i = 0; i < 6; ++i) shs[i] = src[i];
   const double zshift = src[6];
+  KickAhead<KS> ka;
+  if (KickAhead<KS>::ON && pend) ka.fetch(cur, tail, width, inv_width, S, kicks);
   for (; gi < ng; gi += nblk) {
     const int64_t gn = gi + nblk;
     int ln = 0;
@@ -375,9 +418,13 @@ __device__ __forceinline__ void pass_body(
       group_load(gb, gnx, ln, nxt);
     }
     if (pend) {
+      if constexpr (KickAhead<KS>::ON) {
+        ka.apply(cur);
+      } else {
 #pragma unroll
-      for (int k = 0; k < PHX_PPT; ++k)
-        kick_pending<KS>(cur[k], g, nk, tail, width, inv_width, S, kicks);
+        for (int k = 0; k < PHX_PPT; ++k)
+          kick_pending<KS>(cur[k], g, nk, tail, width, inv_width, S, kicks);
+      }
     }
     const bool full = lc == PHX_CHUNK;
     if (REC) {
@@ -415,6 +462,8 @@ __device__ __forceinline__ void pass_body(
     for (int k = 0; k < PHX_PPT; ++k) cur[k] = nxt[k];
     gc = gnx;
     lc = ln;
+    // the next group's kick values, while this block's other waves compute
+    if (KickAhead<KS>::ON && pend && gn < ng) ka.fetch(cur, tail, width, inv_width, S, kicks);
   }
   const double r = block_reduce_slots<L::NV, L::MINM, L::MAXM>(v, pass_lds);
   if (threadIdx.x < L::NV) w.partials[(size_t)threadIdx.x * nblk + bid] = r;
