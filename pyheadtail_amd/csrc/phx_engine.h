@@ -53,7 +53,7 @@ __host__ __device__ inline EngineWs engine_ws(void *ws, int64_t n, int S) {
   p += 256;
   const int H = hist_blocks(n);
   w.acc = acc_parts_layout(p, H, Se, 2);
-  p += ((size_t)H * Se * 4 + 255) / 256 * 256 + (size_t)H * 2 * Se * 16 + (size_t)2 * Se * 8;
+  p += acc_parts_bytes(H, Se, 2);
   w.conv_part = reinterpret_cast<double *>(p);
   return w;
 }

@@ -73,10 +73,25 @@ struct AccTerms {
 
 // Per-block partial histograms (no cross-block atomics: device-scope atomics
 // to one address from many blocks serialise, so each block stores its own
-// histogram and a slice-parallel kernel merges them).
-//   cnt[h][s] (u32), hl[h][t][s] = {hi, lo} (u64 pairs), fallback[t][s]
-//   (f64, only for non-finite or out-of-scale values, merged with FP64
-//   atomics).
+// histogram and a slice-parallel kernel merges them).  Stored in tiles of
+// ACC_TW consecutive slices, every tile holding all H partials, so that a
+// merge block (one or two tiles) reads only whole cache lines of its own:
+//   cnt[tile][h][ACC_TW] (u32), hl[tile][h][t][ACC_TW] = {hi, lo} (u64
+//   pairs; one 128-byte line per tile and partial for t = 2),
+//   fallback[t][s] (f64, only for non-finite or out-of-scale values, merged
+//   with FP64 atomics).
+#define ACC_TW 4
+
+__host__ __device__ inline size_t acc_cnt_index(int H, int h, int s) {
+  return ((size_t)(s / ACC_TW) * H + h) * ACC_TW + s % ACC_TW;
+}
+__host__ __device__ inline size_t acc_hl_index(int H, int T, int h, int i, int s) {
+  return (((size_t)(s / ACC_TW) * H + h) * T + i) * ACC_TW + s % ACC_TW;
+}
+__host__ __device__ inline size_t acc_tiled(int S) {  // S rounded up to whole tiles
+  return ((size_t)S + ACC_TW - 1) / ACC_TW * ACC_TW;
+}
+
 struct AccParts {
   unsigned int *cnt;
   ulonglong2 *hl;
@@ -95,18 +110,20 @@ inline size_t acc_partials_bytes(int64_t n) {
   return ((size_t)phx_pass_grid(n) * 64 * sizeof(double) + 255) / 256 * 256;
 }
 
-inline size_t acc_parts_bytes(int H, int S, int T) {
-  return ((size_t)H * S * 4 + 255) / 256 * 256 + (size_t)H * T * S * 16 + (size_t)T * S * 8;
+__host__ __device__ inline size_t acc_parts_bytes(int H, int S, int T) {
+  const size_t St = acc_tiled(S);
+  return ((size_t)H * St * 4 + 255) / 256 * 256 + (size_t)H * T * St * 16 + (size_t)T * S * 8;
 }
 
 __host__ __device__ inline AccParts acc_parts_layout(void *base, int H, int S, int T) {
+  const size_t St = acc_tiled(S);
   char *p = reinterpret_cast<char *>(base);
   AccParts a;
   a.H = H;
   a.cnt = reinterpret_cast<unsigned int *>(p);
-  p += ((size_t)H * S * 4 + 255) / 256 * 256;
+  p += ((size_t)H * St * 4 + 255) / 256 * 256;
   a.hl = reinterpret_cast<ulonglong2 *>(p);
-  p += (size_t)H * T * S * 16;
+  p += (size_t)H * T * St * 16;
   a.fallback = reinterpret_cast<double *>(p);
   return a;
 }
@@ -176,17 +193,23 @@ __device__ __forceinline__ void acc_add_lds(const AccLds &L, int S, const AccTer
 }
 
 // Store this block's LDS histogram as partial number h (every entry, zeros
-// included, so the merge needs no prior clearing).
+// included, so the merge needs no prior clearing; the slots of a last,
+// partly used tile are never read).
 __device__ __forceinline__ void acc_lds_store(const AccLds &L, int S, int T, const AccParts &P,
                                               int h) {
   lds_sync();
-  for (int s = threadIdx.x; s < S; s += blockDim.x) {
-    P.cnt[(size_t)h * S + s] = L.counts[s];
-    for (int i = 0; i < T; ++i) {
+  for (int s = threadIdx.x; s < S; s += blockDim.x) P.cnt[acc_cnt_index(P.H, h, s)] = L.counts[s];
+  // consecutive threads fill one tile's record [t][ACC_TW] after the other
+  const int rec = T * ACC_TW;
+  const int nq = (int)(acc_tiled(S) / ACC_TW) * rec;
+  for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+    const int r = q % rec, i = r / ACC_TW;
+    const int s = (q / rec) * ACC_TW + r % ACC_TW;
+    if (s < S) {
       ulonglong2 v;
       v.x = L.hi[(size_t)i * S + s];
       v.y = L.lo[(size_t)i * S + s];
-      P.hl[((size_t)h * T + i) * S + s] = v;
+      P.hl[acc_hl_index(P.H, T, h, i, s)] = v;
     }
   }
 }
@@ -196,8 +219,9 @@ __device__ __forceinline__ void acc_lds_store(const AccLds &L, int S, int T, con
 // slice t % SL and the partials h = t / SL, t / SL + G, ... (G = PHX_BLOCK /
 // SL); the G groups of a slice are then combined by lane exchanges within
 // each wave and across the waves in LDS.  The sums are integers, so the
-// order of the additions does not matter.  Threads 0 .. SL-1 receive the
-// totals of their slice.
+// order of the additions does not matter.  With the tiled layout and SL a
+// multiple of ACC_TW, the lanes of a wave read whole lines that no other
+// block reads.  Threads 0 .. SL-1 receive the totals of their slice.
 #define RD_SL 4
 #define RD_G (PHX_BLOCK / RD_SL)
 
@@ -220,10 +244,10 @@ __device__ __forceinline__ bool acc_merge(const AccParts &P, int S, SliceTotals 
   for (int i = 0; i < NVAL; ++i) v[i] = 0ULL;
   if (s < S) {
     for (int h = g; h < P.H; h += (PHX_BLOCK / SL)) {
-      v[0] += P.cnt[(size_t)h * S + s];
+      v[0] += P.cnt[acc_cnt_index(P.H, h, s)];
 #pragma unroll
       for (int i = 0; i < T; ++i) {
-        const ulonglong2 q = P.hl[((size_t)h * T + i) * S + s];
+        const ulonglong2 q = P.hl[acc_hl_index(P.H, T, h, i, s)];
         v[1 + 2 * i] += q.x;
         v[2 + 2 * i] += q.y;
       }
