@@ -542,6 +542,12 @@ enum {
     PHX_SHAPE_NONE = 8,       /* no elements (a space-charge stage of the kick alone) */
     PHX_SHAPE_COUNT = 9
 };
+/* Flag added to PHX_SHAPE_RF1_T: every program's RF sequence is stationary,
+   its drifts with beta_ratio = 1 and its kick with phi_offset = phi_lock =
+   p_increment = +0 and no dispersion.  The pass then drops those terms,
+   which leave the values unchanged (1 * z = z, (a + 0) + 0 = a + 0,
+   b - 0 = b): the same bits as the general form. */
+#define PHX_SHAPE_STATIONARY 16
 /* Space-charge kicks per turn of the fused engine. */
 #define PHX_MAX_SC_STAGES 16
 /* Wake kick sets with a compiled kick step (other sets: PHX_KICKS_GENERIC):
@@ -559,7 +565,9 @@ enum { PHX_DETS_GENERIC = 0, PHX_DETS_NONE = 1, PHX_DETS_C1 = 2, PHX_DETS_C1_AMP
    I = identity, J00 = J11 = J22 = J33 = 0 exactly and no dispersion (equal
    optics at both ends with alpha = 0, as smooth-optics machines have), so
    M00 = I00 cos + J00 sin is cos bit for bit and the segment needs only
-   J01, J10, J23, J32. */
+   J01, J10, J23, J32; an amplitude detuner then also has ax2 = ay2 = 0 and
+   gx, gy > 0, and its 2 alpha u u' term is dropped (a zero added to
+   gx u^2 >= +0). */
 #define PHX_DETS_SMOOTH 8
 /* Wake convolution kernels: resistive-wall kicks in the dot-product form
    (multi-turn memory) have their own, RW_DOT for the VALU precisions and

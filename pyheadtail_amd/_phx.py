@@ -47,6 +47,7 @@ EL_TSEG, EL_LONG, EL_LINMAP = 1, 2, 3
 KICKS_GENERIC, KICKS_DXDY, KICKS_PLATES = 0, 1, 2
 DETS_GENERIC, DETS_NONE, DETS_C1, DETS_C1_AMP = 0, 1, 2, 3
 DETS_SMOOTH = 8          # flag: every segment smooth (PHX_DETS_SMOOTH)
+SHAPE_STATIONARY = 16    # flag on SHAPE_RF1_T: stationary RF (PHX_SHAPE_STATIONARY)
 ENGINE_MAX_N = 536870911
 CONV_GENERAL, CONV_RW_DOT, CONV_RW_MFMA = 1, 2, 4
 (PREC_FP64, PREC_FP32, PREC_FP64_DIRECT, PREC_MFMA_F32, PREC_MFMA_BF16X3,

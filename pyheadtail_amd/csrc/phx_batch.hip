@@ -273,11 +273,13 @@ extern "C" int phx_batch_create(const phx_engine *engines, int32_t n, void *desc
       return phx::fail(PHX_E_INVALID,
                        "phx_batch_create: engine %d has space-charge stages (not batched)", i);
     }
+    // the batch kernels run the general RF code for stationary sequences
+    const int shape = e.program_shape & ~PHX_SHAPE_STATIONARY;
     if (i == 0) {
       B->rec = e.record_stats;
-      B->shape = e.program_shape;
+      B->shape = shape;
       B->dets = e.det_shape;
-    } else if (e.record_stats != B->rec || e.program_shape != B->shape || e.det_shape != B->dets) {
+    } else if (e.record_stats != B->rec || shape != B->shape || e.det_shape != B->dets) {
       delete B;
       return phx::fail(PHX_E_INVALID, "phx_batch_create: engine %d differs from engine 0 in "
                        "record_stats, program_shape or det_shape", i);

@@ -136,6 +136,18 @@ __device__ __forceinline__ void detune(const phx_detuner &d, const P6 &p, double
   }
 }
 
+// Smooth segments (PHX_DETS_SMOOTH) have alpha = 0, and the host sets the
+// flag only with ax2 = ay2 = 0 and gx, gy > 0: then (ax2 x) xp is a zero
+// added to gx x^2 >= +0, which leaves it unchanged, and the term is dropped
+// (the same bits for finite coordinates).
+__device__ __forceinline__ void detune_amplitude_smooth(const phx_detuner &d, const P6 &p,
+                                                        double &qx, double &qy) {
+  const double jx = 0.5 * (d.gx * (p.x * p.x) + d.bx * (p.xp * p.xp));
+  const double jy = 0.5 * (d.gy * (p.y * p.y) + d.by * (p.yp * p.yp));
+  qx = div_rcp(d.dapp_x * jx + d.dapp_xy * jy, d.p0, d.inv_p0);
+  qy = div_rcp(d.dapp_y * jy + d.dapp_yx * jx, d.p0, d.inv_p0);
+}
+
 // The detuner sets with straight-line code (PHX_DETS_*): first-order
 // chromaticity (order-1 Horner: c0 * dp), then amplitude detuning; the
 // arithmetic of detune() for those sets.
@@ -149,7 +161,8 @@ __device__ __forceinline__ void detune_set(const phx_tseg &t, const P6 &p, doubl
   dy = dy + c.cy[0] * p.dp;
   if constexpr ((DS & ~PHX_DETS_SMOOTH) == PHX_DETS_C1_AMP) {
     double qx, qy;
-    detune_amplitude(t.det[1], p, qx, qy);
+    if constexpr ((DS & PHX_DETS_SMOOTH) != 0) detune_amplitude_smooth(t.det[1], p, qx, qy);
+    else detune_amplitude(t.det[1], p, qx, qy);
     dx = dx + qx;
     dy = dy + qy;
   }
@@ -250,17 +263,39 @@ __device__ __forceinline__ void apply_tseg_n(P6 (&p)[N], const phx_tseg &t) {
 // ---------------------------------------------------------------------------
 // Longitudinal maps (trackers/longitudinal_tracking.py).
 // ---------------------------------------------------------------------------
-template <int N>
+// STAT: a stationary RF sequence (PHX_SHAPE_STATIONARY): beta_ratio = 1,
+// phi_offset = phi_lock = p_increment = +0 and no dispersion, checked by the
+// host, so that 1 * z = z, (a + 0) + 0 = a + 0 and b - 0 = b drop those
+// terms with the same bits.
+template <int N, bool STAT = false>
 __device__ __forceinline__ void drift_n(P6 (&p)[N], const phx_long_op &o) {
   // Drift.track: z = beta_ratio*z - eta(dp)*dp*length   (:107-111)
-  const double br = o.beta_ratio, eta0 = o.eta0, len = o.length;
+  const double eta0 = o.eta0, len = o.length;
+  if constexpr (STAT) {
 #pragma unroll
-  for (int k = 0; k < N; ++k) p[k].z = br * p[k].z - (eta0 * p[k].dp) * len;
+    for (int k = 0; k < N; ++k) p[k].z = p[k].z - (eta0 * p[k].dp) * len;
+  } else {
+    const double br = o.beta_ratio;
+#pragma unroll
+    for (int k = 0; k < N; ++k) p[k].z = br * p[k].z - (eta0 * p[k].dp) * len;
+  }
 }
 
-template <int N>
+template <int N, bool STAT = false>
 __device__ __forceinline__ void rf_kick_n(P6 (&p)[N], const phx_long_op &o) {
   // Kick.track_with_dispersion / track_without_dispersion (:185-203)
+  if constexpr (STAT) {
+    const double h = o.harmonic, tp = o.two_pi, C = o.circumference;
+    const double p0b = o.p0_before, p0a = o.p0_after, amp = o.amplitude;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      P6 &q = p[k];
+      const double phi = h * div_rcp(tp * q.z, C, o.inv_circumference) + 0.0;
+      const double delta_p = q.dp * p0b;
+      q.dp = div_rcp(delta_p + amp * phx_sin(phi), p0a, o.inv_p0_after);
+    }
+    return;
+  }
   const bool disp = o.with_dispersion != 0;
   const double Dx = o.D_x, Dy = o.D_y, h = o.harmonic, tp = o.two_pi, C = o.circumference;
   const double off = o.phi_offset, lock = o.phi_lock, p0b = o.p0_before, p0a = o.p0_after;
@@ -319,12 +354,12 @@ __device__ __forceinline__ void apply_long_seq_n(P6 (&p)[N], const phx_long_seq 
 
 // RFSystems' sequence with NH harmonics: op[0] drift, op[1..NH] kicks,
 // op[NH+1] drift.
-template <int NH, int N>
+template <int NH, int N, bool STAT = false>
 __device__ __forceinline__ void apply_rf_seq_n(P6 (&p)[N], const phx_long_seq &s) {
-  drift_n<N>(p, s.op[0]);
+  drift_n<N, STAT>(p, s.op[0]);
 #pragma unroll
-  for (int i = 1; i <= NH; ++i) rf_kick_n<N>(p, s.op[i]);
-  drift_n<N>(p, s.op[NH + 1]);
+  for (int i = 1; i <= NH; ++i) rf_kick_n<N, STAT>(p, s.op[i]);
+  drift_n<N, STAT>(p, s.op[NH + 1]);
   shrink_n<N>(p, s);
 }
 
@@ -362,27 +397,29 @@ __device__ __forceinline__ void apply_program_n(P6 (&p)[N], const phx_turn_progr
 // registers for far better than the interpreter's loop over element kinds.
 template <int SHAPE, int N, int DS = PHX_DETS_GENERIC>
 __device__ __forceinline__ void apply_shape_n(P6 (&p)[N], const phx_turn_program &g) {
-  if constexpr (SHAPE == PHX_SHAPE_RF1_T) {
-    apply_rf_seq_n<1, N>(p, g.lseq[0]);
+  constexpr int SH = SHAPE & ~PHX_SHAPE_STATIONARY;
+  constexpr bool ST = SH != SHAPE;
+  if constexpr (SH == PHX_SHAPE_RF1_T) {
+    apply_rf_seq_n<1, N, ST>(p, g.lseq[0]);
     apply_tseg_n<N, DS>(p, g.tseg[0]);
-  } else if constexpr (SHAPE == PHX_SHAPE_RF2_T) {
-    apply_rf_seq_n<2, N>(p, g.lseq[0]);
+  } else if constexpr (SH == PHX_SHAPE_RF2_T) {
+    apply_rf_seq_n<2, N, ST>(p, g.lseq[0]);
     apply_tseg_n<N, DS>(p, g.tseg[0]);
-  } else if constexpr (SHAPE == PHX_SHAPE_T_RF1) {
+  } else if constexpr (SH == PHX_SHAPE_T_RF1) {
     apply_tseg_n<N, DS>(p, g.tseg[0]);
-    apply_rf_seq_n<1, N>(p, g.lseq[0]);
-  } else if constexpr (SHAPE == PHX_SHAPE_T_RF2) {
+    apply_rf_seq_n<1, N, ST>(p, g.lseq[0]);
+  } else if constexpr (SH == PHX_SHAPE_T_RF2) {
     apply_tseg_n<N, DS>(p, g.tseg[0]);
-    apply_rf_seq_n<2, N>(p, g.lseq[0]);
-  } else if constexpr (SHAPE == PHX_SHAPE_LIN_T) {
+    apply_rf_seq_n<2, N, ST>(p, g.lseq[0]);
+  } else if constexpr (SH == PHX_SHAPE_LIN_T) {
     apply_linmap_n<N>(p, g.lin[0]);
     apply_tseg_n<N, DS>(p, g.tseg[0]);
-  } else if constexpr (SHAPE == PHX_SHAPE_T_LIN) {
+  } else if constexpr (SH == PHX_SHAPE_T_LIN) {
     apply_tseg_n<N, DS>(p, g.tseg[0]);
     apply_linmap_n<N>(p, g.lin[0]);
-  } else if constexpr (SHAPE == PHX_SHAPE_T) {
+  } else if constexpr (SH == PHX_SHAPE_T) {
     apply_tseg_n<N, DS>(p, g.tseg[0]);
-  } else if constexpr (SHAPE == PHX_SHAPE_NONE) {
+  } else if constexpr (SH == PHX_SHAPE_NONE) {
   } else {
     apply_program_n<N>(p, g);
   }
@@ -393,14 +430,16 @@ __device__ __forceinline__ void apply_shape_n(P6 (&p)[N], const phx_turn_program
 // the same z and dp bit for bit.
 template <int SHAPE>
 __device__ __forceinline__ void apply_shape_longitudinal(P6 &q, const phx_turn_program &g) {
+  constexpr int SH = SHAPE & ~PHX_SHAPE_STATIONARY;
+  constexpr bool ST = SH != SHAPE;
   P6(&p)[1] = *reinterpret_cast<P6(*)[1]>(&q);
-  if constexpr (SHAPE == PHX_SHAPE_RF1_T || SHAPE == PHX_SHAPE_T_RF1) {
-    apply_rf_seq_n<1, 1>(p, g.lseq[0]);
-  } else if constexpr (SHAPE == PHX_SHAPE_RF2_T || SHAPE == PHX_SHAPE_T_RF2) {
-    apply_rf_seq_n<2, 1>(p, g.lseq[0]);
-  } else if constexpr (SHAPE == PHX_SHAPE_LIN_T || SHAPE == PHX_SHAPE_T_LIN) {
+  if constexpr (SH == PHX_SHAPE_RF1_T || SH == PHX_SHAPE_T_RF1) {
+    apply_rf_seq_n<1, 1, ST>(p, g.lseq[0]);
+  } else if constexpr (SH == PHX_SHAPE_RF2_T || SH == PHX_SHAPE_T_RF2) {
+    apply_rf_seq_n<2, 1, ST>(p, g.lseq[0]);
+  } else if constexpr (SH == PHX_SHAPE_LIN_T || SH == PHX_SHAPE_T_LIN) {
     apply_linmap_n<1>(p, g.lin[0]);
-  } else if constexpr (SHAPE == PHX_SHAPE_GENERIC) {
+  } else if constexpr (SH == PHX_SHAPE_GENERIC) {
     for (int e = 0; e < g.n_el; ++e) {
       const int k = g.el_index[e];
       if (g.el_type[e] == PHX_EL_LONG) apply_long_seq_n<1>(p, g.lseq[k]);

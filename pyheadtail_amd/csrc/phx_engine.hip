@@ -193,8 +193,12 @@ int check_engine(const phx_engine *e, int *n_kicks, int *T) {
   if (e->n_slices > 0 && (e->n_kicks < 1 || e->hist_capacity < 1))
     return phx::fail(PHX_E_INVALID, "phx_engine: wake field needs kicks and history");
   if (e->record_stats && !e->stats) return phx::fail(PHX_E_INVALID, "phx_engine: stats null");
-  if (e->program_shape < 0 || e->program_shape >= PHX_SHAPE_COUNT)
-    return phx::fail(PHX_E_INVALID, "phx_engine: program_shape %d", e->program_shape);
+  {
+    const int base = e->program_shape & ~PHX_SHAPE_STATIONARY;
+    if (base < 0 || base >= PHX_SHAPE_COUNT ||
+        (base != e->program_shape && base != PHX_SHAPE_RF1_T))
+      return phx::fail(PHX_E_INVALID, "phx_engine: program_shape %d", e->program_shape);
+  }
   if (e->kick_shape < 0 || e->kick_shape >= PHX_KICKS_COUNT)
     return phx::fail(PHX_E_INVALID, "phx_engine: kick_shape %d", e->kick_shape);
   if (e->pass_grid < 0 || e->pass_grid > PHX_MAX_GRID)
@@ -312,6 +316,15 @@ template <int MODE, bool R, bool W>
 PassKernel pick_shape(int shape, int ks, int ds) {
   if constexpr (MODE == 0) {
     switch (shape) {
+      case PHX_SHAPE_RF1_T | PHX_SHAPE_STATIONARY:
+        // compiled for the wake pass with the straight-line kick sets;
+        // otherwise the general RF code, which gives the same bits
+        if constexpr (W && !R) {
+          constexpr int ST = PHX_SHAPE_RF1_T | PHX_SHAPE_STATIONARY;
+          if (ks == PHX_KICKS_DXDY) return pick_dets<MODE, R, W, ST, PHX_KICKS_DXDY>(ds);
+          if (ks == PHX_KICKS_PLATES) return pick_dets<MODE, R, W, ST, PHX_KICKS_PLATES>(ds);
+        }
+        return pick_kicks<MODE, R, W, PHX_SHAPE_RF1_T>(ks, ds);
       case PHX_SHAPE_RF1_T: return pick_kicks<MODE, R, W, PHX_SHAPE_RF1_T>(ks, ds);
       case PHX_SHAPE_RF2_T: return pick_kicks<MODE, R, W, PHX_SHAPE_RF2_T>(ks, ds);
       case PHX_SHAPE_T_RF1: return pick_kicks<MODE, R, W, PHX_SHAPE_T_RF1>(ks, ds);

@@ -23,6 +23,7 @@ programs; identical programs are stored once.  The wake kick of the last
 turn stays pending on the bunch until a coordinate is read (or flush()).
 """
 import ctypes as C
+import math
 
 import numpy as np
 import torch
@@ -243,17 +244,44 @@ class FusedOneTurnMap(Element):
             if dets in (_phx.DETS_C1, _phx.DETS_C1_AMP) and all(
                     FusedOneTurnMap._smooth(prog.tseg[i]) for i in range(prog.n_tseg)):
                 dets |= _phx.DETS_SMOOTH
+        if shape == _phx.SHAPE_RF1_T and FusedOneTurnMap._stationary(prog):
+            shape |= _phx.SHAPE_STATIONARY
         return shape, kicks, dets
+
+    @staticmethod
+    def _stationary(prog):
+        """The program's RF sequences have drifts with beta_ratio = 1 and
+        kicks with phi_offset = phi_lock = p_increment = +0 (sign bit clear)
+        and no dispersion: the terms PHX_SHAPE_STATIONARY drops leave the
+        values unchanged."""
+        for i in range(prog.n_long):
+            seq = prog.lseq[i]
+            for j in range(seq.n_ops):
+                o = seq.op[j]
+                if o.type == _phx.LONG_DRIFT:
+                    if o.beta_ratio != 1.0:
+                        return False
+                elif o.with_dispersion != 0 or any(
+                        v != 0.0 or math.copysign(1.0, v) < 0.0
+                        for v in (o.phi_offset, o.phi_lock, o.p_increment)):
+                    return False
+        return True
 
     @staticmethod
     def _smooth(tseg):
         """A segment record whose matrices give M00 = cos etc. exactly: I the
         identity, J00 = J11 = J22 = J33 = 0 and no dispersion (alpha = 0 and
-        equal beta at both ends, as in smooth optics)."""
+        equal beta at both ends, as in smooth optics); its amplitude
+        detuners with ax2 = ay2 = 0 and gx, gy > 0 (their 2 alpha u u' term
+        a zero added to gx u^2 >= +0)."""
+        amp_ok = all(tseg.det[i].ax2 == 0.0 and tseg.det[i].ay2 == 0.0 and
+                     tseg.det[i].gx > 0.0 and tseg.det[i].gy > 0.0
+                     for i in range(tseg.n_detuners)
+                     if tseg.det[i].type == _phx.DET_AMPLITUDE)
         return (tseg.I00 == 1.0 and tseg.I11 == 1.0 and tseg.I22 == 1.0 and tseg.I33 == 1.0 and
                 tseg.I01 == 0.0 and tseg.I10 == 0.0 and tseg.I23 == 0.0 and tseg.I32 == 0.0 and
                 tseg.J00 == 0.0 and tseg.J11 == 0.0 and tseg.J22 == 0.0 and tseg.J33 == 0.0 and
-                tseg.with_dispersion == 0)
+                tseg.with_dispersion == 0 and amp_ok)
 
     @classmethod
     def _stage_shapes(cls, progs, wake):
@@ -268,7 +296,8 @@ class FusedOneTurnMap(Element):
                 dets.add(ds)
         kicks = cls._shapes(progs[0], wake)[1]
         ds = dets.pop() if len(dets) == 1 else _phx.DETS_GENERIC
-        return (shapes[0], kicks, ds), shapes[1:]
+        # the stage passes run the general RF code (no stationary variant)
+        return (shapes[0], kicks, ds), [sh & ~_phx.SHAPE_STATIONARY for sh in shapes[1:]]
 
     @staticmethod
     def _det_key(tseg):
@@ -306,6 +335,9 @@ class FusedOneTurnMap(Element):
             if key not in index:
                 index[key] = len(progs)
                 progs.append(key)
+                # every program must be stationary for the stationary pass
+                if self._shape[0] & _phx.SHAPE_STATIONARY and not self._stationary(p):
+                    self._shape = (self._shape[0] & ~_phx.SHAPE_STATIONARY,) + self._shape[1:]
             table[t] = index[key] * n_st
             t += 1
             if self._state_key(state) == before:

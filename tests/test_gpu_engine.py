@@ -206,10 +206,10 @@ def _shape_case_machines():
                                                  ParallelHorizontalPlatesResonator)
     from scipy.constants import e, m_p
 
-    def rf(nh):
+    def rf(nh, phase=0.):
         h = [W.H_RF, 4 * W.H_RF][:nh]
         v = [W.V_RF, 0.1 * W.V_RF][:nh]
-        return RFSystems(W.CIRCUMFERENCE, h, v, [0., np.pi][:nh], [W.ALPHA_C], W.GAMMA,
+        return RFSystems(W.CIRCUMFERENCE, h, v, [phase, np.pi][:nh], [W.ALPHA_C], W.GAMMA,
                          p_increment=0., charge=e, mass=m_p)
 
     def tseg(dets='c1_amp'):
@@ -240,6 +240,7 @@ def _shape_case_machines():
         'LIN_T_nodet': lambda: [lin(), tseg('none'), wake(True)],
         'T_RF1_c1': lambda: [tseg('c1'), rf(1), wake(False)],
         'RF1_T_c1_nowake': lambda: [rf(1), tseg('c1')],
+        'RF1_T_phase': lambda: [rf(1, 0.3), tseg(), wake(False)],
     }
 
 
@@ -266,6 +267,31 @@ def test_smooth_segments_equal_general_segments(case, monkeypatch):
         np.testing.assert_array_equal(outs[0][k], outs[1][k], err_msg=k)
 
 
+@pytest.mark.parametrize("case", ['RF1_T', 'RF1_T_c1_nowake'])
+def test_stationary_rf_equals_general_rf(case, monkeypatch):
+    """The stationary RF form (no beta_ratio, phase offsets, momentum
+    increment or dispersion terms) against the general one on the same
+    stationary machine: bit-identical."""
+    _print_case("stationary vs general RF form: %s" % case)
+    from pyheadtail_amd import _phx
+    from pyheadtail_amd.engine import FusedOneTurnMap
+    make = _shape_case_machines()[case]
+    ob = W.nonlinear_bunch(20000, seed=23)
+    outs, shapes = [], []
+    for stat in (True, False):
+        if not stat:
+            monkeypatch.setattr(FusedOneTurnMap, '_stationary', staticmethod(lambda p: False))
+        pb = to_product(ob)
+        eng = _fused(make(), graph_turns=0, pass_grid=_standalone_grid())
+        eng.track_turns(pb, 6)
+        outs.append(to_numpy(pb))
+        shapes.append(eng._shape[0])
+    assert shapes[0] == _phx.SHAPE_RF1_T | _phx.SHAPE_STATIONARY
+    assert shapes[1] == _phx.SHAPE_RF1_T
+    for k in outs[0]:
+        np.testing.assert_array_equal(outs[0][k], outs[1][k], err_msg=k)
+
+
 @pytest.mark.parametrize("case", sorted(_shape_case_machines()))
 @pytest.mark.parametrize("rec", [False, True])
 def test_specialised_pass_equals_generic(case, rec):
@@ -287,8 +313,12 @@ def test_specialised_pass_equals_generic(case, rec):
         if rec:
             stats.append(eng.last_stats['raw'])
     assert shapes[1] == (_phx.SHAPE_GENERIC, _phx.KICKS_GENERIC, _phx.DETS_GENERIC)
-    base = case.replace('_nowake', '').replace('_nodet', '').replace('_c1', '')
-    assert shapes[0][0] == getattr(_phx, 'SHAPE_' + base)
+    base = (case.replace('_nowake', '').replace('_nodet', '').replace('_c1', '')
+            .replace('_phase', ''))
+    want_shape = getattr(_phx, 'SHAPE_' + base)
+    if base == 'RF1_T' and 'phase' not in case:
+        want_shape |= _phx.SHAPE_STATIONARY       # phi = 0, no acceleration, D = 0
+    assert shapes[0][0] == want_shape
     want_dets = (_phx.DETS_NONE if 'nodet' in case else
                  _phx.DETS_C1 if '_c1' in case else _phx.DETS_C1_AMP)
     if want_dets != _phx.DETS_NONE:
