@@ -144,9 +144,10 @@ void stage_blocks(const EngineArgs &a, int (&nb)[BS_COUNT]) {
   nb[BS_HIST] = hist_blocks(e.n);
   nb[BS_MERGE] = (S + merge_sl(S) - 1) / merge_sl(S);
   nb[BS_CONV] = (ck == 0 || (ck & PHX_CONV_GENERAL)) ? (S + WK_TILE - 1) / WK_TILE * a.n_kicks * a.T : 0;
-  nb[BS_RW] = (ck == 0 || (ck & PHX_CONV_RW_DOT)) ? rw_tiles * a.T : 0;
-  nb[BS_RWMF] = (ck == 0 || (ck & PHX_CONV_RW_MFMA)) ? rw_tiles * a.T : 0;
-  nb[BS_RED] = a.T > 1 ? (a.n_kicks * S + 255) / 256 : 0;
+  nb[BS_RW] = (ck == 0 || (ck & PHX_CONV_RW_DOT)) ? rw_tiles * conv_rows(S, a.T) : 0;
+  nb[BS_RWMF] = (ck == 0 || (ck & PHX_CONV_RW_MFMA)) ? rw_tiles * conv_rows(S, a.T) : 0;
+  const bool rw = ck == 0 || (ck & (PHX_CONV_RW_DOT | PHX_CONV_RW_MFMA));
+  nb[BS_RED] = conv_reduces(S, a.T, rw) ? (a.n_kicks * S + 255) / 256 : 0;
   nb[BS_SNAP] = 1;
 }
 

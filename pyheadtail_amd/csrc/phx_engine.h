@@ -32,7 +32,7 @@ struct EngineWs {
   double *partials;   // [EN_NV][pass grid] (slot-major)
   double *p0;         // particle 0 before the pass: the shifts of its sums
   AccParts acc;       // H partial histograms, T = 2 (x, y)
-  double *conv_part;  // [n_kicks][T][S]
+  double *conv_part;  // [n_kicks][conv_rows(S, T)][S]
 };
 
 __host__ __device__ inline int hist_blocks(int64_t n) {
@@ -62,7 +62,7 @@ inline size_t engine_ws_bytes(int64_t n, int S, int n_kicks, int T) {
   const int64_t nc = phx_grid_for(n, PHX_MAX_GRID);
   const int Se = S > 0 ? S : 1;
   size_t b = round256((size_t)nc * EN_NV * 8) + 256 + acc_parts_bytes(hist_blocks(n), Se, 2);
-  b += (size_t)(n_kicks > 0 ? n_kicks : 1) * T * Se * 8;
+  b += (size_t)(n_kicks > 0 ? n_kicks : 1) * conv_rows(Se, T) * Se * 8;
   return round256(b);
 }
 

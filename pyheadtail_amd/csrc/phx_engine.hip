@@ -516,16 +516,17 @@ int enqueue_stage(const EngineArgs &a, int stage, bool has_wake, hipStream_t s) 
     PHX_CHECK_LAUNCH("phx_engine: conv");
   }
   if (e.conv_kinds == 0 || (e.conv_kinds & PHX_CONV_RW_DOT)) {
-    k_engine_conv_rw<0><<<dim3((e.n_slices + RW_TILE - 1) / RW_TILE * a.T), RW_THREADS,
+    k_engine_conv_rw<0><<<dim3((e.n_slices + RW_TILE - 1) / RW_TILE * conv_rows(e.n_slices, a.T)), RW_THREADS,
                           wake_rw_lds_bytes(e.n_slices), s>>>(a);
     PHX_CHECK_LAUNCH("phx_engine: conv resistive wall");
   }
   if (e.conv_kinds == 0 || (e.conv_kinds & PHX_CONV_RW_MFMA)) {
-    k_engine_conv_rw<1><<<dim3((e.n_slices + RW_TILE - 1) / RW_TILE * a.T), RW_THREADS,
+    k_engine_conv_rw<1><<<dim3((e.n_slices + RW_TILE - 1) / RW_TILE * conv_rows(e.n_slices, a.T)), RW_THREADS,
                           wake_rw_lds_bytes(e.n_slices), s>>>(a);
     PHX_CHECK_LAUNCH("phx_engine: conv resistive wall (matrix cores)");
   }
-  if (a.T > 1) {
+  const bool rw = e.conv_kinds == 0 || (e.conv_kinds & (PHX_CONV_RW_DOT | PHX_CONV_RW_MFMA));
+  if (conv_reduces(e.n_slices, a.T, rw)) {
     const int tot = a.n_kicks * e.n_slices;
     k_engine_conv_reduce<<<dim3((tot + 255) / 256), 256, 0, s>>>(a);
     PHX_CHECK_LAUNCH("phx_engine: conv reduce");

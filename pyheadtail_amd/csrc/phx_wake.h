@@ -126,21 +126,23 @@ __device__ __forceinline__ double wake_eval(const phx_wake_kick &w, double dt,
 }
 
 // Kicks whose functions share the shape: same function, same parameters
-// apart from the Yokoya factor, same memory and convolution form.
+// apart from the Yokoya factor, same memory and convolution form.  Every
+// field is compared (no early exit), so that the loads of a scan over the
+// kick table issue together rather than one after the other.
 __device__ __forceinline__ bool same_shape(const phx_wake_kick &a, const phx_wake_kick &b) {
-  if (a.fn != b.fn || a.n_turns_wake != b.n_turns_wake || a.dot_product != b.dot_product ||
-      a.precision != b.precision)
-    return false;
-  if (a.fn == PHX_WAKE_RES_TRANSVERSE)
-    return a.Q == b.Q && a.alpha == b.alpha && a.omegabar == b.omegabar;
-  if (a.fn == PHX_WAKE_RES_LONGITUDINAL)
-    return a.Q == b.Q && a.alpha == b.alpha && a.omegabar == b.omegabar &&
-           a.alpha_over_omegabar == b.alpha_over_omegabar;
-  if (a.fn == PHX_WAKE_TABLE)
-    return a.table_t == b.table_t && a.table_w == b.table_w && a.table_n == b.table_n &&
-           a.table_kind == b.table_kind;
-  return a.L == b.L && a.b3 == b.b3 && a.dt_min_abs == b.dt_min_abs && a.k_sigma == b.k_sigma &&
-         a.sqrt_z0c == b.sqrt_z0c;
+  const bool head = (a.fn == b.fn) & (a.n_turns_wake == b.n_turns_wake) &
+                    (a.dot_product == b.dot_product) & (a.precision == b.precision);
+  const bool res = (a.Q == b.Q) & (a.alpha == b.alpha) & (a.omegabar == b.omegabar);
+  const bool lon = res & (a.alpha_over_omegabar == b.alpha_over_omegabar);
+  const bool tab = (a.table_t == b.table_t) & (a.table_w == b.table_w) &
+                   (a.table_n == b.table_n) & (a.table_kind == b.table_kind);
+  const bool rw = (a.L == b.L) & (a.b3 == b.b3) & (a.dt_min_abs == b.dt_min_abs) &
+                  (a.k_sigma == b.k_sigma) & (a.sqrt_z0c == b.sqrt_z0c);
+  const bool body = a.fn == PHX_WAKE_RES_TRANSVERSE     ? res
+                    : a.fn == PHX_WAKE_RES_LONGITUDINAL ? lon
+                    : a.fn == PHX_WAKE_TABLE            ? tab
+                                                        : rw;
+  return head & body;
 }
 
 __device__ __forceinline__ void rw_prefixes(const phx_wake_kick &w, double beta_src,
@@ -215,6 +217,19 @@ struct WakeKicks {
   phx_wake_kick k[PHX_MAX_KICKS];
 };
 
+// Copy of the first nk kicks into dst (LDS) by every thread of the block,
+// then a barrier.  The blocks scan the table for the members of a shape; in
+// global memory (the engine's turn programs) each step of such a scan waits
+// for a load from L2, in LDS for an LDS read.
+static_assert(sizeof(phx_wake_kick) % sizeof(uint64_t) == 0, "kick records are 8-byte words");
+__device__ __forceinline__ void stage_kicks(const WakeKicks &src, int nk, WakeKicks &dst) {
+  constexpr int words = (int)(sizeof(phx_wake_kick) / sizeof(uint64_t));
+  const uint64_t *s = reinterpret_cast<const uint64_t *>(src.k);
+  uint64_t *d = reinterpret_cast<uint64_t *>(dst.k);
+  for (int i = threadIdx.x; i < nk * words; i += blockDim.x) d[i] = s[i];
+  lds_sync();
+}
+
 // Resistive-wall kicks in the dot-product form have a kernel of their own
 // (wake_rw_block); the general block serves every other kick.
 __device__ __host__ __forceinline__ bool is_rw_dot(const phx_wake_kick &k) {
@@ -223,9 +238,33 @@ __device__ __host__ __forceinline__ bool is_rw_dot(const phx_wake_kick &k) {
 
 #define WK_GMAX 4  // kicks one block serves
 
+// Rows of partial sums per kick.  The sources of the current turn, whose
+// pairs straddle dt = -|dt_min| and are summed pair by pair at every
+// precision, are split over rw_chunks(S) blocks per target tile (up to
+// RW_CHUNK_MAX of them, each with a row of its own), so that this turn runs on
+// as many compute units as the older ones; the split depends on S alone, so
+// the sums do not depend on the memory depth T of the launch.  Row r of a
+// resistive-wall kick in the dot-product form: r < nc, chunk r of turn 0;
+// r >= nc, turn r - nc + 1.  Every other kick: row t, turn t (with T = 1 the
+// general block writes the kick itself).  The reduction adds a kick's rows in
+// order; a resistive-wall kick with a single row (S <= RW_CHUNK, T = 1) is
+// written by its block.
+#define RW_CHUNK 128
+#define RW_CHUNK_MAX 16
+__host__ __device__ inline int rw_chunks(int S) {
+  const int c = (S + RW_CHUNK - 1) / RW_CHUNK;
+  return c < 1 ? 1 : (c < RW_CHUNK_MAX ? c : RW_CHUNK_MAX);
+}
+__host__ __device__ inline int conv_rows(int S, int T) { return T + rw_chunks(S) - 1; }
+// Whether the launch needs the reduction: kicks of the general block with
+// memory (T > 1), or resistive-wall kicks with more than one row.
+__host__ __device__ inline bool conv_reduces(int S, int T, bool rw) {
+  return T > 1 || (rw && rw_chunks(S) > 1);
+}
+
 // Block (tile, ki, t) of nk kick rows: blockIdx.(x, y, z) and gridDim.y of
 // the standalone launch, explicit in a batch.
-__device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const WakeKicks &kk,
+__device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const WakeKicks &table,
                                                 int S, double *part, int T, double *out,
                                                 int gmax, int tile, int ki, int t, int nk) {
   // LDS, convolution form: [WK_TILE + S - 1] wake values, then [S] source
@@ -233,6 +272,8 @@ __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const Wa
   extern __shared__ double vbuf[];
   __shared__ double red[WK_GMAX][WK_THREADS];
   __shared__ double rwp[WK_GMAX][3];
+  __shared__ WakeKicks kk;
+  stage_kicks(table, nk, kk);
   const phx_wake_kick &w = kk.k[ki];
   if (is_rw_dot(w)) return;  // wake_rw_block's
   const int nf = h->n_filled, head = h->head, cap = h->capacity;
@@ -308,12 +349,12 @@ __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const Wa
       for (int q = 1; q < WK_GROUPS; ++q) r = r + red[m][q * WK_TILE + jj];
       const int km = mem[m];
       if (T == 1) out[(size_t)km * S + j] = kk.k[km].wake_factor * (0.0 + r);
-      else part[((size_t)km * T + t) * S + j] = r;
+      else part[((size_t)km * conv_rows(S, T) + t) * S + j] = r;
     }
   }
 }
 
-// Sum of the per-turn convolutions in turn order, times the wake factor.
+// Sum of a kick's rows of partial sums in order, times the wake factor.
 __device__ __forceinline__ void wake_conv_reduce(const phx_wake_hist *h, const WakeKicks &kk,
                                                  int n_kicks, int S, const double *part, int T,
                                                  double *out, int bid) {
@@ -322,18 +363,20 @@ __device__ __forceinline__ void wake_conv_reduce(const phx_wake_hist *h, const W
   const int ki = idx / S, j = idx % S;
   const phx_wake_kick &w = kk.k[ki];
   const int nf = h->n_filled;
+  if (T == 1 && !is_rw_dot(w)) return;  // written by its block
   const int n_turns = nf < w.n_turns_wake ? nf : w.n_turns_wake;
-  const double *p = part + (size_t)ki * T * S + j;
+  const int rows = is_rw_dot(w) && n_turns > 0 ? n_turns + rw_chunks(S) - 1 : n_turns;
+  const double *p = part + (size_t)ki * conv_rows(S, T) * S + j;
   double acc = 0.0;
   int q = 0;
-  for (; q + 8 <= n_turns; q += 8) {
+  for (; q + 8 <= rows; q += 8) {
     double v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = p[(size_t)(q + r) * S];
 #pragma unroll
     for (int r = 0; r < 8; ++r) acc = acc + v[r];
   }
-  for (; q < n_turns; ++q) acc = acc + p[(size_t)q * S];
+  for (; q < rows; ++q) acc = acc + p[(size_t)q * S];
   out[(size_t)ki * S + j] = w.wake_factor * acc;
 }
 
@@ -608,16 +651,46 @@ __device__ __forceinline__ double wave_min_nan(double v, int &bad) {
   return v;
 }
 
-#define RW_STATIC_LDS (RW_WAVES * (4 * sizeof(double) + sizeof(int)))
-// Dynamic LDS of wake_rw_block: the sources, then (reused) the wave sums of
-// one moment at a time.
+// Static LDS of wake_rw_block: the kick table, each kick's coefficient, group
+// (first kick of its shape), moment mask and sum slot, and the waves' extremes.
+#define RW_STATIC_LDS                                                       \
+  (sizeof(WakeKicks) + PHX_MAX_KICKS * (sizeof(double) + 3 * sizeof(int)) + \
+   RW_WAVES * (4 * sizeof(double) + sizeof(int)))
+// Parts of the source range in the power sums of the past-turn expansion
+// (one thread each).
+#define RW_UP_PARTS 128
+#define RW_UP_SUMS (3 * RW_EXP_K)
+#define RW_UP_LD (RW_UP_PARTS + 1)  // row stride of the parts (LDS banks)
+// Reduction area: the waves' sums of one moment (pair loops), or the parts of
+// the power sums and then the sums themselves (expansion).
+#define RW_RED_DOUBLES                                                        \
+  (RW_WAVES * RW_TILE > RW_UP_SUMS * (RW_UP_LD + 1) ? RW_WAVES * RW_TILE      \
+                                                     : RW_UP_SUMS * (RW_UP_LD + 1))
+// Dynamic LDS of wake_rw_block: the source turn's times and its three
+// moments (staged once for every group of the block), then the reduction
+// area.
 inline size_t wake_rw_lds_bytes(int S) {
-  const size_t src = (size_t)4 * S, red = (size_t)RW_WAVES * RW_TILE;
-  return (src > red ? src : red) * sizeof(double);
+  return ((size_t)4 * S + (size_t)RW_RED_DOUBLES) * sizeof(double);
 }
 
+// What wake_rw_block stages for its groups: the sources [sb, sb + n) of
+// source turn t (ring slot `slot`) at the start of the dynamic LDS (times,
+// then moments n, n x, n y, S apart), the lane's RW_JT targets as
+// t_target - age, and the extremes of the sources and of the tile's targets
+// over the block, with a NaN flag.
+struct RwStage {
+  int t, row, n, head;
+  double age;
+  double tja[RW_JT];
+  double mn, smax, tgmin, tgmax;
+  int bad;
+};
+
 // The group of kick ki (its first kick) for targets [tile * RW_TILE, +RW_TILE)
-// and source turn t; called by all threads of the block.
+// and the staged sources, into row st.row of R; called by all threads of the
+// block.  mask: the moments the group reads; per kick k, coef[k] (pre0
+// sqrt(-k_sigma) sqrt(Z0 c)), klead[k] (its group's first kick) and kslot[k]
+// (the rank of its moment in the group's mask).
 // KM: the kernel's kind, 0 for the VALU precisions (FP64 with the past-turn
 // expansion, FP64 pair by pair, FP32), 1 for the matrix-core ones; each
 // kind is its own kernel, so that neither carries the other's registers.
@@ -628,16 +701,12 @@ __host__ __device__ __forceinline__ bool rw_is_mfma(int precision) {
 
 template <int KM>
 __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const WakeKicks &kk, int nk,
-                                              int S, double *part, int T, double *out, int ki,
-                                              int tile, int t) {
+                                              int S, double *part, int R, double *out, int ki,
+                                              int tile, const RwStage &st, unsigned mask,
+                                              const double *coef, const int *klead,
+                                              const int *kslot) {
   extern __shared__ double rw_lds[];
-  __shared__ double wmin[RW_WAVES], wmax[RW_WAVES], wtmax[RW_WAVES], wtmin[RW_WAVES];
-  __shared__ int wbad[RW_WAVES];
   const phx_wake_kick &w = kk.k[ki];
-  const int head = h->head, cap = h->capacity;
-  unsigned mask = 0;                     // moments the group reads
-  for (int k = ki; k < nk; ++k)
-    if (same_shape(kk.k[k], w)) mask |= 1u << kk.k[k].moment;
   const int nu = __popc(mask);
   const int mo0 = __ffs(mask) - 1;
   const int mo1 = nu > 1 ? __ffs(mask & ~(1u << mo0)) - 1 : mo0;
@@ -645,113 +714,68 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const Wake
 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int slot = ring_slot(head, t, cap);
-  const double age = h->age[slot];
-  const double *tsrc = h->times + (size_t)slot * S;
-  const double *mom = h->mom + (size_t)slot * 3 * S;
-  double *ts = rw_lds, *m0 = rw_lds + S, *m1 = rw_lds + 2 * S, *m2 = rw_lds + 3 * S;
-  int bad = 0;
-  double mn = tsrc[0], mxs = tsrc[0];
-  for (int i = threadIdx.x; i < S; i += RW_THREADS) {
-    const double v = tsrc[i];
-    ts[i] = v;
-    bad |= v != v;
-    mn = v < mn ? v : mn;
-    mxs = v > mxs ? v : mxs;
-    m0[i] = mom[(size_t)mo0 * S + i];
-    if (nu > 1) m1[i] = mom[(size_t)mo1 * S + i];
-    if (nu > 2) m2[i] = mom[(size_t)mo2 * S + i];
-  }
-  // dt = (t_target - age) - t_source: the age leaves the pair loop
-  double tja[RW_JT];
-  double mx = -INFINITY, mnt = INFINITY;
-#pragma unroll
-  for (int q = 0; q < RW_JT; ++q) {
-    const int j = tile * RW_TILE + q * 64 + lane;
-    tja[q] = h->times[(size_t)head * S + (j < S ? j : S - 1)] - age;
-    bad |= tja[q] != tja[q];
-    mx = tja[q] > mx ? tja[q] : mx;
-    mnt = tja[q] < mnt ? tja[q] : mnt;
-  }
-  mn = wave_min_nan(mn, bad);
-  mxs = -wave_min_nan(-mxs, bad);
-  const double tmax = -wave_min_nan(-mx, bad);
-  const double tmin = wave_min_nan(mnt, bad);
-  const int wave_bad = __ballot(bad) != 0ULL;
-  if (lane == 0) {
-    wmin[wave] = mn;
-    wmax[wave] = mxs;
-    wtmax[wave] = tmax;
-    wtmin[wave] = tmin;
-    wbad[wave] = wave_bad;
-  }
-  lds_sync();  // the sources, wave extremes and NaN flags are the block's
-  bad = 0;
-  double smax = mxs, tgmax = tmax, tgmin = tmin;
-  for (int q = 0; q < RW_WAVES; ++q) {
-    mn = wmin[q] < mn ? wmin[q] : mn;
-    smax = wmax[q] > smax ? wmax[q] : smax;
-    tgmax = wtmax[q] > tgmax ? wtmax[q] : tgmax;
-    tgmin = wtmin[q] < tgmin ? wtmin[q] : tgmin;
-    bad |= wbad[q];
-  }
+  const int n = st.n;
+  const double *ts = rw_lds;
+  const double *m0 = rw_lds + (size_t)(1 + mo0) * S, *m1 = rw_lds + (size_t)(1 + mo1) * S,
+               *m2 = rw_lds + (size_t)(1 + mo2) * S;
+  double *red = rw_lds + (size_t)4 * S;
   // the extremes of the tile's targets: every wave holds all of them
-  const bool uniform = !bad && (tgmax - mn) + w.dt_min_abs < 0.0;
+  const bool uniform = !st.bad && (st.tgmax - st.mn) + w.dt_min_abs < 0.0;
   const double dmin = w.dt_min_abs;
-  double *red = rw_lds;  // [wave][RW_TILE], reused after the sources
   double tot[3] = {0.0, 0.0, 0.0};
   // the past-turn expansion (FP64 precision): centres, their distance A and
   // the largest relative offset r = (max|u| + max|v|) / A
-  const double c_s = 0.5 * (mn + smax), c_t = 0.5 * (tgmin + tgmax);
+  const double c_s = 0.5 * (st.mn + st.smax), c_t = 0.5 * (st.tgmin + st.tgmax);
   const double A = c_s - c_t;
-  const double r_off = ((smax - c_s) + (tgmax - c_t)) / A;
-  const bool expand = w.precision == PHX_PREC_FP64 && uniform && t > 0 && A > 0.0 &&
+  const double r_off = ((st.smax - c_s) + (st.tgmax - c_t)) / A;
+  const bool expand = w.precision == PHX_PREC_FP64 && uniform && st.t > 0 && A > 0.0 &&
                       r_off <= RW_EXP_R;
   if (KM == 0 && expand) {
-    // U_p of each distinct moment: every thread sums its sources in order,
-    // then the lanes (fixed butterfly) and the waves in order
-    double U[3][RW_EXP_K] = {};
+    // U_p of each distinct moment: thread r < P sums part r of the sources in
+    // source order, every power of every moment (term m x^p as ((m x) x)...);
+    // thread q < 3 K then adds the parts of sum q in order
     const double inv_A = 1.0 / A;
-    for (int i = threadIdx.x; i < S; i += RW_THREADS) {
-      const double x = (ts[i] - c_s) * inv_A;
-      const double mm[3] = {m0[i], nu > 1 ? m1[i] : 0.0, nu > 2 ? m2[i] : 0.0};
+    if (threadIdx.x < RW_UP_PARTS) {
+      const int len = (n + RW_UP_PARTS - 1) / RW_UP_PARTS;
+      const int ib = min((int)threadIdx.x * len, n), ie = min(ib + len, n);
+      double U[3][RW_EXP_K] = {};
+      for (int i = ib; i < ie; ++i) {
+        const double x = (ts[i] - c_s) * inv_A;
+        const double mm[3] = {m0[i], nu > 1 ? m1[i] : 0.0, nu > 2 ? m2[i] : 0.0};
 #pragma unroll
-      for (int u = 0; u < 3; ++u) {
-        double pw = mm[u];
+        for (int u = 0; u < 3; ++u) {
+          double pw = mm[u];
 #pragma unroll
-        for (int p = 0; p < RW_EXP_K; ++p) {
-          U[u][p] = U[u][p] + pw;
-          pw = pw * x;
+          for (int p = 0; p < RW_EXP_K; ++p) {
+            U[u][p] = U[u][p] + pw;
+            pw = pw * x;
+          }
         }
       }
-    }
-#pragma unroll
-    for (int u = 0; u < 3; ++u)
-#pragma unroll
-      for (int p = 0; p < RW_EXP_K; ++p)
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) U[u][p] = U[u][p] + __shfl_xor(U[u][p], off);
-    lds_sync();  // every wave is done with the sources
-    if (lane == 0) {
 #pragma unroll
       for (int u = 0; u < 3; ++u)
 #pragma unroll
-        for (int p = 0; p < RW_EXP_K; ++p) red[(wave * 3 + u) * RW_EXP_K + p] = U[u][p];
+        for (int p = 0; p < RW_EXP_K; ++p)
+          red[(u * RW_EXP_K + p) * RW_UP_LD + threadIdx.x] = U[u][p];
+    }
+    lds_sync();
+    double *usum = red + RW_UP_SUMS * RW_UP_LD;
+    if (threadIdx.x < RW_UP_SUMS) {
+      const double *pr = red + threadIdx.x * RW_UP_LD;
+      double v = pr[0];
+#pragma unroll 8
+      for (int q = 1; q < RW_UP_PARTS; ++q) v = v + pr[q];
+      usum[threadIdx.x] = v;
     }
     lds_sync();
     if (threadIdx.x < RW_TILE) {
-      const int jt = tile * RW_TILE + threadIdx.x;
-      const double tj = h->times[(size_t)head * S + (jt < S ? jt : S - 1)] - age;
+      const double tj = wave == 0 ? st.tja[0] : st.tja[RW_JT - 1];
       const double ny = -((tj - c_t) * inv_A);
       const double scale = 1.0 / sqrt(A);
       for (int u = 0; u < nu; ++u) {
         double Up[RW_EXP_K];
 #pragma unroll
-        for (int p = 0; p < RW_EXP_K; ++p) {
-          double v = red[u * RW_EXP_K + p];
-          for (int q = 1; q < RW_WAVES; ++q) v = v + red[(q * 3 + u) * RW_EXP_K + p];
-          Up[p] = v;
-        }
+        for (int p = 0; p < RW_EXP_K; ++p) Up[p] = usum[u * RW_EXP_K + p];
         double f = 0.0;  // Horner in -y over q = K-1 .. 0
 #pragma unroll
         for (int q = RW_EXP_K - 1; q >= 0; --q) {
@@ -770,20 +794,19 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const Wake
 #pragma unroll
     for (int q = 0; q < RW_MF_QUADS; ++q) {
       const int jt = tile * RW_TILE + wave * 4 * RW_MF_QUADS + 4 * q + (lane & 3);
-      tq[q] = h->times[(size_t)head * S + (jt < S ? jt : S - 1)] - age;
+      tq[q] = h->times[(size_t)st.head * S + (jt < S ? jt : S - 1)] - st.age;
     }
     double accd[RW_MF_QUADS][4] = {};
     const int mf = w.precision == PHX_PREC_MFMA_F32 ? 0 : (w.precision == PHX_PREC_MFMA_BF16X3 ? 1 : 2);
     if (uniform) {
-      if (mf == 0) rw_mfma<true, 0>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
-      else if (mf == 1) rw_mfma<true, 1>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
-      else rw_mfma<true, 2>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
+      if (mf == 0) rw_mfma<true, 0>(ts, m0, m1, m2, n, nu, tq, dmin, accd);
+      else if (mf == 1) rw_mfma<true, 1>(ts, m0, m1, m2, n, nu, tq, dmin, accd);
+      else rw_mfma<true, 2>(ts, m0, m1, m2, n, nu, tq, dmin, accd);
     } else {
-      if (mf == 0) rw_mfma<false, 0>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
-      else if (mf == 1) rw_mfma<false, 1>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
-      else rw_mfma<false, 2>(ts, m0, m1, m2, S, nu, tq, dmin, accd);
+      if (mf == 0) rw_mfma<false, 0>(ts, m0, m1, m2, n, nu, tq, dmin, accd);
+      else if (mf == 1) rw_mfma<false, 1>(ts, m0, m1, m2, n, nu, tq, dmin, accd);
+      else rw_mfma<false, 2>(ts, m0, m1, m2, n, nu, tq, dmin, accd);
     }
-    lds_sync();  // every wave is done with the sources
     if (lane < 4) {
 #pragma unroll
       for (int q = 0; q < RW_MF_QUADS; ++q)
@@ -795,19 +818,19 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const Wake
     if (threadIdx.x < RW_TILE)
       for (int u = 0; u < nu; ++u) tot[u] = red[u * RW_TILE + threadIdx.x];
   } else {
-    const int chunk = (S + RW_WAVES - 1) / RW_WAVES;
-    const int ib = min(wave * chunk, S), ie = min(ib + chunk, S);
+    const int chunk = (n + RW_WAVES - 1) / RW_WAVES;
+    const int ib = min(wave * chunk, n), ie = min(ib + chunk, n);
     double acc[3][RW_JT] = {};
     if (w.precision == PHX_PREC_FP32)
-      rw_dispatch<true>(nu, uniform, ts, m0, m1, m2, ib, ie, tja, dmin, acc);
+      rw_dispatch<true>(nu, uniform, ts, m0, m1, m2, ib, ie, st.tja, dmin, acc);
     else
-      rw_dispatch<false>(nu, uniform, ts, m0, m1, m2, ib, ie, tja, dmin, acc);
-    // the sources' LDS takes the wave sums, one moment at a time; thread
+      rw_dispatch<false>(nu, uniform, ts, m0, m1, m2, ib, ie, st.tja, dmin, acc);
+    // the reduction area takes the wave sums, one moment at a time; thread
     // r < RW_TILE finishes target j0 + r
 #pragma unroll
     for (int u = 0; u < 3; ++u) {
       if (u >= nu) break;
-      lds_sync();
+      if (u > 0) lds_sync();
 #pragma unroll
       for (int q = 0; q < RW_JT; ++q) red[wave * RW_TILE + q * 64 + lane] = acc[u][q];
       lds_sync();
@@ -820,44 +843,135 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const Wake
   }
   const int j = tile * RW_TILE + threadIdx.x;
   if (threadIdx.x < RW_TILE && j < S) {
-    const double beta = h->beta[slot], root = sqrt(-w.k_sigma);
     for (int k = ki; k < nk; ++k) {
-      const phx_wake_kick &kw = kk.k[k];
-      if (!same_shape(kw, w)) continue;
-      const int u = __popc(mask & ((1u << kw.moment) - 1u));
+      if (klead[k] != ki) continue;
+      const int u = kslot[k];
       const double sum = u == 0 ? tot[0] : (u == 1 ? tot[1] : tot[2]);
-      const double v = ((rw_prefix0(kw, beta) * root) * kw.sqrt_z0c) * sum;
-      if (T == 1) out[(size_t)k * S + j] = kw.wake_factor * (0.0 + v);
-      else part[((size_t)k * T + t) * S + j] = v;
+      const double v = coef[k] * sum;
+      if (R == 1) out[(size_t)k * S + j] = kk.k[k].wake_factor * (0.0 + v);
+      else part[((size_t)k * R + st.row) * S + j] = v;
     }
   }
-  lds_sync();  // the LDS is the next group's
+  lds_sync();  // the reduction area is the next group's
 }
 
-// One block = (RW_TILE targets, source turn): blockIdx.x = tile + tiles *
-// turn, so that consecutive blocks (which go to different XCDs) cover
-// different turns.  The block serves every resistive-wall group (in kick
-// order) whose memory reaches turn t.
+// One block = (RW_TILE targets, row): blockIdx.x = tile + tiles * row
+// (conv_rows(S, T) rows: the chunks of turn 0, then turns 1..T-1), so that
+// consecutive blocks (which go to different XCDs) cover different rows.  The
+// block stages the row's sources once and serves every resistive-wall group
+// (in kick order) whose memory reaches the row's turn.  The loads that do
+// not depend on the kick table (the ring's state, then the sources, the
+// targets, the slot's age and beta) are issued before it is read, each
+// dependent load being a round trip to memory that the next one waits for.
 template <int KM>
-__device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const WakeKicks &kk, int nk,
-                                              int S, double *part, int T, double *out, int bid) {
+__device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const WakeKicks &table,
+                                              int nk, int S, double *part, int T, double *out,
+                                              int bid) {
+  extern __shared__ double rw_lds[];
+  __shared__ WakeKicks kk;
+  __shared__ double coef[PHX_MAX_KICKS];
+  __shared__ int klead[PHX_MAX_KICKS], kmask[PHX_MAX_KICKS], kslot[PHX_MAX_KICKS];
+  __shared__ double wmin[RW_WAVES], wmax[RW_WAVES], wtmax[RW_WAVES], wtmin[RW_WAVES];
+  __shared__ int wbad[RW_WAVES];
+  const int head = h->head, cap = h->capacity, nf = h->n_filled;
   const int tiles = (S + RW_TILE - 1) / RW_TILE;
-  const int tile = bid % tiles, t = bid / tiles;
-  const int nf = h->n_filled;
+  const int nc = rw_chunks(S), R = T + nc - 1;
+  const int tile = bid % tiles, row = bid / tiles;
+  RwStage st;
+  st.t = row < nc ? 0 : row - nc + 1;
+  st.row = row;
+  st.head = head;
+  if (st.t >= nf) return;  // beyond every group's memory (the whole block)
+  const int cs = (S + nc - 1) / nc;
+  const int sb = row < nc ? min(row * cs, S) : 0;
+  st.n = (row < nc ? min(sb + cs, S) : S) - sb;
+  const int slot = ring_slot(head, st.t, cap);
+  const double age = h->age[slot], beta = h->beta[slot];
+  st.age = age;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const double *tsrc = h->times + (size_t)slot * S + sb;
+  const double *mom = h->mom + (size_t)slot * 3 * S + sb;
+  int bad = 0;
+  double mn = INFINITY, mxs = -INFINITY;
+  for (int i = threadIdx.x; i < st.n; i += RW_THREADS) {
+    const double v = tsrc[i];
+    rw_lds[i] = v;
+    bad |= v != v;
+    mn = v < mn ? v : mn;
+    mxs = v > mxs ? v : mxs;
+    rw_lds[S + i] = mom[i];
+    rw_lds[2 * S + i] = mom[(size_t)S + i];
+    rw_lds[3 * S + i] = mom[(size_t)2 * S + i];
+  }
+  // dt = (t_target - age) - t_source: the age leaves the pair loop
+  double mx = -INFINITY, mnt = INFINITY;
+#pragma unroll
+  for (int q = 0; q < RW_JT; ++q) {
+    const int j = tile * RW_TILE + q * 64 + lane;
+    st.tja[q] = h->times[(size_t)head * S + (j < S ? j : S - 1)] - age;
+    bad |= st.tja[q] != st.tja[q];
+    mx = st.tja[q] > mx ? st.tja[q] : mx;
+    mnt = st.tja[q] < mnt ? st.tja[q] : mnt;
+  }
+  mn = wave_min_nan(mn, bad);
+  mxs = -wave_min_nan(-mxs, bad);
+  const double tmax = -wave_min_nan(-mx, bad);
+  const double tmin = wave_min_nan(mnt, bad);
+  const int wave_bad = __ballot(bad) != 0ULL;
+  if (lane == 0) {
+    wmin[wave] = mn;
+    wmax[wave] = mxs;
+    wtmax[wave] = tmax;
+    wtmin[wave] = tmin;
+    wbad[wave] = wave_bad;
+  }
+  // the barrier of the kick table also makes the sources and the waves'
+  // extremes the block's
+  stage_kicks(table, nk, kk);
+  // thread k: kick k's group (the first kick of its shape), the moments of
+  // the group, its moment's rank among them and its coefficient
+  if (threadIdx.x < nk) {
+    const int k = threadIdx.x;
+    const phx_wake_kick &kw = kk.k[k];
+    int lead = k;
+    unsigned mask = 0;
+    for (int m = nk - 1; m >= 0; --m) {
+      if (!same_shape(kk.k[m], kw)) continue;
+      mask |= 1u << kk.k[m].moment;
+      lead = m < lead ? m : lead;
+    }
+    klead[k] = lead;
+    kmask[k] = (int)mask;
+    kslot[k] = __popc(mask & ((1u << kw.moment) - 1u));
+    coef[k] = is_rw_dot(kw) ? (rw_prefix0(kw, beta) * sqrt(-kw.k_sigma)) * kw.sqrt_z0c : 0.0;
+  }
+  st.bad = 0;
+  st.mn = mn;
+  st.smax = mxs;
+  st.tgmax = tmax;
+  st.tgmin = tmin;
+  for (int q = 0; q < RW_WAVES; ++q) {
+    st.mn = wmin[q] < st.mn ? wmin[q] : st.mn;
+    st.smax = wmax[q] > st.smax ? wmax[q] : st.smax;
+    st.tgmax = wtmax[q] > st.tgmax ? wtmax[q] : st.tgmax;
+    st.tgmin = wtmin[q] < st.tgmin ? wtmin[q] : st.tgmin;
+    st.bad |= wbad[q];
+  }
+  lds_sync();  // the kicks' groups and coefficients
   for (int ki = 0; ki < nk; ++ki) {
     const phx_wake_kick &w = kk.k[ki];
-    if (!is_rw_dot(w) || rw_is_mfma(w.precision) != (KM == 1)) continue;
+    if (klead[ki] != ki || !is_rw_dot(w) || rw_is_mfma(w.precision) != (KM == 1)) continue;
     const int n_turns = nf < w.n_turns_wake ? nf : w.n_turns_wake;
-    if (t >= n_turns) continue;
-    bool first = true;
-    for (int k = 0; k < ki; ++k) first = first && !same_shape(kk.k[k], w);
-    if (first) wake_rw_group<KM>(h, kk, nk, S, part, T, out, ki, tile, t);
+    if (st.t >= n_turns) continue;
+    wake_rw_group<KM>(h, kk, nk, S, part, R, out, ki, tile, st, (unsigned)kmask[ki], coef, klead,
+                      kslot);
   }
 }
 
 // Static LDS of wake_conv_block; its dynamic LDS serves up to gmax kicks per
 // block in the dot-product form.
-#define WK_STATIC_LDS ((WK_GMAX * WK_THREADS + 3 * WK_GMAX) * sizeof(double))
+#define WK_STATIC_LDS ((WK_GMAX * WK_THREADS + 3 * WK_GMAX) * sizeof(double) + sizeof(WakeKicks))
 inline size_t wake_conv_lds_bytes(int S, int gmax) {
   const size_t conv = (size_t)(WK_TILE + 2 * S - 1), dot = (size_t)(1 + gmax) * S;
   return (conv > dot ? conv : dot) * sizeof(double);

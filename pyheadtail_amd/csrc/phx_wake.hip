@@ -98,7 +98,7 @@ static int check_kicks(const phx_wake_kick *kicks, int32_t n_kicks, const char *
 }
 
 size_t phx_wake_conv_ws_bytes(int32_t n_kicks, int32_t n_turns, int32_t n_slices) {
-  return (size_t)n_kicks * n_turns * n_slices * sizeof(double) + 256;
+  return (size_t)n_kicks * conv_rows(n_slices, n_turns) * n_slices * sizeof(double) + 256;
 }
 
 int phx_wake_convolve_ws(const phx_wake_hist *hist, const phx_wake_kick *kicks, int32_t n_kicks,
@@ -131,18 +131,18 @@ int phx_wake_convolve_ws(const phx_wake_hist *hist, const phx_wake_kick *kicks, 
     PHX_CHECK_LAUNCH("phx_wake_convolve");
   }
   if (rw_valu) {
-    k_wake_rw<0><<<dim3((n_slices + RW_TILE - 1) / RW_TILE * T), RW_THREADS,
+    k_wake_rw<0><<<dim3((n_slices + RW_TILE - 1) / RW_TILE * conv_rows(n_slices, T)), RW_THREADS,
                    wake_rw_lds_bytes(n_slices), s>>>(hist, kk, n_kicks, n_slices, part, T,
                                                      kicks_out);
     PHX_CHECK_LAUNCH("phx_wake_convolve: resistive wall");
   }
   if (rw_mfma) {
-    k_wake_rw<1><<<dim3((n_slices + RW_TILE - 1) / RW_TILE * T), RW_THREADS,
+    k_wake_rw<1><<<dim3((n_slices + RW_TILE - 1) / RW_TILE * conv_rows(n_slices, T)), RW_THREADS,
                    wake_rw_lds_bytes(n_slices), s>>>(hist, kk, n_kicks, n_slices, part, T,
                                                      kicks_out);
     PHX_CHECK_LAUNCH("phx_wake_convolve: resistive wall (matrix cores)");
   }
-  if (T > 1) {
+  if (conv_reduces(n_slices, T, rw)) {
     const int tot = n_kicks * n_slices;
     k_wake_conv_reduce<<<dim3((tot + 255) / 256), 256, 0, s>>>(hist, kk, n_kicks, n_slices, part,
                                                               T, kicks_out);
