@@ -142,12 +142,12 @@ void stage_blocks(const EngineArgs &a, int (&nb)[BS_COUNT]) {
   nb[BS_PASS] = a.grid;
   nb[BS_CUTS] = 1;
   nb[BS_HIST] = hist_blocks(e.n);
-  nb[BS_MERGE] = (S + merge_sl(S) - 1) / merge_sl(S);
+  const bool has_wake = S > 0 && a.n_kicks > 0;
+  nb[BS_MERGE] = (S + merge_sl(S) - 1) / merge_sl(S) + (has_wake && engine_has_rw(e) ? 1 : 0);
   nb[BS_CONV] = (ck == 0 || (ck & PHX_CONV_GENERAL)) ? (S + WK_TILE - 1) / WK_TILE * a.n_kicks * a.T : 0;
   nb[BS_RW] = (ck == 0 || (ck & PHX_CONV_RW_DOT)) ? rw_tiles * conv_rows(S, a.T) : 0;
   nb[BS_RWMF] = (ck == 0 || (ck & PHX_CONV_RW_MFMA)) ? rw_tiles * conv_rows(S, a.T) : 0;
-  const bool rw = ck == 0 || (ck & (PHX_CONV_RW_DOT | PHX_CONV_RW_MFMA));
-  nb[BS_RED] = conv_reduces(S, a.T, rw) ? (a.n_kicks * S + 255) / 256 : 0;
+  nb[BS_RED] = conv_reduces(S, a.T, engine_has_rw(e)) ? (a.n_kicks * S + 255) / 256 : 0;
   nb[BS_SNAP] = 1;
 }
 

@@ -58,11 +58,26 @@ __host__ __device__ inline EngineWs engine_ws(void *ws, int64_t n, int S) {
   return w;
 }
 
+// Whether the engine's convolution may have resistive-wall kicks in the
+// dot-product form (conv_kinds 0: every kind).
+__host__ __device__ inline bool engine_has_rw(const phx_engine &e) {
+  return e.conv_kinds == 0 || (e.conv_kinds & (PHX_CONV_RW_DOT | PHX_CONV_RW_MFMA));
+}
+
+// The resistive-wall kick description (rw_desc_build), after the partial
+// sums of the convolution.
+__host__ __device__ inline RwKick *engine_rw_desc(const EngineWs &w, int n_kicks, int S, int T) {
+  const int Se = S > 0 ? S : 1;
+  return reinterpret_cast<RwKick *>(w.conv_part +
+                                    (size_t)(n_kicks > 0 ? n_kicks : 1) * conv_rows(Se, T) * Se);
+}
+
 inline size_t engine_ws_bytes(int64_t n, int S, int n_kicks, int T) {
   const int64_t nc = phx_grid_for(n, PHX_MAX_GRID);
   const int Se = S > 0 ? S : 1;
   size_t b = round256((size_t)nc * EN_NV * 8) + 256 + acc_parts_bytes(hist_blocks(n), Se, 2);
   b += (size_t)(n_kicks > 0 ? n_kicks : 1) * conv_rows(Se, T) * Se * 8;
+  b += rw_desc_bytes(T);
   return round256(b);
 }
 
@@ -670,6 +685,9 @@ __device__ __forceinline__ void hist_body(const EngineArgs &a, int bid, int nblk
 // Merge the partial histograms; means and the new history entry:
 // times = z_centers / (beta c) (particles/slicing.py:124-126, 303-307),
 // moments n, n*mean_x, n*mean_y (impedances/wake_kicks.py:217, 235).
+// Block (S + SL - 1) / SL, launched when the engine has resistive-wall kicks
+// in the dot-product form, describes the turn's kicks for the convolution
+// (rw_desc_build) while the others merge.
 template <int SL>
 __device__ __forceinline__ void merge_body(const EngineArgs &a, int bid) {
   const phx_engine &e = a.e;
@@ -677,6 +695,13 @@ __device__ __forceinline__ void merge_body(const EngineArgs &a, int bid) {
   const int S = e.n_slices;
   SliceTotals tot;
   const int turn = e.state->turn;
+  if (bid == (S + SL - 1) / SL) {
+    if (a.n_kicks < 1 || !e.hist) return;
+    const phx_turn_program &g = e.programs[e.turn_program[turn]];
+    rw_desc_build(e.hist, *reinterpret_cast<const WakeKicks *>(g.kick), a.n_kicks, a.T,
+                  engine_rw_desc(w, a.n_kicks, S, a.T));
+    return;
+  }
   if (bid == 0 && threadIdx.x == 0) *e.turn_counter = turn + 1;
   if (!acc_merge<2, SL>(w.acc, S, tot, bid)) return;
   const int s = bid * SL + threadIdx.x % SL;
@@ -900,10 +925,9 @@ template <int KM>
 __device__ __forceinline__ void conv_rw_body(const EngineArgs &a, int bid) {
   const phx_engine &e = a.e;
   EngineWs w = engine_ws(e.ws, e.n, e.n_slices);
-  const phx_turn_program &g = e.programs[e.turn_program[e.state->turn]];
-  const WakeKicks &kk = *reinterpret_cast<const WakeKicks *>(g.kick);
   if (bid == 0 && threadIdx.x == 0) *e.pending = 1;
-  wake_rw_block<KM>(e.hist, kk, a.n_kicks, e.n_slices, w.conv_part, a.T, e.kicks, bid);
+  wake_rw_block<KM>(e.hist, engine_rw_desc(w, a.n_kicks, e.n_slices, a.T), a.n_kicks,
+                    e.n_slices, w.conv_part, a.T, e.kicks, bid);
 }
 
 __device__ __forceinline__ void conv_reduce_body(const EngineArgs &a, int bid) {

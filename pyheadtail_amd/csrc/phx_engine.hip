@@ -502,10 +502,12 @@ int enqueue_stage(const EngineArgs &a, int stage, bool has_wake, hipStream_t s) 
     else
       k_engine_hist<false><<<dim3(hist_blocks(e.n)), EN_HIST_THREADS, acc_lds_bytes(e.n_slices, 2), s>>>(a);
     PHX_CHECK_LAUNCH("phx_engine: hist");
+    // one more block describes the resistive-wall kicks for the convolution
+    const int desc = has_wake && engine_has_rw(e) ? 1 : 0;
     if (merge_sl(e.n_slices) == 4)
-      k_engine_merge<4><<<dim3((e.n_slices + 3) / 4), PHX_BLOCK, 0, s>>>(a);
+      k_engine_merge<4><<<dim3((e.n_slices + 3) / 4 + desc), PHX_BLOCK, 0, s>>>(a);
     else
-      k_engine_merge<8><<<dim3((e.n_slices + 7) / 8), PHX_BLOCK, 0, s>>>(a);
+      k_engine_merge<8><<<dim3((e.n_slices + 7) / 8 + desc), PHX_BLOCK, 0, s>>>(a);
     PHX_CHECK_LAUNCH("phx_engine: merge");
     return PHX_OK;
   }
@@ -525,8 +527,7 @@ int enqueue_stage(const EngineArgs &a, int stage, bool has_wake, hipStream_t s) 
                           wake_rw_lds_bytes(e.n_slices), s>>>(a);
     PHX_CHECK_LAUNCH("phx_engine: conv resistive wall (matrix cores)");
   }
-  const bool rw = e.conv_kinds == 0 || (e.conv_kinds & (PHX_CONV_RW_DOT | PHX_CONV_RW_MFMA));
-  if (conv_reduces(e.n_slices, a.T, rw)) {
+  if (conv_reduces(e.n_slices, a.T, engine_has_rw(e))) {
     const int tot = a.n_kicks * e.n_slices;
     k_engine_conv_reduce<<<dim3((tot + 255) / 256), 256, 0, s>>>(a);
     PHX_CHECK_LAUNCH("phx_engine: conv reduce");

@@ -651,10 +651,82 @@ __device__ __forceinline__ double wave_min_nan(double v, int &bad) {
   return v;
 }
 
-// Static LDS of wake_rw_block: the kick table, each kick's coefficient, group
-// (first kick of its shape), moment mask and sum slot, and the waves' extremes.
-#define RW_STATIC_LDS                                                       \
-  (sizeof(WakeKicks) + PHX_MAX_KICKS * (sizeof(double) + 3 * sizeof(int)) + \
+// Whether the precision runs on the matrix cores.
+__host__ __device__ __forceinline__ bool rw_is_mfma(int precision) {
+  return precision == PHX_PREC_MFMA_F32 || precision == PHX_PREC_MFMA_BF16X3 ||
+         precision == PHX_PREC_MFMA_BF16X6;
+}
+
+// Description of a launch's kicks for the resistive-wall blocks, built once
+// per convolution (rw_desc_build) ahead of them: per kick its group (the
+// first kick of its shape), the moments the group reads, the rank of its
+// moment among them, its flags, memory depth, precision, |dt_min| and wake
+// factor; then per source turn t < T (age index) and kick k the coefficient
+// pre0 sqrt(-k_sigma) sqrt(Z0 c) at that turn's beta, at coef[t * nk + k].
+// The blocks read it with one load each, instead of walking the kick table
+// for the groups and evaluating the coefficients themselves.
+#define RWD_RW 1    // resistive wall in the dot-product form
+#define RWD_MFMA 2  // on the matrix cores
+struct RwKick {
+  int lead, mask, slot, flags, n_turns, precision;
+  double dmin, wake_factor;
+};
+static_assert(sizeof(RwKick) % sizeof(uint64_t) == 0, "kick descriptions are 8-byte words");
+__host__ __device__ inline size_t rw_desc_bytes(int T) {
+  return PHX_MAX_KICKS * sizeof(RwKick) + (size_t)(T > 0 ? T : 1) * PHX_MAX_KICKS * sizeof(double);
+}
+__host__ __device__ inline double *rw_desc_coef(RwKick *d) {
+  return reinterpret_cast<double *>(d + PHX_MAX_KICKS);
+}
+__host__ __device__ inline const double *rw_desc_coef(const RwKick *d) {
+  return reinterpret_cast<const double *>(d + PHX_MAX_KICKS);
+}
+
+// Builds the description of the nk kicks `table` over T turns of history h;
+// called by every thread of one block (any size).
+__device__ __forceinline__ void rw_desc_build(const phx_wake_hist *h, const WakeKicks &table,
+                                              int nk, int T, RwKick *d) {
+  __shared__ WakeKicks kk;
+  stage_kicks(table, nk, kk);
+  if (threadIdx.x < nk) {
+    const int k = threadIdx.x;
+    const phx_wake_kick &kw = kk.k[k];
+    int lead = k;
+    unsigned mask = 0;
+    for (int m = nk - 1; m >= 0; --m) {
+      if (!same_shape(kk.k[m], kw)) continue;
+      mask |= 1u << kk.k[m].moment;
+      lead = m < lead ? m : lead;
+    }
+    RwKick r;
+    r.lead = lead;
+    r.mask = (int)mask;
+    r.slot = __popc(mask & ((1u << kw.moment) - 1u));
+    r.flags = (is_rw_dot(kw) ? RWD_RW : 0) | (rw_is_mfma(kw.precision) ? RWD_MFMA : 0);
+    r.n_turns = kw.n_turns_wake;
+    r.precision = kw.precision;
+    r.dmin = kw.dt_min_abs;
+    r.wake_factor = kw.wake_factor;
+    d[k] = r;
+  }
+  const int nf = h->n_filled, head = h->head, cap = h->capacity;
+  double *coef = rw_desc_coef(d);
+  for (int i = threadIdx.x; i < T * nk; i += blockDim.x) {
+    const int t = i / nk, k = i % nk;
+    const phx_wake_kick &kw = kk.k[k];
+    double c = 0.0;
+    if (t < nf && is_rw_dot(kw)) {
+      const double beta = h->beta[ring_slot(head, t, cap)];
+      c = (rw_prefix0(kw, beta) * sqrt(-kw.k_sigma)) * kw.sqrt_z0c;
+    }
+    coef[i] = c;
+  }
+}
+
+// Static LDS of wake_rw_block: the kick descriptions, the block's turn's
+// coefficients and the waves' extremes.
+#define RW_STATIC_LDS                                                     \
+  (PHX_MAX_KICKS * (sizeof(RwKick) + sizeof(double)) +                    \
    RW_WAVES * (4 * sizeof(double) + sizeof(int)))
 // Parts of the source range in the power sums of the past-turn expansion
 // (one thread each).
@@ -688,25 +760,19 @@ struct RwStage {
 
 // The group of kick ki (its first kick) for targets [tile * RW_TILE, +RW_TILE)
 // and the staged sources, into row st.row of R; called by all threads of the
-// block.  mask: the moments the group reads; per kick k, coef[k] (pre0
-// sqrt(-k_sigma) sqrt(Z0 c)), klead[k] (its group's first kick) and kslot[k]
-// (the rank of its moment in the group's mask).
+// block.  dk: the kick descriptions; coef[k]: kick k's coefficient at the
+// staged turn.
 // KM: the kernel's kind, 0 for the VALU precisions (FP64 with the past-turn
 // expansion, FP64 pair by pair, FP32), 1 for the matrix-core ones; each
 // kind is its own kernel, so that neither carries the other's registers.
-__host__ __device__ __forceinline__ bool rw_is_mfma(int precision) {
-  return precision == PHX_PREC_MFMA_F32 || precision == PHX_PREC_MFMA_BF16X3 ||
-         precision == PHX_PREC_MFMA_BF16X6;
-}
 
 template <int KM>
-__device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const WakeKicks &kk, int nk,
+__device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const RwKick *dk, int nk,
                                               int S, double *part, int R, double *out, int ki,
-                                              int tile, const RwStage &st, unsigned mask,
-                                              const double *coef, const int *klead,
-                                              const int *kslot) {
+                                              int tile, const RwStage &st, const double *coef) {
   extern __shared__ double rw_lds[];
-  const phx_wake_kick &w = kk.k[ki];
+  const RwKick &w = dk[ki];
+  const unsigned mask = (unsigned)w.mask;
   const int nu = __popc(mask);
   const int mo0 = __ffs(mask) - 1;
   const int mo1 = nu > 1 ? __ffs(mask & ~(1u << mo0)) - 1 : mo0;
@@ -720,8 +786,8 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const Wake
                *m2 = rw_lds + (size_t)(1 + mo2) * S;
   double *red = rw_lds + (size_t)4 * S;
   // the extremes of the tile's targets: every wave holds all of them
-  const bool uniform = !st.bad && (st.tgmax - st.mn) + w.dt_min_abs < 0.0;
-  const double dmin = w.dt_min_abs;
+  const bool uniform = !st.bad && (st.tgmax - st.mn) + w.dmin < 0.0;
+  const double dmin = w.dmin;
   double tot[3] = {0.0, 0.0, 0.0};
   // the past-turn expansion (FP64 precision): centres, their distance A and
   // the largest relative offset r = (max|u| + max|v|) / A
@@ -844,11 +910,11 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const Wake
   const int j = tile * RW_TILE + threadIdx.x;
   if (threadIdx.x < RW_TILE && j < S) {
     for (int k = ki; k < nk; ++k) {
-      if (klead[k] != ki) continue;
-      const int u = kslot[k];
+      if (dk[k].lead != ki) continue;
+      const int u = dk[k].slot;
       const double sum = u == 0 ? tot[0] : (u == 1 ? tot[1] : tot[2]);
       const double v = coef[k] * sum;
-      if (R == 1) out[(size_t)k * S + j] = kk.k[k].wake_factor * (0.0 + v);
+      if (R == 1) out[(size_t)k * S + j] = dk[k].wake_factor * (0.0 + v);
       else part[((size_t)k * R + st.row) * S + j] = v;
     }
   }
@@ -863,14 +929,14 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const Wake
 // not depend on the kick table (the ring's state, then the sources, the
 // targets, the slot's age and beta) are issued before it is read, each
 // dependent load being a round trip to memory that the next one waits for.
+// desc: the kicks' description (rw_desc_build).
 template <int KM>
-__device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const WakeKicks &table,
+__device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const RwKick *desc,
                                               int nk, int S, double *part, int T, double *out,
                                               int bid) {
   extern __shared__ double rw_lds[];
-  __shared__ WakeKicks kk;
+  __shared__ RwKick dk[PHX_MAX_KICKS];
   __shared__ double coef[PHX_MAX_KICKS];
-  __shared__ int klead[PHX_MAX_KICKS], kmask[PHX_MAX_KICKS], kslot[PHX_MAX_KICKS];
   __shared__ double wmin[RW_WAVES], wmax[RW_WAVES], wtmax[RW_WAVES], wtmin[RW_WAVES];
   __shared__ int wbad[RW_WAVES];
   const int head = h->head, cap = h->capacity, nf = h->n_filled;
@@ -886,7 +952,7 @@ __device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const Wake
   const int sb = row < nc ? min(row * cs, S) : 0;
   st.n = (row < nc ? min(sb + cs, S) : S) - sb;
   const int slot = ring_slot(head, st.t, cap);
-  const double age = h->age[slot], beta = h->beta[slot];
+  const double age = h->age[slot];
   st.age = age;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -926,26 +992,14 @@ __device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const Wake
     wtmin[wave] = tmin;
     wbad[wave] = wave_bad;
   }
-  // the barrier of the kick table also makes the sources and the waves'
-  // extremes the block's
-  stage_kicks(table, nk, kk);
-  // thread k: kick k's group (the first kick of its shape), the moments of
-  // the group, its moment's rank among them and its coefficient
-  if (threadIdx.x < nk) {
-    const int k = threadIdx.x;
-    const phx_wake_kick &kw = kk.k[k];
-    int lead = k;
-    unsigned mask = 0;
-    for (int m = nk - 1; m >= 0; --m) {
-      if (!same_shape(kk.k[m], kw)) continue;
-      mask |= 1u << kk.k[m].moment;
-      lead = m < lead ? m : lead;
-    }
-    klead[k] = lead;
-    kmask[k] = (int)mask;
-    kslot[k] = __popc(mask & ((1u << kw.moment) - 1u));
-    coef[k] = is_rw_dot(kw) ? (rw_prefix0(kw, beta) * sqrt(-kw.k_sigma)) * kw.sqrt_z0c : 0.0;
+  {  // the kick descriptions and the turn's coefficients
+    constexpr int words = (int)(sizeof(RwKick) / sizeof(uint64_t));
+    const uint64_t *src = reinterpret_cast<const uint64_t *>(desc);
+    uint64_t *dst = reinterpret_cast<uint64_t *>(dk);
+    for (int i = threadIdx.x; i < nk * words; i += RW_THREADS) dst[i] = src[i];
+    if (threadIdx.x < nk) coef[threadIdx.x] = rw_desc_coef(desc)[st.t * nk + threadIdx.x];
   }
+  lds_sync();  // the sources, the waves' extremes and the kick descriptions
   st.bad = 0;
   st.mn = mn;
   st.smax = mxs;
@@ -958,14 +1012,12 @@ __device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const Wake
     st.tgmin = wtmin[q] < st.tgmin ? wtmin[q] : st.tgmin;
     st.bad |= wbad[q];
   }
-  lds_sync();  // the kicks' groups and coefficients
   for (int ki = 0; ki < nk; ++ki) {
-    const phx_wake_kick &w = kk.k[ki];
-    if (klead[ki] != ki || !is_rw_dot(w) || rw_is_mfma(w.precision) != (KM == 1)) continue;
-    const int n_turns = nf < w.n_turns_wake ? nf : w.n_turns_wake;
+    const RwKick &w = dk[ki];
+    if (w.lead != ki || !(w.flags & RWD_RW) || ((w.flags & RWD_MFMA) != 0) != (KM == 1)) continue;
+    const int n_turns = nf < w.n_turns ? nf : w.n_turns;
     if (st.t >= n_turns) continue;
-    wake_rw_group<KM>(h, kk, nk, S, part, R, out, ki, tile, st, (unsigned)kmask[ki], coef, klead,
-                      kslot);
+    wake_rw_group<KM>(h, dk, nk, S, part, R, out, ki, tile, st, coef);
   }
 }
 

@@ -18,11 +18,15 @@ __global__ void __launch_bounds__(WK_THREADS)
   wake_conv_block(h, kk, S, part, T, out, gmax, blockIdx.x, blockIdx.y, blockIdx.z, gridDim.y);
 }
 
+__global__ void k_wake_rw_desc(const phx_wake_hist *h, WakeKicks kk, int nk, int T, RwKick *desc) {
+  rw_desc_build(h, kk, nk, T, desc);
+}
+
 template <int KM>
 __global__ void __launch_bounds__(RW_THREADS)
-    k_wake_rw(const phx_wake_hist *h, WakeKicks kk, int nk, int S, double *part, int T,
+    k_wake_rw(const phx_wake_hist *h, const RwKick *desc, int nk, int S, double *part, int T,
               double *out) {
-  wake_rw_block<KM>(h, kk, nk, S, part, T, out, blockIdx.x);
+  wake_rw_block<KM>(h, desc, nk, S, part, T, out, blockIdx.x);
 }
 
 __global__ void k_wake_conv_reduce(const phx_wake_hist *h, WakeKicks kk, int n_kicks, int S,
@@ -97,8 +101,10 @@ static int check_kicks(const phx_wake_kick *kicks, int32_t n_kicks, const char *
   return PHX_OK;
 }
 
+// The partial sums, then the resistive-wall kick description.
 size_t phx_wake_conv_ws_bytes(int32_t n_kicks, int32_t n_turns, int32_t n_slices) {
-  return (size_t)n_kicks * conv_rows(n_slices, n_turns) * n_slices * sizeof(double) + 256;
+  return (size_t)n_kicks * conv_rows(n_slices, n_turns) * n_slices * sizeof(double) +
+         rw_desc_bytes(n_turns) + 256;
 }
 
 int phx_wake_convolve_ws(const phx_wake_hist *hist, const phx_wake_kick *kicks, int32_t n_kicks,
@@ -130,15 +136,20 @@ int phx_wake_convolve_ws(const phx_wake_hist *hist, const phx_wake_kick *kicks, 
         hist, kk, n_slices, part, T, kicks_out, gmax);
     PHX_CHECK_LAUNCH("phx_wake_convolve");
   }
+  RwKick *desc = reinterpret_cast<RwKick *>(part + (size_t)n_kicks * conv_rows(n_slices, T) * n_slices);
+  if (rw) {
+    k_wake_rw_desc<<<1, 256, 0, s>>>(hist, kk, n_kicks, T, desc);
+    PHX_CHECK_LAUNCH("phx_wake_convolve: resistive-wall kicks");
+  }
   if (rw_valu) {
     k_wake_rw<0><<<dim3((n_slices + RW_TILE - 1) / RW_TILE * conv_rows(n_slices, T)), RW_THREADS,
-                   wake_rw_lds_bytes(n_slices), s>>>(hist, kk, n_kicks, n_slices, part, T,
+                   wake_rw_lds_bytes(n_slices), s>>>(hist, desc, n_kicks, n_slices, part, T,
                                                      kicks_out);
     PHX_CHECK_LAUNCH("phx_wake_convolve: resistive wall");
   }
   if (rw_mfma) {
     k_wake_rw<1><<<dim3((n_slices + RW_TILE - 1) / RW_TILE * conv_rows(n_slices, T)), RW_THREADS,
-                   wake_rw_lds_bytes(n_slices), s>>>(hist, kk, n_kicks, n_slices, part, T,
+                   wake_rw_lds_bytes(n_slices), s>>>(hist, desc, n_kicks, n_slices, part, T,
                                                      kicks_out);
     PHX_CHECK_LAUNCH("phx_wake_convolve: resistive wall (matrix cores)");
   }
