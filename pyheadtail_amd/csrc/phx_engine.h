@@ -578,6 +578,23 @@ __device__ __forceinline__ void cuts_body(const EngineArgs &a, double *res,
   if (FOLD_ALL && threadIdx.x < 6) w.p0[threadIdx.x] = e.coords[threadIdx.x][0];
   fold_partials<L::MINM, L::MAXM>(w.partials, a.grid, 0, ZB_NV, res);
   if constexpr (REC && FOLD_ALL) fold_partials<L::MINM, L::MAXM>(w.partials, a.grid, ZB_NV, L::NV, res);
+  // the wake history: every stored entry ages by the turn's increment, one
+  // thread per entry (a serial loop made the block wait on one round trip
+  // per entry); the slot the new entry takes when the ring is full is
+  // skipped, since thread 0 then sets it to age 0
+  phx_wake_hist *h = e.hist;
+  int cap = 1, nf = 0, nh = 0;
+  if (FOLD_ALL) {
+    const int head = h->head;
+    cap = h->capacity;
+    nf = h->n_filled;
+    nh = (head + 1) % cap;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+      const int sl = ring_slot(head, i, cap);
+      if (sl != nh) h->age[sl] = h->age[sl] + g.age_increment;
+    }
+    lds_sync();  // every thread has read the ring's state before thread 0 moves it
+  }
   if (threadIdx.x == 0) {
     phx_slice_state st;
     compute_cuts(g.slicer, e.n, res[0], res[1], res[2], res[3], res[4], st);
@@ -587,13 +604,6 @@ __device__ __forceinline__ void cuts_body(const EngineArgs &a, double *res,
     *stp = st;
     if (FOLD_ALL) {
       *e.state = st;
-      phx_wake_hist *h = e.hist;
-      const int cap = h->capacity, nf = h->n_filled, head = h->head;
-      for (int i = 0; i < nf; ++i) {
-        const int sl = ring_slot(head, i, cap);
-        h->age[sl] = h->age[sl] + g.age_increment;
-      }
-      const int nh = (head + 1) % cap;
       h->head = nh;
       h->n_filled = nf + 1 < cap ? nf + 1 : cap;
       h->age[nh] = 0.0;

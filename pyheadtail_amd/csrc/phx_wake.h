@@ -171,23 +171,22 @@ __device__ __forceinline__ void hist_push_device(phx_wake_hist *h, const double 
                                                  const int32_t *counts, const double *mean_x,
                                                  const double *mean_y, double age_inc,
                                                  double beta) {
-  __shared__ int sh_head;
   const int cap = h->capacity, S = h->n_slices;
+  const int nf = h->n_filled, head = h->head;
+  const int nh = (head + 1) % cap;
+  // one thread per stored entry; the slot of the new entry (ring full) is
+  // set to age 0 below
+  for (int i = threadIdx.x; i < nf; i += blockDim.x) {
+    const int sl = ring_slot(head, i, cap);
+    if (sl != nh) h->age[sl] = h->age[sl] + age_inc;
+  }
+  lds_sync();  // every thread has read the ring's state before thread 0 moves it
   if (threadIdx.x == 0) {
-    const int nf = h->n_filled, head = h->head;
-    for (int i = 0; i < nf; ++i) {
-      const int sl = ring_slot(head, i, cap);
-      h->age[sl] = h->age[sl] + age_inc;
-    }
-    const int nh = (head + 1) % cap;
     h->head = nh;
     h->n_filled = nf + 1 < cap ? nf + 1 : cap;
     h->age[nh] = 0.0;
     h->beta[nh] = beta;
-    sh_head = nh;
   }
-  lds_sync();
-  const int nh = sh_head;
   double *tt = h->times + (size_t)nh * S;
   double *mm = h->mom + (size_t)nh * 3 * S;
   for (int s = threadIdx.x; s < S; s += blockDim.x) {
