@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(WK_THREADS) k_batch_conv(BatchArgs ba) {
 }
 
 template <int KM>
-__global__ void __launch_bounds__(RW_THREADS) k_batch_conv_rw(BatchArgs ba) {
+__global__ void __launch_bounds__(rw_threads<KM>()) k_batch_conv_rw(BatchArgs ba) {
   int loc;
   const int b = batch_find(ba, KM == 0 ? BS_RW : BS_RWMF, blockIdx.x, loc);
   conv_rw_body<KM>(ba.args[b], loc);
@@ -208,11 +208,11 @@ int batch_turn(const Batch &B, hipStream_t s) {
     PHX_CHECK_LAUNCH("phx_batch: convolution");
   }
   if (B.total[BS_RW]) {
-    k_batch_conv_rw<0><<<dim3(B.total[BS_RW]), RW_THREADS, B.lds_rw, s>>>(ba);
+    k_batch_conv_rw<0><<<dim3(B.total[BS_RW]), rw_threads<0>(), B.lds_rw, s>>>(ba);
     PHX_CHECK_LAUNCH("phx_batch: resistive wall");
   }
   if (B.total[BS_RWMF]) {
-    k_batch_conv_rw<1><<<dim3(B.total[BS_RWMF]), RW_THREADS, B.lds_rw, s>>>(ba);
+    k_batch_conv_rw<1><<<dim3(B.total[BS_RWMF]), rw_threads<1>(), B.lds_rw, s>>>(ba);
     PHX_CHECK_LAUNCH("phx_batch: resistive wall (matrix cores)");
   }
   if (B.total[BS_RED]) {

@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(WK_THREADS) k_engine_conv(EngineArgs a) {
 }
 
 template <int KM>
-__global__ void __launch_bounds__(RW_THREADS) k_engine_conv_rw(EngineArgs a) {
+__global__ void __launch_bounds__(rw_threads<KM>()) k_engine_conv_rw(EngineArgs a) {
   conv_rw_body<KM>(a, blockIdx.x);
 }
 
@@ -518,12 +518,12 @@ int enqueue_stage(const EngineArgs &a, int stage, bool has_wake, hipStream_t s) 
     PHX_CHECK_LAUNCH("phx_engine: conv");
   }
   if (e.conv_kinds == 0 || (e.conv_kinds & PHX_CONV_RW_DOT)) {
-    k_engine_conv_rw<0><<<dim3((e.n_slices + RW_TILE - 1) / RW_TILE * conv_rows(e.n_slices, a.T)), RW_THREADS,
+    k_engine_conv_rw<0><<<dim3((e.n_slices + RW_TILE - 1) / RW_TILE * conv_rows(e.n_slices, a.T)), rw_threads<0>(),
                           wake_rw_lds_bytes(e.n_slices), s>>>(a);
     PHX_CHECK_LAUNCH("phx_engine: conv resistive wall");
   }
   if (e.conv_kinds == 0 || (e.conv_kinds & PHX_CONV_RW_MFMA)) {
-    k_engine_conv_rw<1><<<dim3((e.n_slices + RW_TILE - 1) / RW_TILE * conv_rows(e.n_slices, a.T)), RW_THREADS,
+    k_engine_conv_rw<1><<<dim3((e.n_slices + RW_TILE - 1) / RW_TILE * conv_rows(e.n_slices, a.T)), rw_threads<1>(),
                           wake_rw_lds_bytes(e.n_slices), s>>>(a);
     PHX_CHECK_LAUNCH("phx_engine: conv resistive wall (matrix cores)");
   }

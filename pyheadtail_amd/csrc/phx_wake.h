@@ -248,7 +248,7 @@ __device__ __host__ __forceinline__ bool is_rw_dot(const phx_wake_kick &k) {
 // general block writes the kick itself).  The reduction adds a kick's rows in
 // order; a resistive-wall kick with a single row (S <= RW_CHUNK, T = 1) is
 // written by its block.
-#define RW_CHUNK 128
+#define RW_CHUNK 64
 #define RW_CHUNK_MAX 16
 __host__ __device__ inline int rw_chunks(int S) {
   const int c = (S + RW_CHUNK - 1) / RW_CHUNK;
@@ -396,8 +396,20 @@ __device__ __forceinline__ void wake_conv_reduce(const phx_wake_hist *h, const W
 // of turns gone by, decided from the extreme times: rounding is monotonic)
 // skip the class selection.  The sums run in a fixed order: each wave in
 // source order, then the waves in order.
-#define RW_WAVES 8
+#define RW_WAVES 8  // the most waves per block (the matrix-core kernel's)
 #define RW_THREADS (RW_WAVES * 64)
+// Waves per block of each kernel kind (KM, see wake_rw_group): the VALU
+// kernel runs 4-wave blocks, so that three fit a compute unit (LDS) and every
+// block of a 1000-slice, 64-turn convolution is resident at once; the
+// matrix-core kernel's tile mapping takes 8 waves of 16 targets.
+template <int KM>
+__host__ __device__ constexpr int rw_waves() {
+  return KM == 0 ? 4 : 8;
+}
+template <int KM>
+__host__ __device__ constexpr int rw_threads() {
+  return rw_waves<KM>() * 64;
+}
 #define RW_JT 2
 #define RW_TILE (RW_JT * 64)
 
@@ -883,7 +895,8 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const RwKi
     if (threadIdx.x < RW_TILE)
       for (int u = 0; u < nu; ++u) tot[u] = red[u * RW_TILE + threadIdx.x];
   } else {
-    const int chunk = (n + RW_WAVES - 1) / RW_WAVES;
+    constexpr int WV = rw_waves<KM>();
+    const int chunk = (n + WV - 1) / WV;
     const int ib = min(wave * chunk, n), ie = min(ib + chunk, n);
     double acc[3][RW_JT] = {};
     if (w.precision == PHX_PREC_FP32)
@@ -901,7 +914,7 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const RwKi
       lds_sync();
       if (threadIdx.x < RW_TILE) {
         double r = red[threadIdx.x];
-        for (int q = 1; q < RW_WAVES; ++q) r = r + red[q * RW_TILE + threadIdx.x];
+        for (int q = 1; q < WV; ++q) r = r + red[q * RW_TILE + threadIdx.x];
         tot[u] = r;
       }
     }
@@ -959,7 +972,7 @@ __device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const RwKi
   const double *mom = h->mom + (size_t)slot * 3 * S + sb;
   int bad = 0;
   double mn = INFINITY, mxs = -INFINITY;
-  for (int i = threadIdx.x; i < st.n; i += RW_THREADS) {
+  for (int i = threadIdx.x; i < st.n; i += rw_threads<KM>()) {
     const double v = tsrc[i];
     rw_lds[i] = v;
     bad |= v != v;
@@ -995,7 +1008,7 @@ __device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const RwKi
     constexpr int words = (int)(sizeof(RwKick) / sizeof(uint64_t));
     const uint64_t *src = reinterpret_cast<const uint64_t *>(desc);
     uint64_t *dst = reinterpret_cast<uint64_t *>(dk);
-    for (int i = threadIdx.x; i < nk * words; i += RW_THREADS) dst[i] = src[i];
+    for (int i = threadIdx.x; i < nk * words; i += rw_threads<KM>()) dst[i] = src[i];
     if (threadIdx.x < nk) coef[threadIdx.x] = rw_desc_coef(desc)[st.t * nk + threadIdx.x];
   }
   lds_sync();  // the sources, the waves' extremes and the kick descriptions
@@ -1004,7 +1017,7 @@ __device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const RwKi
   st.smax = mxs;
   st.tgmax = tmax;
   st.tgmin = tmin;
-  for (int q = 0; q < RW_WAVES; ++q) {
+  for (int q = 0; q < rw_waves<KM>(); ++q) {
     st.mn = wmin[q] < st.mn ? wmin[q] : st.mn;
     st.smax = wmax[q] > st.smax ? wmax[q] : st.smax;
     st.tgmax = wtmax[q] > st.tgmax ? wtmax[q] : st.tgmax;

@@ -23,7 +23,7 @@ __global__ void k_wake_rw_desc(const phx_wake_hist *h, WakeKicks kk, int nk, int
 }
 
 template <int KM>
-__global__ void __launch_bounds__(RW_THREADS)
+__global__ void __launch_bounds__(rw_threads<KM>())
     k_wake_rw(const phx_wake_hist *h, const RwKick *desc, int nk, int S, double *part, int T,
               double *out) {
   wake_rw_block<KM>(h, desc, nk, S, part, T, out, blockIdx.x);
@@ -142,13 +142,13 @@ int phx_wake_convolve_ws(const phx_wake_hist *hist, const phx_wake_kick *kicks, 
     PHX_CHECK_LAUNCH("phx_wake_convolve: resistive-wall kicks");
   }
   if (rw_valu) {
-    k_wake_rw<0><<<dim3((n_slices + RW_TILE - 1) / RW_TILE * conv_rows(n_slices, T)), RW_THREADS,
+    k_wake_rw<0><<<dim3((n_slices + RW_TILE - 1) / RW_TILE * conv_rows(n_slices, T)), rw_threads<0>(),
                    wake_rw_lds_bytes(n_slices), s>>>(hist, desc, n_kicks, n_slices, part, T,
                                                      kicks_out);
     PHX_CHECK_LAUNCH("phx_wake_convolve: resistive wall");
   }
   if (rw_mfma) {
-    k_wake_rw<1><<<dim3((n_slices + RW_TILE - 1) / RW_TILE * conv_rows(n_slices, T)), RW_THREADS,
+    k_wake_rw<1><<<dim3((n_slices + RW_TILE - 1) / RW_TILE * conv_rows(n_slices, T)), rw_threads<1>(),
                    wake_rw_lds_bytes(n_slices), s>>>(hist, desc, n_kicks, n_slices, part, T,
                                                      kicks_out);
     PHX_CHECK_LAUNCH("phx_wake_convolve: resistive wall (matrix cores)");
