@@ -389,15 +389,15 @@ __device__ __forceinline__ void wake_conv_reduce(const phx_wake_hist *h, const W
 // coef and in the source moment (n, n x, n y), so a block sums
 //   acc_u[j] = sum_i m_u[i] * sel(dt_ij) * rsqrt(x_ij)
 // once per distinct moment u and writes coef_k * acc_u(k) for every kick k.
-// One block = (RW_TILE targets, source turn), for each shape group: the block stages
-// the turn's source times and moments in LDS, each wave runs over its share
+// One block = (RW_TILE targets, row of sources: a chunk of the current turn
+// or one older turn, conv_rows): the block stages the row's source times and
+// moments in LDS once for every shape group, each wave runs over its share
 // of the sources (the same source in all lanes: LDS broadcasts) for two
 // targets per lane.  Blocks whose every pair has dt < -|dt_min| (all sources
 // of turns gone by, decided from the extreme times: rounding is monotonic)
 // skip the class selection.  The sums run in a fixed order: each wave in
 // source order, then the waves in order.
 #define RW_WAVES 8  // the most waves per block (the matrix-core kernel's)
-#define RW_THREADS (RW_WAVES * 64)
 // Waves per block of each kernel kind (KM, see wake_rw_group): the VALU
 // kernel runs 4-wave blocks, so that three fit a compute unit (LDS) and every
 // block of a 1000-slice, 64-turn convolution is resident at once; the
@@ -937,11 +937,11 @@ __device__ __forceinline__ void wake_rw_group(const phx_wake_hist *h, const RwKi
 // (conv_rows(S, T) rows: the chunks of turn 0, then turns 1..T-1), so that
 // consecutive blocks (which go to different XCDs) cover different rows.  The
 // block stages the row's sources once and serves every resistive-wall group
-// (in kick order) whose memory reaches the row's turn.  The loads that do
-// not depend on the kick table (the ring's state, then the sources, the
-// targets, the slot's age and beta) are issued before it is read, each
-// dependent load being a round trip to memory that the next one waits for.
-// desc: the kicks' description (rw_desc_build).
+// (in kick order) whose memory reaches the row's turn.  The ring's loads
+// (its state, then the row's sources, the targets and the slot's age) are
+// issued before the kick description is read, each dependent load being a
+// round trip to memory that the next one waits for.  desc: the kicks'
+// description (rw_desc_build).
 template <int KM>
 __device__ __forceinline__ void wake_rw_block(const phx_wake_hist *h, const RwKick *desc,
                                               int nk, int S, double *part, int T, double *out,
