@@ -147,7 +147,7 @@ void stage_blocks(const EngineArgs &a, int (&nb)[BS_COUNT]) {
   nb[BS_CONV] = (ck == 0 || (ck & PHX_CONV_GENERAL)) ? (S + WK_TILE - 1) / WK_TILE * a.n_kicks * a.T : 0;
   nb[BS_RW] = (ck == 0 || (ck & PHX_CONV_RW_DOT)) ? rw_tiles * conv_rows(S, a.T) : 0;
   nb[BS_RWMF] = (ck == 0 || (ck & PHX_CONV_RW_MFMA)) ? rw_tiles * conv_rows(S, a.T) : 0;
-  nb[BS_RED] = conv_reduces(S, a.T, engine_has_rw(e)) ? (a.n_kicks * S + 255) / 256 : 0;
+  nb[BS_RED] = conv_reduces(S, a.T, engine_has_rw(e)) ? (a.n_kicks * S * RED_LANES + 255) / 256 : 0;
   nb[BS_SNAP] = 1;
 }
 

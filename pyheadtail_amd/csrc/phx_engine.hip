@@ -529,7 +529,7 @@ int enqueue_stage(const EngineArgs &a, int stage, bool has_wake, hipStream_t s) 
   }
   if (conv_reduces(e.n_slices, a.T, engine_has_rw(e))) {
     const int tot = a.n_kicks * e.n_slices;
-    k_engine_conv_reduce<<<dim3((tot + 255) / 256), 256, 0, s>>>(a);
+    k_engine_conv_reduce<<<dim3((tot * RED_LANES + 255) / 256), 256, 0, s>>>(a);
     PHX_CHECK_LAUNCH("phx_engine: conv reduce");
   }
   return PHX_OK;

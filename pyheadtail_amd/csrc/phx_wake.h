@@ -353,30 +353,48 @@ __device__ __forceinline__ void wake_conv_block(const phx_wake_hist *h, const Wa
   }
 }
 
-// Sum of a kick's rows of partial sums in order, times the wake factor.
+// Sum of a kick's rows of partial sums, times the wake factor: RED_LANES
+// consecutive lanes per (kick, target) each sum a contiguous part of the rows
+// in order, and the first adds the parts in order.  Threads:
+// RED_LANES * n_kicks * S (blocks of a multiple of 64 threads).
+#define RED_LANES 4
 __device__ __forceinline__ void wake_conv_reduce(const phx_wake_hist *h, const WakeKicks &kk,
                                                  int n_kicks, int S, const double *part, int T,
                                                  double *out, int bid) {
-  const int idx = bid * blockDim.x + threadIdx.x;
-  if (idx >= n_kicks * S) return;
-  const int ki = idx / S, j = idx % S;
-  const phx_wake_kick &w = kk.k[ki];
-  const int nf = h->n_filled;
-  if (T == 1 && !is_rw_dot(w)) return;  // written by its block
-  const int n_turns = nf < w.n_turns_wake ? nf : w.n_turns_wake;
-  const int rows = is_rw_dot(w) && n_turns > 0 ? n_turns + rw_chunks(S) - 1 : n_turns;
-  const double *p = part + (size_t)ki * conv_rows(S, T) * S + j;
-  double acc = 0.0;
-  int q = 0;
-  for (; q + 8 <= rows; q += 8) {
-    double v[8];
+  const int gidx = bid * blockDim.x + threadIdx.x;
+  const int idx = gidx / RED_LANES, sub = gidx % RED_LANES;
+  double acc = 0.0, wf = 0.0;
+  bool write = false;
+  int ki = 0, j = 0;
+  if (idx < n_kicks * S) {  // no return before the lane exchange below
+    ki = idx / S;
+    j = idx % S;
+    const phx_wake_kick &w = kk.k[ki];
+    if (!(T == 1 && !is_rw_dot(w))) {  // otherwise written by its block
+      const int nf = h->n_filled;
+      const int n_turns = nf < w.n_turns_wake ? nf : w.n_turns_wake;
+      const int rows = is_rw_dot(w) && n_turns > 0 ? n_turns + rw_chunks(S) - 1 : n_turns;
+      const int len = (rows + RED_LANES - 1) / RED_LANES;
+      const int r0 = min(sub * len, rows), r1 = min(r0 + len, rows);
+      const double *p = part + (size_t)ki * conv_rows(S, T) * S + j;
+      int q = r0;
+      for (; q + 8 <= r1; q += 8) {
+        double v[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = p[(size_t)(q + r) * S];
+        for (int r = 0; r < 8; ++r) v[r] = p[(size_t)(q + r) * S];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) acc = acc + v[r];
+        for (int r = 0; r < 8; ++r) acc = acc + v[r];
+      }
+      for (; q < r1; ++q) acc = acc + p[(size_t)q * S];
+      write = true;
+      wf = w.wake_factor;
+    }
   }
-  for (; q < rows; ++q) acc = acc + p[(size_t)q * S];
-  out[(size_t)ki * S + j] = w.wake_factor * acc;
+  const int lane = threadIdx.x & 63;
+  double tot = acc;
+#pragma unroll
+  for (int r = 1; r < RED_LANES; ++r) tot = tot + __shfl(acc, lane + r);
+  if (write && sub == 0) out[(size_t)ki * S + j] = wf * tot;
 }
 
 // ---------------------------------------------------------------------------

@@ -155,7 +155,7 @@ int phx_wake_convolve_ws(const phx_wake_hist *hist, const phx_wake_kick *kicks, 
   }
   if (conv_reduces(n_slices, T, rw)) {
     const int tot = n_kicks * n_slices;
-    k_wake_conv_reduce<<<dim3((tot + 255) / 256), 256, 0, s>>>(hist, kk, n_kicks, n_slices, part,
+    k_wake_conv_reduce<<<dim3((tot * RED_LANES + 255) / 256), 256, 0, s>>>(hist, kk, n_kicks, n_slices, part,
                                                               T, kicks_out);
     PHX_CHECK_LAUNCH("phx_wake_convolve: reduce");
   }
